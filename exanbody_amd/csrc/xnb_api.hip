@@ -1,0 +1,957 @@
+// xnb_api.hip -- C ABI (include/xnb_amd.h) over the gfx950 kernels: one context per box,
+// owning its device arrays and one HIP stream.  Host code orchestrates the reference
+// operator sequence; every particle operation runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/xnb_amd.h"
+#include "xnb_kernels.h"
+
+using namespace xnb;
+
+namespace {
+
+template <class T>
+struct DArr {
+  T* p = nullptr;
+  size_t n = 0;
+};
+
+struct TimerRec {
+  int id;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct xnb_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  Geom g{};
+  xnb_domain dom{};
+  bool nbh_set = false;
+  double nbh_dist_lab = 0., max_displ = 0., ghost_dist = 0.;
+
+  uint32_t N = 0, G = 0;
+  size_t cap = 0;  // particle slots allocated in each buffer
+  PState A{}, B{};
+  DArr<uint32_t> slot_cell_A, slot_cell_B;
+  DArr<double4> rA, rB;
+  DArr<double> vA[3], vB[3], fA[3], fB[3], eA, eB;
+  DArr<uint64_t> idA, idB;
+
+  // per cell
+  DArr<uint32_t> cell_off, cell_n, cnt_out, cnt_in, new_n, new_off, in_cur, gcnt, goff, clen, coff;
+  // per slot scratch
+  DArr<uint32_t> mv_dst, pk, newloc, in_list, plen, pstart;
+  DArr<uint32_t> ghost_src, ghost_shift, backup;
+  // neighbour lists
+  DArr<uint32_t> nbr, nbr_cnt;
+  uint32_t nbr_cap = 128;
+  uint32_t list_atoms = 0;
+  uint32_t max_neighbors = 0;
+  xnb_chunk_neighbors_config list_cfg{1, 0, 0, 0, 0};
+  bool lists_valid = false, ghosts_valid = false, backup_valid = false;
+  DArr<uint16_t> stream_buf;
+
+  // scan / scalar scratch
+  DArr<uint32_t> bsum, u32s;            // u32s: [0] err, [1] scan total, [2] max count
+  DArr<unsigned long long> u64s;        // [0] displacement count, [1..2] pair counts
+  DArr<unsigned long long> step_counts; // per-step displacement counts (interval mode)
+  DArr<double> red_part, red_out;
+  uint32_t* h_u32 = nullptr;            // pinned
+  unsigned long long* h_u64 = nullptr;  // pinned
+
+  // timers
+  bool timing = false;
+  std::vector<TimerRec> pending;
+  std::vector<hipEvent_t> pool;
+  double t_ms[XNB_TIMER_COUNT] = {};
+  int64_t t_n[XNB_TIMER_COUNT] = {};
+  int t_cur = -1;
+  hipEvent_t t_a = nullptr;
+
+  double last_rcut = 0.;
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(xnb_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                            \
+  do {                                                                                               \
+    hipError_t _e = (expr);                                                                          \
+    if (_e != hipSuccess)                                                                            \
+      return fail((ctx), XNB_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));              \
+  } while (0)
+
+template <class T>
+int dalloc(xnb_ctx* c, DArr<T>& a, size_t n) {
+  if (a.n >= n && a.p) return XNB_OK;
+  if (a.p) hipFree(a.p);
+  a.p = nullptr;
+  a.n = 0;
+  size_t nn = n ? n : 1;
+  hipError_t e = hipMalloc((void**)&a.p, nn * sizeof(T));
+  if (e != hipSuccess) {
+    a.p = nullptr;
+    return fail(c, XNB_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  }
+  a.n = nn;
+  return XNB_OK;
+}
+
+template <class T>
+void dfree(DArr<T>& a) {
+  if (a.p) hipFree(a.p);
+  a.p = nullptr;
+  a.n = 0;
+}
+
+inline uint32_t nb256(size_t n) { return (uint32_t)((n + 255) / 256); }
+
+// ---- timers
+hipEvent_t get_event(xnb_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  hipEventCreate(&e);
+  return e;
+}
+
+void flush_timers(xnb_ctx* c) {
+  if (c->pending.empty()) return;
+  hipStreamSynchronize(c->st);
+  for (auto& t : c->pending) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, t.a, t.b);
+    c->t_ms[t.id] += ms;
+    c->t_n[t.id] += 1;
+    c->pool.push_back(t.a);
+    c->pool.push_back(t.b);
+  }
+  c->pending.clear();
+}
+
+void tstart(xnb_ctx* c, int id) {
+  if (!c->timing) return;
+  c->t_cur = id;
+  c->t_a = get_event(c);
+  hipEventRecord(c->t_a, c->st);
+}
+
+void tstop(xnb_ctx* c) {
+  if (!c->timing || c->t_cur < 0) return;
+  hipEvent_t b = get_event(c);
+  hipEventRecord(b, c->st);
+  c->pending.push_back({c->t_cur, c->t_a, b});
+  c->t_cur = -1;
+  if (c->pending.size() >= 2048) flush_timers(c);
+}
+
+// ---- geometry
+int set_geometry(xnb_ctx* c, double ghost_dist) {
+  const xnb_domain& d = c->dom;
+  Geom g{};
+  g.Di = (int)d.grid_dims[0];
+  g.Dj = (int)d.grid_dims[1];
+  g.Dk = (int)d.grid_dims[2];
+  g.pi = d.periodic[0] ? 1 : 0;
+  g.pj = d.periodic[1] ? 1 : 0;
+  g.pk = d.periodic[2] ? 1 : 0;
+  g.ox = d.bounds_min[0];
+  g.oy = d.bounds_min[1];
+  g.oz = d.bounds_min[2];
+  g.ex = d.bounds_max[0];
+  g.ey = d.bounds_max[1];
+  g.ez = d.bounds_max[2];
+  g.cs = d.cell_size;
+  g.ghost_dist = ghost_dist;
+  g.gl = (int)std::ceil(ghost_dist / d.cell_size);
+  if ((g.pi && g.gl > g.Di) || (g.pj && g.gl > g.Dj) || (g.pk && g.gl > g.Dk))
+    return fail(c, XNB_E_DOMAIN, "ghost layers exceed the periodic domain (one image shift per side)");
+  g.di = g.Di + 2 * g.gl;
+  g.dj = g.Dj + 2 * g.gl;
+  g.dk = g.Dk + 2 * g.gl;
+  const size_t nc = (size_t)g.di * g.dj * g.dk;
+  if (nc >= (size_t)scan_max_items()) return fail(c, XNB_E_CAPACITY, "too many cells");
+  g.ncells = (uint32_t)nc;
+  c->g = g;
+  int e;
+  for (DArr<uint32_t>* a : {&c->cell_off, &c->cell_n, &c->cnt_out, &c->cnt_in, &c->new_n, &c->new_off, &c->in_cur, &c->gcnt,
+                            &c->goff, &c->clen, &c->coff})
+    if ((e = dalloc(c, *a, nc + 1))) return e;
+  return XNB_OK;
+}
+
+void bind_states(xnb_ctx* c) {
+  c->A = PState{c->rA.p, c->vA[0].p, c->vA[1].p, c->vA[2].p, c->fA[0].p, c->fA[1].p, c->fA[2].p, c->eA.p, c->idA.p};
+  c->B = PState{c->rB.p, c->vB[0].p, c->vB[1].p, c->vB[2].p, c->fB[0].p, c->fB[1].p, c->fB[2].p, c->eB.p, c->idB.p};
+}
+
+// (Re)allocate particle buffers to hold ncap slots; the first keep slots of buffer A are kept.
+int ensure_capacity(xnb_ctx* c, size_t ncap, size_t keep) {
+  if (ncap <= c->cap && c->rA.p) return XNB_OK;
+  size_t newcap = std::max(ncap, c->cap + c->cap / 4);
+  newcap = (newcap + 255) & ~(size_t)255;
+  // grow A preserving [0, keep)
+  auto grow = [&](auto& arr, size_t elems) -> int {
+    using T = std::remove_pointer_t<decltype(arr.p)>;
+    DArr<T> na;
+    int e = dalloc(c, na, elems);
+    if (e) return e;
+    if (arr.p && keep) {
+      hipError_t he = hipMemcpyAsync(na.p, arr.p, keep * sizeof(T), hipMemcpyDeviceToDevice, c->st);
+      if (he != hipSuccess) return fail(c, XNB_E_HIP, "capacity copy failed");
+    }
+    hipStreamSynchronize(c->st);
+    dfree(arr);
+    arr = na;
+    return XNB_OK;
+  };
+  int e;
+  if ((e = grow(c->rA, newcap))) return e;
+  for (int a = 0; a < 3; a++) {
+    if ((e = grow(c->vA[a], newcap))) return e;
+    if ((e = grow(c->fA[a], newcap))) return e;
+  }
+  if ((e = grow(c->eA, newcap))) return e;
+  if ((e = grow(c->idA, newcap))) return e;
+  if ((e = grow(c->slot_cell_A, newcap))) return e;
+  // B and per-slot scratch need no content
+  dfree(c->rB);
+  if ((e = dalloc(c, c->rB, newcap))) return e;
+  for (int a = 0; a < 3; a++) {
+    dfree(c->vB[a]);
+    dfree(c->fB[a]);
+    if ((e = dalloc(c, c->vB[a], newcap))) return e;
+    if ((e = dalloc(c, c->fB[a], newcap))) return e;
+  }
+  dfree(c->eB);
+  dfree(c->idB);
+  dfree(c->slot_cell_B);
+  if ((e = dalloc(c, c->eB, newcap))) return e;
+  if ((e = dalloc(c, c->idB, newcap))) return e;
+  if ((e = dalloc(c, c->slot_cell_B, newcap))) return e;
+  for (DArr<uint32_t>* a : {&c->mv_dst, &c->pk, &c->newloc, &c->in_list, &c->plen, &c->pstart, &c->ghost_src, &c->ghost_shift}) {
+    dfree(*a);
+    if ((e = dalloc(c, *a, newcap))) return e;
+  }
+  dfree(c->backup);
+  if ((e = dalloc(c, c->backup, 3 * newcap))) return e;
+  c->cap = newcap;
+  bind_states(c);
+  return XNB_OK;
+}
+
+void swap_buffers(xnb_ctx* c) {
+  std::swap(c->rA, c->rB);
+  for (int a = 0; a < 3; a++) {
+    std::swap(c->vA[a], c->vB[a]);
+    std::swap(c->fA[a], c->fB[a]);
+  }
+  std::swap(c->eA, c->eB);
+  std::swap(c->idA, c->idB);
+  std::swap(c->slot_cell_A, c->slot_cell_B);
+  bind_states(c);
+}
+
+int read_u32(xnb_ctx* c, const uint32_t* dptr, uint32_t* out) {
+  HIPCHK(c, hipMemcpyAsync(c->h_u32, dptr, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  *out = c->h_u32[0];
+  return XNB_OK;
+}
+
+int scan(xnb_ctx* c, const uint32_t* in, uint32_t n, uint32_t add, uint32_t* out) {
+  const uint32_t nb = (n + 2047) / 2048;
+  int e;
+  if ((e = dalloc(c, c->bsum, nb + 1))) return e;
+  scan_exclusive(in, n, add, out, c->bsum.p, c->u32s.p + 1, c->st);
+  return XNB_OK;
+}
+
+// ---- operators
+
+int op_move(xnb_ctx* c) {
+  const Geom& g = c->g;
+  c->lists_valid = false;
+  c->ghosts_valid = false;
+  c->backup_valid = false;
+  if (c->N == 0) return XNB_OK;
+  tstart(c, XNB_TIMER_REBIN);
+  HIPCHK(c, hipMemsetAsync(c->cnt_out.p, 0, g.ncells * sizeof(uint32_t), c->st));
+  HIPCHK(c, hipMemsetAsync(c->cnt_in.p, 0, g.ncells * sizeof(uint32_t), c->st));
+  HIPCHK(c, hipMemsetAsync(c->in_cur.p, 0, g.ncells * sizeof(uint32_t), c->st));
+  HIPCHK(c, hipMemsetAsync(c->u32s.p, 0, sizeof(uint32_t), c->st));
+  launch_move_flag(g, c->N, c->A.r, c->slot_cell_A.p, c->mv_dst.p, c->cnt_out.p, c->cnt_in.p, c->u32s.p, c->st);
+  launch_move_pack(g, c->cell_off.p, c->cell_n.p, c->cnt_out.p, c->cnt_in.p, c->mv_dst.p, c->pk.p, c->newloc.p, c->new_n.p, c->st);
+  int e;
+  if ((e = scan(c, c->new_n.p, g.ncells, 0, c->new_off.p))) return e;
+  launch_move_scatter(c->N, c->A, c->B, c->slot_cell_A.p, c->mv_dst.p, c->newloc.p, c->new_off.p, c->cell_n.p, c->cnt_out.p,
+                      c->in_cur.p, c->in_list.p, c->slot_cell_B.p, c->st);
+  launch_move_incoming(g, c->A, c->B, c->new_off.p, c->cell_n.p, c->cnt_out.p, c->cnt_in.p, c->in_list.p, c->slot_cell_B.p, c->st);
+  launch_set_inner_cells(g, c->new_n.p, c->new_off.p, c->cell_n.p, c->cell_off.p, c->st);
+  tstop(c);
+  HIPCHK(c, hipGetLastError());
+  uint32_t errflag = 0;
+  if ((e = read_u32(c, c->u32s.p, &errflag))) return e;
+  swap_buffers(c);
+  // ghost cells are empty until ghost_update_all rebuilds them
+  c->G = 0;
+  if (errflag) return fail(c, XNB_E_OUTSIDE, "a particle left the domain along a non-periodic axis");
+  return XNB_OK;
+}
+
+int op_ghost_all(xnb_ctx* c) {
+  const Geom& g = c->g;
+  int e;
+  tstart(c, XNB_TIMER_GHOST);
+  launch_ghost_count(g, c->A.r, c->cell_off.p, c->cell_n.p, c->gcnt.p, c->st);
+  if ((e = scan(c, c->gcnt.p, g.ncells, 0, c->goff.p))) return e;
+  tstop(c);
+  uint32_t G = 0;
+  if ((e = read_u32(c, c->u32s.p + 1, &G))) return e;
+  if ((e = ensure_capacity(c, (size_t)c->N + G, c->N))) return e;
+  tstart(c, XNB_TIMER_GHOST);
+  launch_ghost_fill(g, c->N, c->A, c->goff.p, c->cell_off.p, c->cell_n.p, c->slot_cell_A.p, c->ghost_src.p, c->ghost_shift.p, c->st);
+  tstop(c);
+  HIPCHK(c, hipGetLastError());
+  c->G = G;
+  c->ghosts_valid = true;
+  c->lists_valid = false;
+  return XNB_OK;
+}
+
+int op_backup(xnb_ctx* c) {
+  launch_backup_r(c->g, c->N, c->A.r, c->slot_cell_A.p, c->backup.p, c->st);
+  HIPCHK(c, hipGetLastError());
+  c->backup_valid = true;
+  return XNB_OK;
+}
+
+int op_nbh(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& cfg, int all_cells) {
+  const uint32_t n_atoms = all_cells ? c->N + c->G : c->N;
+  const double max2 = nbh_dist * nbh_dist;
+  int e;
+  for (int attempt = 0; attempt < 3; attempt++) {
+    const size_t rows = ((size_t)n_atoms + 63) / 64 * 64;
+    if ((e = dalloc(c, c->nbr, rows * c->nbr_cap))) return e;
+    if ((e = dalloc(c, c->nbr_cnt, rows))) return e;
+    HIPCHK(c, hipMemsetAsync(c->u32s.p + 2, 0, sizeof(uint32_t), c->st));
+    tstart(c, XNB_TIMER_NBH);
+    launch_nbh_build(c->g, n_atoms, c->A.r, c->slot_cell_A.p, c->cell_off.p, c->cell_n.p, max2, cfg.half_symmetric,
+                     cfg.skip_ghosts, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, c->u32s.p + 2, c->st);
+    tstop(c);
+    HIPCHK(c, hipGetLastError());
+    uint32_t mx = 0;
+    if ((e = read_u32(c, c->u32s.p + 2, &mx))) return e;
+    c->max_neighbors = mx;
+    if (mx <= c->nbr_cap) {
+      c->list_atoms = n_atoms;
+      c->list_cfg = cfg;
+      c->lists_valid = true;
+      return XNB_OK;
+    }
+    c->nbr_cap = (uint32_t)((mx + mx / 8 + 15) & ~15u);
+  }
+  return fail(c, XNB_E_OVERFLOW, "neighbour list capacity");
+}
+
+int full_update(xnb_ctx* c) {
+  int e;
+  if ((e = op_move(c))) return e;
+  if ((e = op_backup(c))) return e;
+  if ((e = op_ghost_all(c))) return e;
+  xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
+  return op_nbh(c, c->nbh_dist_lab, cfg, 0);
+}
+
+int check_ctx(xnb_ctx* c) {
+  if (!c) return XNB_E_ARG;
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return fail(c, XNB_E_HIP, hipGetErrorString(e));
+  return XNB_OK;
+}
+
+}  // namespace
+
+// =========================================================================== C ABI
+
+extern "C" {
+
+int xnb_version(void) { return 100; }
+
+const char* xnb_error_string(int code) {
+  switch (code) {
+    case XNB_OK: return "ok";
+    case XNB_E_ARG: return "invalid argument";
+    case XNB_E_DOMAIN: return "invalid domain";
+    case XNB_E_OUTSIDE: return "particle outside the domain";
+    case XNB_E_STATE: return "operator called in an invalid state";
+    case XNB_E_HIP: return "HIP runtime error";
+    case XNB_E_NOMEM: return "device allocation failed";
+    case XNB_E_OVERFLOW: return "neighbour list overflow";
+    case XNB_E_AMR: return "cell needs AMR sub-grids";
+    case XNB_E_CAPACITY: return "capacity exceeded";
+    default: return "unknown error";
+  }
+}
+
+const char* xnb_last_error(const xnb_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+int xnb_device_count(int* count) {
+  if (!count) return XNB_E_ARG;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return XNB_OK;
+}
+
+int xnb_create(xnb_ctx** out, const xnb_domain* d, int device) {
+  if (!out || !d) return fail(nullptr, XNB_E_ARG, "null argument");
+  *out = nullptr;
+  for (int a = 0; a < 3; a++) {
+    if (d->grid_dims[a] <= 0 || !(d->cell_size > 0.)) return fail(nullptr, XNB_E_ARG, "bad grid dims or cell size");
+    // check_domain (src/core/lib/domain.cpp:78-85)
+    const double gsz = (double)d->grid_dims[a] * d->cell_size;
+    const double dsz = d->bounds_max[a] - d->bounds_min[a];
+    if (!(std::fabs(1. - gsz / dsz) < 1.e-15)) return fail(nullptr, XNB_E_DOMAIN, "grid_dims*cell_size != domain size");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(nullptr, XNB_E_HIP, "no HIP device");
+  if (device < 0 || device >= ndev) return fail(nullptr, XNB_E_ARG, "bad device index");
+  xnb_ctx* c = new xnb_ctx();
+  c->device = device;
+  c->dom = *d;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(nullptr, XNB_E_HIP, "stream creation failed");
+  }
+  int e;
+  if ((e = set_geometry(c, 0.0)) || (e = dalloc(c, c->u32s, 8)) || (e = dalloc(c, c->u64s, 8)) || (e = dalloc(c, c->red_out, 2))) {
+    xnb_destroy(c);
+    return e;
+  }
+  if (hipHostMalloc((void**)&c->h_u32, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_u64, 64 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+    xnb_destroy(c);
+    return fail(nullptr, XNB_E_NOMEM, "pinned allocation failed");
+  }
+  HIPCHK(c, hipMemsetAsync(c->cell_n.p, 0, c->cell_n.n * sizeof(uint32_t), c->st));
+  HIPCHK(c, hipMemsetAsync(c->cell_off.p, 0, c->cell_off.n * sizeof(uint32_t), c->st));
+  *out = c;
+  return XNB_OK;
+}
+
+int xnb_destroy(xnb_ctx* c) {
+  if (!c) return XNB_OK;
+  hipSetDevice(c->device);
+  if (c->st) hipStreamSynchronize(c->st);
+  for (auto& t : c->pending) {
+    hipEventDestroy(t.a);
+    hipEventDestroy(t.b);
+  }
+  for (auto ev : c->pool) hipEventDestroy(ev);
+  dfree(c->slot_cell_A); dfree(c->slot_cell_B); dfree(c->rA); dfree(c->rB);
+  for (int a = 0; a < 3; a++) { dfree(c->vA[a]); dfree(c->vB[a]); dfree(c->fA[a]); dfree(c->fB[a]); }
+  dfree(c->eA); dfree(c->eB); dfree(c->idA); dfree(c->idB);
+  for (DArr<uint32_t>* a : {&c->cell_off, &c->cell_n, &c->cnt_out, &c->cnt_in, &c->new_n, &c->new_off, &c->in_cur, &c->gcnt, &c->goff,
+                            &c->clen, &c->coff, &c->mv_dst, &c->pk, &c->newloc, &c->in_list, &c->plen, &c->pstart, &c->ghost_src,
+                            &c->ghost_shift, &c->backup, &c->nbr, &c->nbr_cnt, &c->bsum, &c->u32s})
+    dfree(*a);
+  dfree(c->stream_buf); dfree(c->u64s); dfree(c->step_counts); dfree(c->red_part); dfree(c->red_out);
+  if (c->h_u32) hipHostFree(c->h_u32);
+  if (c->h_u64) hipHostFree(c->h_u64);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+  return XNB_OK;
+}
+
+int xnb_nbh_dist(xnb_ctx* c, double rcut_max, double rcut_inc, double* nbh_dist_lab, double* max_displ, double* ghost_dist) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (!(rcut_max >= 0.) || !(rcut_inc >= 0.)) return fail(c, XNB_E_ARG, "negative distance");
+  // nbh_dist.cpp:47-71 with identity xform and no bonds
+  const double lab = rcut_max + rcut_inc;
+  const double md = rcut_inc / 2.0;
+  const double gd = std::max(rcut_inc, rcut_max + rcut_inc);
+  const int new_gl = (int)std::ceil(gd / c->dom.cell_size);
+  if (c->N > 0 && new_gl != c->g.gl) {
+    // re-grid: keep particles, in cell order (per-cell order preserved by stable binning)
+    std::vector<double> r(3 * (size_t)c->N), v(3 * (size_t)c->N);
+    std::vector<uint64_t> id(c->N);
+    std::vector<uint8_t> ty(c->N);
+    if ((e = xnb_get_particles(c, 0, nullptr, r.data(), v.data(), nullptr, nullptr, id.data(), ty.data()))) return e;
+    const size_t n = c->N;
+    if ((e = set_geometry(c, gd))) return e;
+    c->nbh_set = true;
+    if ((e = xnb_set_particles(c, n, r.data(), v.data(), id.data(), ty.data()))) return e;
+  } else if ((e = set_geometry(c, gd))) {
+    return e;
+  } else if (c->N == 0) {
+    HIPCHK(c, hipMemsetAsync(c->cell_n.p, 0, c->cell_n.n * sizeof(uint32_t), c->st));
+    HIPCHK(c, hipMemsetAsync(c->cell_off.p, 0, c->cell_off.n * sizeof(uint32_t), c->st));
+  }
+  c->nbh_set = true;
+  c->nbh_dist_lab = lab;
+  c->max_displ = md;
+  c->ghost_dist = gd;
+  c->lists_valid = false;
+  if (nbh_dist_lab) *nbh_dist_lab = lab;
+  if (max_displ) *max_displ = md;
+  if (ghost_dist) *ghost_dist = gd;
+  return XNB_OK;
+}
+
+int xnb_set_particles(xnb_ctx* c, size_t n, const double* r, const double* v, const uint64_t* id, const uint8_t* type) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (n > 0 && !r) return fail(c, XNB_E_ARG, "null positions");
+  if (n >= 0x7FFFFFFFu) return fail(c, XNB_E_CAPACITY, "too many particles");
+  const Geom& g = c->g;
+  // host binning in input order (domain_periodic_location, domain.h:160-191)
+  std::vector<uint32_t> cell(n), count(g.ncells, 0);
+  std::vector<double> wr(3 * n);
+  for (size_t p = 0; p < n; p++) {
+    double x = r[3 * p], y = r[3 * p + 1], z = r[3 * p + 2];
+    int li, lj, lk;
+    periodic_location(g, x, y, z, li, lj, lk);
+    if (li < 0 || lj < 0 || lk < 0 || li >= g.Di || lj >= g.Dj || lk >= g.Dk)
+      return fail(c, XNB_E_OUTSIDE, "particle outside a non-periodic domain");
+    wr[3 * p] = x;
+    wr[3 * p + 1] = y;
+    wr[3 * p + 2] = z;
+    cell[p] = cell_index(g, li + g.gl, lj + g.gl, lk + g.gl);
+    count[cell[p]]++;
+  }
+  std::vector<uint32_t> off(g.ncells + 1, 0);
+  uint32_t acc = 0;
+  for (uint32_t q = 0; q < g.ncells; q++) {
+    int i, j, k;
+    cell_ijk(g, q, i, j, k);
+    off[q] = acc;
+    if (!is_ghost(g, i, j, k)) acc += count[q];
+  }
+  for (uint32_t q = 0; q < g.ncells; q++) {  // ghost cells: empty, placed after inner slots
+    int i, j, k;
+    cell_ijk(g, q, i, j, k);
+    if (is_ghost(g, i, j, k)) off[q] = acc;
+  }
+  std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+  std::vector<double4> hr(n);
+  std::vector<double> hv[3];
+  for (int a = 0; a < 3; a++) hv[a].assign(n, 0.0);
+  std::vector<uint64_t> hid(n);
+  std::vector<uint32_t> hsc(n);
+  for (size_t p = 0; p < n; p++) {
+    const uint32_t t = cur[cell[p]]++;
+    hr[t] = make_double4(wr[3 * p], wr[3 * p + 1], wr[3 * p + 2], type ? (double)type[p] : 0.0);
+    for (int a = 0; a < 3; a++) hv[a][t] = v ? v[3 * p + a] : 0.0;
+    hid[t] = id ? id[p] : (uint64_t)p;
+    hsc[t] = cell[p];
+  }
+  c->N = (uint32_t)n;
+  c->G = 0;
+  const size_t est = n + std::max<size_t>(n / 2, 4096);
+  if ((e = ensure_capacity(c, est, 0))) return e;
+  if (n > 0) {
+    HIPCHK(c, hipMemcpyAsync(c->A.r, hr.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->A.vx, hv[0].data(), n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->A.vy, hv[1].data(), n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->A.vz, hv[2].data(), n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->A.id, hid.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->slot_cell_A.p, hsc.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, c->st));
+    launch_zero_force((uint32_t)n, c->A, c->st);
+  }
+  std::vector<uint32_t> cn(g.ncells);
+  for (uint32_t q = 0; q < g.ncells; q++) {
+    int i, j, k;
+    cell_ijk(g, q, i, j, k);
+    cn[q] = is_ghost(g, i, j, k) ? 0u : count[q];
+  }
+  HIPCHK(c, hipMemcpyAsync(c->cell_n.p, cn.data(), g.ncells * sizeof(uint32_t), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->cell_off.p, off.data(), g.ncells * sizeof(uint32_t), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->lists_valid = c->ghosts_valid = c->backup_valid = false;
+  return XNB_OK;
+}
+
+int xnb_num_particles(const xnb_ctx* c, size_t* n_inner, size_t* n_ghost) {
+  if (!c) return XNB_E_ARG;
+  if (n_inner) *n_inner = c->N;
+  if (n_ghost) *n_ghost = c->G;
+  return XNB_OK;
+}
+
+int xnb_grid_info(const xnb_ctx* c, int64_t dims[3], int* ghost_layers, size_t* n_cells) {
+  if (!c) return XNB_E_ARG;
+  if (dims) {
+    dims[0] = c->g.di;
+    dims[1] = c->g.dj;
+    dims[2] = c->g.dk;
+  }
+  if (ghost_layers) *ghost_layers = c->g.gl;
+  if (n_cells) *n_cells = c->g.ncells;
+  return XNB_OK;
+}
+
+int xnb_get_cell_counts(xnb_ctx* c, uint32_t* counts) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (!counts) return fail(c, XNB_E_ARG, "null output");
+  HIPCHK(c, hipMemcpyAsync(counts, c->cell_n.p, c->g.ncells * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
+
+int xnb_get_particles(xnb_ctx* c, int include_ghosts, uint32_t* cell, double* r, double* v, double* f, double* ep, uint64_t* id,
+                      uint8_t* type) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  const size_t tot = (size_t)c->N + c->G;
+  const Geom& g = c->g;
+  std::vector<double4> hr(tot);
+  std::vector<double> hv[3], hf[3], he(tot);
+  std::vector<uint64_t> hid(tot);
+  std::vector<uint32_t> off(g.ncells), cn(g.ncells);
+  for (int a = 0; a < 3; a++) {
+    hv[a].resize(tot);
+    hf[a].resize(tot);
+  }
+  if (tot) {
+    HIPCHK(c, hipMemcpyAsync(hr.data(), c->A.r, tot * sizeof(double4), hipMemcpyDeviceToHost, c->st));
+    double* dv[3] = {c->A.vx, c->A.vy, c->A.vz};
+    double* df[3] = {c->A.fx, c->A.fy, c->A.fz};
+    for (int a = 0; a < 3; a++) {
+      HIPCHK(c, hipMemcpyAsync(hv[a].data(), dv[a], tot * sizeof(double), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(c, hipMemcpyAsync(hf[a].data(), df[a], tot * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    }
+    HIPCHK(c, hipMemcpyAsync(he.data(), c->A.ep, tot * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(hid.data(), c->A.id, tot * sizeof(uint64_t), hipMemcpyDeviceToHost, c->st));
+  }
+  HIPCHK(c, hipMemcpyAsync(off.data(), c->cell_off.p, g.ncells * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(cn.data(), c->cell_n.p, g.ncells * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  size_t k = 0;
+  for (uint32_t q = 0; q < g.ncells; q++) {
+    int i, j, kk;
+    cell_ijk(g, q, i, j, kk);
+    if (!include_ghosts && is_ghost(g, i, j, kk)) continue;
+    for (uint32_t p = 0; p < cn[q]; p++, k++) {
+      const size_t s = (size_t)off[q] + p;
+      if (cell) cell[k] = q;
+      if (r) {
+        r[3 * k] = hr[s].x;
+        r[3 * k + 1] = hr[s].y;
+        r[3 * k + 2] = hr[s].z;
+      }
+      for (int a = 0; a < 3; a++) {
+        if (v) v[3 * k + a] = hv[a][s];
+        if (f) f[3 * k + a] = hf[a][s];
+      }
+      if (ep) ep[k] = he[s];
+      if (id) id[k] = hid[s];
+      if (type) type[k] = (uint8_t)hr[s].w;
+    }
+  }
+  return XNB_OK;
+}
+
+int xnb_move_particles(xnb_ctx* c) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  return op_move(c);
+}
+
+int xnb_backup_r(xnb_ctx* c) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  return op_backup(c);
+}
+
+int xnb_ghost_update_all(xnb_ctx* c) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (!c->nbh_set) return fail(c, XNB_E_STATE, "nbh_dist must run before ghost_update_all");
+  return op_ghost_all(c);
+}
+
+int xnb_ghost_update_r(xnb_ctx* c) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (!c->ghosts_valid) return fail(c, XNB_E_STATE, "ghost_update_r needs a ghost scheme (ghost_update_all)");
+  tstart(c, XNB_TIMER_GHOST);
+  launch_ghost_update_r(c->g, c->N, c->G, c->A.r, c->ghost_src.p, c->ghost_shift.p, c->st);
+  tstop(c);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
+
+int xnb_particle_displ_over(xnb_ctx* c, double threshold, uint64_t* count) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (!count) return fail(c, XNB_E_ARG, "null output");
+  if (!c->backup_valid) return fail(c, XNB_E_STATE, "particle_displ_over needs backup_r");
+  HIPCHK(c, hipMemsetAsync(c->u64s.p, 0, sizeof(unsigned long long), c->st));
+  tstart(c, XNB_TIMER_TRIGGER);
+  launch_displ_over(c->g, c->N, c->A.r, c->slot_cell_A.p, c->backup.p, threshold * threshold, c->u64s.p, c->st);
+  tstop(c);
+  HIPCHK(c, hipMemcpyAsync(c->h_u64, c->u64s.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  *count = c->h_u64[0];
+  return XNB_OK;
+}
+
+int xnb_chunk_neighbors(xnb_ctx* c, double nbh_dist_lab, const xnb_chunk_neighbors_config* cfg, int all_cells) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  xnb_chunk_neighbors_config k = cfg ? *cfg : xnb_chunk_neighbors_config{1, 0, 0, 0, 0};
+  if (k.dual_particle_offset) k.build_particle_offset = 1;  // chunk_neighbors_config.h:76-80
+  return op_nbh(c, nbh_dist_lab, k, all_cells);
+}
+
+int xnb_chunk_neighbors_stream(xnb_ctx* c, uint16_t* out, size_t cap, uint64_t* cell_offsets, size_t* total,
+                               uint32_t* max_neighbors) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (!c->lists_valid) return fail(c, XNB_E_STATE, "no neighbour lists (chunk_neighbors)");
+  const Geom& g = c->g;
+  const int ntables = c->list_cfg.build_particle_offset ? (c->list_cfg.dual_particle_offset ? 2 : 1) : 0;
+  launch_stream_plen(c->list_atoms, c->slot_cell_A.p, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, c->list_cfg.random_access, c->plen.p,
+                     c->st);
+  launch_stream_cell_len(g, c->list_atoms, c->cell_off.p, c->cell_n.p, c->plen.p, ntables, c->pstart.p, c->clen.p, c->st);
+  if ((e = scan(c, c->clen.p, g.ncells, 0, c->coff.p))) return e;
+  uint32_t tot = 0;
+  if ((e = read_u32(c, c->u32s.p + 1, &tot))) return e;
+  if (total) *total = tot;
+  if (max_neighbors) *max_neighbors = c->max_neighbors;
+  if (!out) return XNB_OK;
+  if (cap < tot) return fail(c, XNB_E_CAPACITY, "stream output buffer too small");
+  if ((e = dalloc(c, c->stream_buf, (size_t)tot + 1))) return e;
+  launch_stream_fill(g, c->list_atoms, c->slot_cell_A.p, c->cell_off.p, c->cell_n.p, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, c->pstart.p,
+                     c->plen.p, c->coff.p, ntables, c->list_cfg.random_access, c->stream_buf.p, c->st);
+  HIPCHK(c, hipGetLastError());
+  std::vector<uint32_t> co(g.ncells);
+  HIPCHK(c, hipMemcpyAsync(out, c->stream_buf.p, (size_t)tot * sizeof(uint16_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(co.data(), c->coff.p, g.ncells * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (cell_offsets) {
+    for (uint32_t q = 0; q < g.ncells; q++) cell_offsets[q] = co[q];
+    cell_offsets[g.ncells] = tot;
+  }
+  return XNB_OK;
+}
+
+int xnb_zero_particle_force(xnb_ctx* c, int ghost) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  launch_zero_force(ghost ? c->N + c->G : c->N, c->A, c->st);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
+
+int xnb_lennard_jones_force(xnb_ctx* c, const xnb_lj_params* p, double rcut, int precision, int ghost) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (!p || (precision != XNB_PRECISION_FP64 && precision != XNB_PRECISION_MIXED)) return fail(c, XNB_E_ARG, "bad LJ arguments");
+  const uint32_t n_atoms = ghost ? c->N + c->G : c->N;
+  if (!c->lists_valid || c->list_atoms < n_atoms) return fail(c, XNB_E_STATE, "neighbour lists do not cover the requested cells");
+  c->last_rcut = rcut;
+  tstart(c, XNB_TIMER_FORCE);
+  launch_lj(precision, true, false, n_atoms, c->A, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, p->epsilon, p->sigma, rcut, 0.0, c->st);
+  tstop(c);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
+
+int xnb_lennard_jones_force_multimat(xnb_ctx* c, int ntypes, const double* eps, const double* sigma, const double* rcut,
+                                     int precision) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (ntypes <= 0 || ntypes > 4 || !eps || !sigma || !rcut) return fail(c, XNB_E_ARG, "bad multimat arguments (1..4 types)");
+  if (!c->lists_valid) return fail(c, XNB_E_STATE, "no neighbour lists");
+  tstart(c, XNB_TIMER_FORCE);
+  if (launch_lj_multimat(precision, c->N, c->A, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, ntypes, eps, sigma, rcut, c->st))
+    return fail(c, XNB_E_ARG, "too many type pairs");
+  tstop(c);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
+
+int xnb_push_f_v_r(xnb_ctx* c, double dt, double dt_scale) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  const double delta_t = dt * dt_scale;
+  const double delta_t2 = delta_t * delta_t * 0.5;
+  tstart(c, XNB_TIMER_PUSH);
+  launch_push_f_v_r(c->N, c->A, delta_t, delta_t2, c->st);
+  tstop(c);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
+
+int xnb_push_f_v(xnb_ctx* c, double dt, double dt_scale) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  tstart(c, XNB_TIMER_PUSH);
+  launch_push_f_v(c->N, c->A, dt * dt_scale, c->st);
+  tstop(c);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
+
+int xnb_simulation_energies(xnb_ctx* c, double* ke, double* pe) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  const uint32_t nparts = std::max<uint32_t>(1, std::min<uint32_t>(1024, nb256(c->N)));
+  if ((e = dalloc(c, c->red_part, 2 * (size_t)nparts))) return e;
+  launch_energies(c->N, c->A, c->red_part.p, nparts, c->red_out.p, c->st);
+  double h[2];
+  HIPCHK(c, hipMemcpyAsync(h, c->red_out.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (ke) *ke = h[0];
+  if (pe) *pe = h[1];
+  return XNB_OK;
+}
+
+int xnb_init(xnb_ctx* c, const xnb_lj_params* p, double rcut, double rcut_inc, int precision) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (!p) return fail(c, XNB_E_ARG, "null LJ parameters");
+  if ((e = xnb_nbh_dist(c, rcut, rcut_inc, nullptr, nullptr, nullptr))) return e;
+  if ((e = full_update(c))) return e;
+  c->last_rcut = rcut;
+  tstart(c, XNB_TIMER_FORCE);
+  launch_lj(precision, false, false, c->N, c->A, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, p->epsilon, p->sigma, rcut, 0.0, c->st);
+  tstop(c);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
+
+int xnb_run(xnb_ctx* c, xnb_run_params* p) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (!p || p->steps < 0) return fail(c, XNB_E_ARG, "bad run parameters");
+  if (!c->lists_valid || !c->ghosts_valid || !c->backup_valid) return fail(c, XNB_E_STATE, "xnb_init must run first");
+  const double dt = p->dt * 1.0, dt2 = dt * dt * 0.5, dth = p->dt * 0.5;
+  const double thr2 = c->max_displ * c->max_displ;
+  const bool interval = p->rebuild_every > 0;
+  c->last_rcut = p->rcut;
+  p->n_rebuilds = 0;
+  p->n_violations = 0;
+  if (interval && p->steps > 0) {
+    if ((e = dalloc(c, c->step_counts, (size_t)p->steps))) return e;
+    HIPCHK(c, hipMemsetAsync(c->step_counts.p, 0, (size_t)p->steps * sizeof(unsigned long long), c->st));
+  }
+  std::vector<char> rebuilt(interval ? (size_t)p->steps : 0, 0);
+  for (int64_t step = 1; step <= p->steps; step++) {
+    unsigned long long* cnt = interval ? c->step_counts.p + (step - 1) : c->u64s.p;
+    if (!interval) HIPCHK(c, hipMemsetAsync(c->u64s.p, 0, sizeof(unsigned long long), c->st));
+    tstart(c, XNB_TIMER_PUSH);
+    launch_verlet_first_half(c->g, c->N, c->A, dt, dt2, dth, c->slot_cell_A.p, c->backup.p, thr2, cnt, c->st);
+    tstop(c);
+    bool trigger;
+    if (interval) {
+      trigger = (step % p->rebuild_every) == 0;
+    } else {
+      HIPCHK(c, hipMemcpyAsync(c->h_u64, c->u64s.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(c, hipStreamSynchronize(c->st));
+      trigger = c->h_u64[0] > 0;
+    }
+    if (trigger) {
+      if ((e = full_update(c))) return e;
+      p->n_rebuilds++;
+      if (interval) rebuilt[(size_t)step - 1] = 1;
+    } else {
+      tstart(c, XNB_TIMER_GHOST);
+      launch_ghost_update_r(c->g, c->N, c->G, c->A.r, c->ghost_src.p, c->ghost_shift.p, c->st);
+      tstop(c);
+    }
+    tstart(c, XNB_TIMER_FORCE);
+    launch_lj(p->precision, false, true, c->N, c->A, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, p->lj.epsilon, p->lj.sigma, p->rcut, dth,
+              c->st);
+    tstop(c);
+  }
+  HIPCHK(c, hipGetLastError());
+  if (interval && p->steps > 0) {
+    std::vector<unsigned long long> h((size_t)p->steps);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->step_counts.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (size_t q = 0; q < h.size(); q++)
+      if (h[q] > 0 && !rebuilt[q]) p->n_violations++;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
+
+int xnb_timer_enable(xnb_ctx* c, int enable) {
+  if (!c) return XNB_E_ARG;
+  c->timing = enable != 0;
+  return XNB_OK;
+}
+
+int xnb_timer_reset(xnb_ctx* c) {
+  if (!c) return XNB_E_ARG;
+  flush_timers(c);
+  for (int q = 0; q < XNB_TIMER_COUNT; q++) {
+    c->t_ms[q] = 0.;
+    c->t_n[q] = 0;
+  }
+  return XNB_OK;
+}
+
+int xnb_timer_read(xnb_ctx* c, int timer, double* total_ms, int64_t* launches) {
+  if (!c || timer < 0 || timer >= XNB_TIMER_COUNT) return XNB_E_ARG;
+  flush_timers(c);
+  if (total_ms) *total_ms = c->t_ms[timer];
+  if (launches) *launches = c->t_n[timer];
+  return XNB_OK;
+}
+
+int xnb_pair_counts(xnb_ctx* c, uint64_t* list_entries, uint64_t* cutoff_pairs) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (!c->lists_valid) return fail(c, XNB_E_STATE, "no neighbour lists");
+  HIPCHK(c, hipMemsetAsync(c->u64s.p + 1, 0, 2 * sizeof(unsigned long long), c->st));
+  launch_pair_counts(c->N, c->A.r, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, c->last_rcut * c->last_rcut, c->u64s.p + 1, c->st);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(c->h_u64, c->u64s.p + 1, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (list_entries) *list_entries = c->h_u64[0];
+  if (cutoff_pairs) *cutoff_pairs = c->h_u64[1];
+  return XNB_OK;
+}
+
+int xnb_synchronize(xnb_ctx* c) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+// xnb_kernels.h -- launch interface of the gfx950 kernels (host side of xnb_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "xnb_device.h"
+
+namespace xnb {
+
+// Particle arrays of one storage buffer (slots [0, N) inner, [N, N+G) ghosts).
+struct PState {
+  double4* r;  // x, y, z, type
+  double *vx, *vy, *vz;
+  double *fx, *fy, *fz;
+  double* ep;  // per-atom potential energy (sum of pair energies over the full list)
+  uint64_t* id;
+};
+
+void scan_exclusive(const uint32_t* in, uint32_t n, uint32_t add, uint32_t* out, uint32_t* bsum, uint32_t* total, hipStream_t st);
+uint32_t scan_max_items();
+
+void launch_move_flag(const Geom& g, uint32_t N, const double4* r, const uint32_t* slot_cell, uint32_t* mv_dst, uint32_t* cnt_out,
+                      uint32_t* cnt_in, uint32_t* err, hipStream_t st);
+void launch_move_pack(const Geom& g, const uint32_t* cell_off, const uint32_t* cell_n, const uint32_t* cnt_out,
+                      const uint32_t* cnt_in, const uint32_t* mv_dst, uint32_t* pk, uint32_t* newloc, uint32_t* new_n,
+                      hipStream_t st);
+void launch_move_scatter(uint32_t N, const PState& A, const PState& B, const uint32_t* slot_cell, const uint32_t* mv_dst,
+                         const uint32_t* newloc, const uint32_t* new_off, const uint32_t* cell_n, const uint32_t* cnt_out,
+                         uint32_t* in_cur, uint32_t* in_list, uint32_t* slot_cell_B, hipStream_t st);
+void launch_move_incoming(const Geom& g, const PState& A, const PState& B, const uint32_t* new_off, const uint32_t* cell_n,
+                          const uint32_t* cnt_out, const uint32_t* cnt_in, uint32_t* in_list, uint32_t* slot_cell_B,
+                          hipStream_t st);
+void launch_set_inner_cells(const Geom& g, const uint32_t* new_n, const uint32_t* new_off, uint32_t* cell_n, uint32_t* cell_off,
+                            hipStream_t st);
+void launch_ghost_count(const Geom& g, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, uint32_t* gcnt,
+                        hipStream_t st);
+void launch_ghost_fill(const Geom& g, uint32_t N, const PState& B, const uint32_t* goff, uint32_t* cell_off, uint32_t* cell_n,
+                       uint32_t* slot_cell, uint32_t* ghost_src, uint32_t* ghost_shift, hipStream_t st);
+void launch_ghost_update_r(const Geom& g, uint32_t N, uint32_t G, double4* r, const uint32_t* ghost_src, const uint32_t* ghost_shift,
+                           hipStream_t st);
+void launch_backup_r(const Geom& g, uint32_t N, const double4* r, const uint32_t* slot_cell, uint32_t* backup, hipStream_t st);
+void launch_displ_over(const Geom& g, uint32_t N, const double4* r, const uint32_t* slot_cell, const uint32_t* backup, double thr2,
+                       unsigned long long* count, hipStream_t st);
+void launch_push_f_v_r(uint32_t N, const PState& A, double dt, double dt2, hipStream_t st);
+void launch_push_f_v(uint32_t N, const PState& A, double dt, hipStream_t st);
+void launch_verlet_first_half(const Geom& g, uint32_t N, const PState& A, double dt, double dt2, double dth,
+                              const uint32_t* slot_cell, const uint32_t* backup, double thr2, unsigned long long* count,
+                              hipStream_t st);
+void launch_zero_force(uint32_t n, const PState& A, hipStream_t st);
+void launch_nbh_build(const Geom& g, uint32_t n_atoms, const double4* r, const uint32_t* slot_cell, const uint32_t* cell_off,
+                      const uint32_t* cell_n, double max2, int half_symmetric, int skip_ghosts, uint32_t* nbr, uint32_t* nbr_cnt,
+                      uint32_t cap, uint32_t* max_cnt, hipStream_t st);
+void launch_lj(int precision, bool accum, bool push, uint32_t n_atoms, const PState& A, const uint32_t* nbr,
+               const uint32_t* nbr_cnt, uint32_t cap, double eps, double sigma, double rcut, double dth, hipStream_t st);
+int launch_lj_multimat(int precision, uint32_t n_atoms, const PState& A, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap,
+                       int ntypes, const double* eps, const double* sigma, const double* rcut, hipStream_t st);
+void launch_pair_counts(uint32_t n_atoms, const double4* r, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap, double rc2,
+                        unsigned long long* out, hipStream_t st);
+void launch_stream_plen(uint32_t n_atoms, const uint32_t* slot_cell, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap,
+                        int random_access, uint32_t* plen, hipStream_t st);
+void launch_stream_cell_len(const Geom& g, uint32_t n_atoms, const uint32_t* cell_off, const uint32_t* cell_n, const uint32_t* plen,
+                            int ntables, uint32_t* pstart, uint32_t* clen, hipStream_t st);
+void launch_stream_fill(const Geom& g, uint32_t n_atoms, const uint32_t* slot_cell, const uint32_t* cell_off, const uint32_t* cell_n,
+                        const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap, const uint32_t* pstart, const uint32_t* plen,
+                        const uint32_t* coff, int ntables, int random_access, uint16_t* out, hipStream_t st);
+void launch_energies(uint32_t N, const PState& A, double* part, uint32_t nparts, double* out, hipStream_t st);
+
+}  // namespace xnb

@@ -1,0 +1,1037 @@
+// xnb_kernels.hip -- gfx950 kernels of the short-range LJ hot path.
+//
+// Storage (per box, in HBM): particles sorted by cell, inner cells first in cell-index
+// order in slots [0, N), ghost cells after in [N, N+G).  cell_off/cell_n give each cell's
+// slot range.  Positions are double4 (x, y, z, type) so one 32-byte load fetches a
+// neighbour; velocities, forces and energies are SoA doubles.
+//
+// Compiled with -ffp-contract=off: FP64 arithmetic is evaluated in source order without
+// contraction.  Explicit fma() is used only in the mixed-precision force kernel.
+#include "xnb_kernels.h"
+
+namespace xnb {
+
+// ----------------------------------------------------------------------------- scans
+// Exclusive scan of n u32 values: per-block sums, one-block scan of block sums, then
+// per-block scan with offsets.  SCAN_ITEMS items per thread, 256 threads per block.
+static constexpr int SCAN_THREADS = 256;
+static constexpr int SCAN_ITEMS = 8;
+static constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[SCAN_THREADS / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  uint32_t woff = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_THREADS / 64; i++) {
+    if (i < w) woff += wsum[i];
+    tot += wsum[i];
+  }
+  __syncthreads();
+  *total = tot;
+  return x - v + woff;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_tile_sums(const uint32_t* in, uint32_t n, uint32_t* bsum) {
+  const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+  uint32_t s = 0;
+#pragma unroll
+  for (int q = 0; q < SCAN_ITEMS; q++)
+    if (base + q < n) s += in[base + q];
+  uint32_t tot;
+  block_excl_scan(s, &tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// one block of SCAN_THREADS threads scans nb <= SCAN_TILE block sums in place
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_bsums(uint32_t* bsum, uint32_t nb, uint32_t* total) {
+  const uint32_t base = threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  uint32_t s = 0;
+#pragma unroll
+  for (int q = 0; q < SCAN_ITEMS; q++) {
+    v[q] = (base + q < nb) ? bsum[base + q] : 0u;
+    s += v[q];
+  }
+  uint32_t tot;
+  uint32_t off = block_excl_scan(s, &tot);
+#pragma unroll
+  for (int q = 0; q < SCAN_ITEMS; q++) {
+    if (base + q < nb) bsum[base + q] = off;
+    off += v[q];
+  }
+  if (threadIdx.x == 0) *total = tot;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t* in, uint32_t n, const uint32_t* bsum, uint32_t add,
+                                                             uint32_t* out) {
+  const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  uint32_t s = 0;
+#pragma unroll
+  for (int q = 0; q < SCAN_ITEMS; q++) {
+    v[q] = (base + q < n) ? in[base + q] : 0u;
+    s += v[q];
+  }
+  uint32_t tot;
+  uint32_t off = block_excl_scan(s, &tot) + bsum[blockIdx.x] + add;
+#pragma unroll
+  for (int q = 0; q < SCAN_ITEMS; q++) {
+    if (base + q < n) out[base + q] = off;
+    off += v[q];
+  }
+}
+
+// in and out may alias.  bsum needs ceil(n/SCAN_TILE) <= SCAN_TILE entries.
+void scan_exclusive(const uint32_t* in, uint32_t n, uint32_t add, uint32_t* out, uint32_t* bsum, uint32_t* total,
+                    hipStream_t st) {
+  uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (nb == 0) nb = 1;
+  k_scan_tile_sums<<<nb, SCAN_THREADS, 0, st>>>(in, n, bsum);
+  k_scan_bsums<<<1, SCAN_THREADS, 0, st>>>(bsum, nb, total);
+  k_scan_apply<<<nb, SCAN_THREADS, 0, st>>>(in, n, bsum, add, out);
+}
+
+uint32_t scan_max_items() { return (uint32_t)SCAN_TILE * (uint32_t)SCAN_TILE; }
+
+// --------------------------------------------------------------- move_particles
+// Re-binning (move_particles_across_cells.h:78-235) with the order one OpenMP thread
+// produces: stayers packed by swap-with-last, then incoming movers in source-slot order
+// (source cells are visited in index order, particles in cell order, and inner slots are
+// laid out in exactly that order).
+
+__global__ void k_move_flag(Geom g, uint32_t N, const double4* __restrict__ r, const uint32_t* __restrict__ slot_cell,
+                            uint32_t* __restrict__ mv_dst, uint32_t* __restrict__ cnt_out, uint32_t* __restrict__ cnt_in,
+                            uint32_t* __restrict__ err) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  const uint32_t c = slot_cell[s];
+  int si, sj, sk;
+  cell_ijk(g, c, si, sj, sk);
+  const double4 p = r[s];
+  double x = p.x, y = p.y, z = p.z;
+  int li, lj, lk;
+  periodic_location(g, x, y, z, li, lj, lk);
+  const int di = li + g.gl, dj = lj + g.gl, dk = lk + g.gl;
+  bool moved = (di != si || dj != sj || dk != sk);
+  if (!moved) {
+    // min_distance2_between(ro, cell_bounds(dst)) >= epsilon_cell_size2 (grid.h:60,89-90)
+    const double eps = g.cs * (1.0 / 281474976710656.0);
+    const double eps2 = eps * eps;
+    const double b0[3] = {cell_origin(g.ox, g.gl, di, g.cs), cell_origin(g.oy, g.gl, dj, g.cs), cell_origin(g.oz, g.gl, dk, g.cs)};
+    const double b1[3] = {cell_origin(g.ox, g.gl, di + 1, g.cs), cell_origin(g.oy, g.gl, dj + 1, g.cs),
+                          cell_origin(g.oz, g.gl, dk + 1, g.cs)};
+    const double ro[3] = {p.x, p.y, p.z};
+    double d[3];
+    for (int a = 0; a < 3; a++) {
+      d[a] = 0.;
+      if (ro[a] < b0[a]) d[a] = b0[a] - ro[a];
+      else if (ro[a] > b1[a]) d[a] = ro[a] - b1[a];
+    }
+    moved = (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) >= eps2;
+  }
+  if (!moved) {
+    mv_dst[s] = NONE;
+    return;
+  }
+  if (di < g.gl || dj < g.gl || dk < g.gl || di >= g.di - g.gl || dj >= g.dj - g.gl || dk >= g.dk - g.gl) {
+    atomicOr(err, 1u);  // leaves the domain on a non-periodic axis (otb_particles)
+    mv_dst[s] = NONE;
+    return;
+  }
+  const uint32_t dc = cell_index(g, di, dj, dk);
+  mv_dst[s] = dc;
+  atomicAdd(&cnt_out[c], 1u);
+  atomicAdd(&cnt_in[dc], 1u);
+}
+
+// one thread per cell: swap-with-last packing (move_particles_across_cells.h:186-209)
+__global__ void k_move_pack(Geom g, const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
+                            const uint32_t* __restrict__ cnt_out, const uint32_t* __restrict__ cnt_in,
+                            const uint32_t* __restrict__ mv_dst, uint32_t* __restrict__ pk, uint32_t* __restrict__ newloc,
+                            uint32_t* __restrict__ new_n) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncells) return;
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  if (is_ghost(g, i, j, k)) {
+    new_n[c] = 0;
+    return;
+  }
+  const uint32_t n = cell_n[c], off = cell_off[c], nout = cnt_out[c];
+  new_n[c] = n - nout + cnt_in[c];
+  if (nout == 0) {
+    for (uint32_t q = 0; q < n; q++) newloc[off + q] = q;
+    return;
+  }
+  for (uint32_t q = 0; q < n; q++) pk[off + q] = (mv_dst[off + q] == NONE) ? q : NONE;
+  int m = (int)n;
+  for (int q = (int)n - 1; q >= 0; q--) {
+    if (pk[off + q] == NONE) {
+      if (q < m - 1) pk[off + q] = pk[off + m - 1];
+      --m;
+    }
+  }
+  for (int q = 0; q < m; q++) newloc[off + pk[off + q]] = (uint32_t)q;
+}
+
+__device__ inline void copy_particle(const PState& A, uint32_t s, const PState& B, uint32_t t) {
+  B.r[t] = A.r[s];
+  B.vx[t] = A.vx[s]; B.vy[t] = A.vy[s]; B.vz[t] = A.vz[s];
+  B.fx[t] = A.fx[s]; B.fy[t] = A.fy[s]; B.fz[t] = A.fz[s];
+  B.ep[t] = A.ep[s];
+  B.id[t] = A.id[s];
+}
+
+__global__ void k_move_scatter(uint32_t N, PState A, PState B, const uint32_t* __restrict__ slot_cell,
+                               const uint32_t* __restrict__ mv_dst, const uint32_t* __restrict__ newloc,
+                               const uint32_t* __restrict__ new_off, const uint32_t* __restrict__ cell_n,
+                               const uint32_t* __restrict__ cnt_out, uint32_t* __restrict__ in_cur,
+                               uint32_t* __restrict__ in_list, uint32_t* __restrict__ slot_cell_B) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  const uint32_t c = slot_cell[s];
+  const uint32_t dc = mv_dst[s];
+  if (dc == NONE) {
+    const uint32_t t = new_off[c] + newloc[s];
+    copy_particle(A, s, B, t);
+    slot_cell_B[t] = c;
+  } else {
+    const uint32_t pos = atomicAdd(&in_cur[dc], 1u);
+    in_list[new_off[dc] + (cell_n[dc] - cnt_out[dc]) + pos] = s;
+  }
+}
+
+// one thread per cell: order incoming movers by source slot, copy them with wrapped positions
+__global__ void k_move_incoming(Geom g, PState A, PState B, const uint32_t* __restrict__ new_off,
+                                const uint32_t* __restrict__ cell_n, const uint32_t* __restrict__ cnt_out,
+                                const uint32_t* __restrict__ cnt_in, uint32_t* __restrict__ in_list,
+                                uint32_t* __restrict__ slot_cell_B) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncells) return;
+  const uint32_t nin = cnt_in[c];
+  if (nin == 0) return;
+  const uint32_t b = new_off[c] + (cell_n[c] - cnt_out[c]);
+  for (uint32_t q = 1; q < nin; q++) {  // insertion sort (few movers per cell)
+    uint32_t v = in_list[b + q];
+    int p = (int)q - 1;
+    while (p >= 0 && in_list[b + p] > v) {
+      in_list[b + p + 1] = in_list[b + p];
+      --p;
+    }
+    in_list[b + p + 1] = v;
+  }
+  for (uint32_t q = 0; q < nin; q++) {
+    const uint32_t s = in_list[b + q], t = b + q;
+    copy_particle(A, s, B, t);
+    double4 p = A.r[s];
+    int li, lj, lk;
+    periodic_location(g, p.x, p.y, p.z, li, lj, lk);
+    B.r[t] = p;
+    slot_cell_B[t] = c;
+  }
+}
+
+__global__ void k_set_inner_cells(uint32_t ncells, const uint32_t* __restrict__ new_n, const uint32_t* __restrict__ new_off,
+                                  Geom g, uint32_t* __restrict__ cell_n, uint32_t* __restrict__ cell_off) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  if (is_ghost(g, i, j, k)) return;
+  cell_n[c] = new_n[c];
+  cell_off[c] = new_off[c];
+}
+
+// ------------------------------------------------------------------------ ghosts
+// ghost_comm_scheme + ghost_update_all on one rank (update_ghosts_comm_scheme.cpp:400-470):
+// ghost cell at domain location gd takes inner cell gd - shift*D; a source particle is kept
+// iff its shifted image is inside enlarge(domain bounds, ghost_dist), bounds inclusive.
+
+__device__ inline bool ghost_source(const Geom& g, uint32_t c, uint32_t& src, int& si, int& sj, int& sk) {
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  if (!is_ghost(g, i, j, k)) return false;
+  const int gi = i - g.gl, gj = j - g.gl, gk = k - g.gl;
+  si = gi < 0 ? -1 : (gi >= g.Di ? 1 : 0);
+  sj = gj < 0 ? -1 : (gj >= g.Dj ? 1 : 0);
+  sk = gk < 0 ? -1 : (gk >= g.Dk ? 1 : 0);
+  if ((si && !g.pi) || (sj && !g.pj) || (sk && !g.pk)) return false;
+  src = cell_index(g, gi - si * g.Di + g.gl, gj - sj * g.Dj + g.gl, gk - sk * g.Dk + g.gl);
+  return true;
+}
+
+__device__ inline bool ghost_keep(const Geom& g, double4 p, int si, int sj, int sk, double4& q) {
+  const double bx0 = ((double)0 * g.cs + g.ox) - g.ghost_dist, bx1 = ((double)g.Di * g.cs + g.ox) + g.ghost_dist;
+  const double by0 = ((double)0 * g.cs + g.oy) - g.ghost_dist, by1 = ((double)g.Dj * g.cs + g.oy) + g.ghost_dist;
+  const double bz0 = ((double)0 * g.cs + g.oz) - g.ghost_dist, bz1 = ((double)g.Dk * g.cs + g.oz) + g.ghost_dist;
+  q.x = shift_coord(p.x, si, g.ox, g.ex);
+  q.y = shift_coord(p.y, sj, g.oy, g.ey);
+  q.z = shift_coord(p.z, sk, g.oz, g.ez);
+  q.w = p.w;
+  return q.x >= bx0 && q.x <= bx1 && q.y >= by0 && q.y <= by1 && q.z >= bz0 && q.z <= bz1;
+}
+
+__global__ void k_ghost_count(Geom g, const double4* __restrict__ r, const uint32_t* __restrict__ cell_off,
+                              const uint32_t* __restrict__ cell_n, uint32_t* __restrict__ gcnt) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncells) return;
+  uint32_t src;
+  int si, sj, sk;
+  uint32_t cnt = 0;
+  if (ghost_source(g, c, src, si, sj, sk)) {
+    const uint32_t off = cell_off[src], n = cell_n[src];
+    for (uint32_t q = 0; q < n; q++) {
+      double4 gp;
+      if (ghost_keep(g, r[off + q], si, sj, sk, gp)) ++cnt;
+    }
+  }
+  gcnt[c] = cnt;
+}
+
+__global__ void k_ghost_fill(Geom g, uint32_t N, PState B, const uint32_t* __restrict__ goff, uint32_t* __restrict__ cell_off,
+                             uint32_t* __restrict__ cell_n, uint32_t* __restrict__ slot_cell, uint32_t* __restrict__ ghost_src,
+                             uint32_t* __restrict__ ghost_shift) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncells) return;
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  if (!is_ghost(g, i, j, k)) return;
+  uint32_t src;
+  int si, sj, sk;
+  const uint32_t base = N + goff[c];
+  uint32_t cnt = 0;
+  if (ghost_source(g, c, src, si, sj, sk)) {
+    const uint32_t off = cell_off[src], n = cell_n[src];
+    const uint32_t code = pack_shift(si, sj, sk);
+    for (uint32_t q = 0; q < n; q++) {
+      double4 gp;
+      if (ghost_keep(g, B.r[off + q], si, sj, sk, gp)) {
+        const uint32_t t = base + cnt;
+        B.r[t] = gp;
+        B.vx[t] = B.vx[off + q]; B.vy[t] = B.vy[off + q]; B.vz[t] = B.vz[off + q];
+        B.fx[t] = 0.; B.fy[t] = 0.; B.fz[t] = 0.; B.ep[t] = 0.;
+        B.id[t] = B.id[off + q];
+        slot_cell[t] = c;
+        ghost_src[t - N] = off + q;
+        ghost_shift[t - N] = code;
+        ++cnt;
+      }
+    }
+  }
+  cell_off[c] = base;
+  cell_n[c] = cnt;
+}
+
+// ghost_update_r (update_ghosts.cu:58): re-copy shifted positions along the stored scheme
+__global__ void k_ghost_update_r(Geom g, uint32_t N, uint32_t G, double4* __restrict__ r, const uint32_t* __restrict__ ghost_src,
+                                 const uint32_t* __restrict__ ghost_shift) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= G) return;
+  const uint32_t code = ghost_shift[q];
+  const double4 p = r[ghost_src[q]];
+  double4 o;
+  o.x = shift_coord(p.x, (int)(code & 3u) - 1, g.ox, g.ex);
+  o.y = shift_coord(p.y, (int)((code >> 2) & 3u) - 1, g.oy, g.ey);
+  o.z = shift_coord(p.z, (int)((code >> 4) & 3u) - 1, g.oz, g.ez);
+  o.w = p.w;
+  r[N + q] = o;
+}
+
+// --------------------------------------------------------------- backup / trigger
+
+__global__ void k_backup_r(Geom g, uint32_t N, const double4* __restrict__ r, const uint32_t* __restrict__ slot_cell,
+                           uint32_t* __restrict__ backup) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  int i, j, k;
+  cell_ijk(g, slot_cell[s], i, j, k);
+  const double4 p = r[s];
+  backup[3 * (size_t)s + 0] = encode_double_u32(p.x, cell_origin(g.ox, g.gl, i, g.cs), g.cs);
+  backup[3 * (size_t)s + 1] = encode_double_u32(p.y, cell_origin(g.oy, g.gl, j, g.cs), g.cs);
+  backup[3 * (size_t)s + 2] = encode_double_u32(p.z, cell_origin(g.oz, g.gl, k, g.cs), g.cs);
+}
+
+__device__ inline bool displaced(const Geom& g, double4 p, uint32_t c, const uint32_t* backup, uint32_t s, double thr2) {
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  const double dx = p.x - restore_u32_double(backup[3 * (size_t)s + 0], cell_origin(g.ox, g.gl, i, g.cs), g.cs);
+  const double dy = p.y - restore_u32_double(backup[3 * (size_t)s + 1], cell_origin(g.oy, g.gl, j, g.cs), g.cs);
+  const double dz = p.z - restore_u32_double(backup[3 * (size_t)s + 2], cell_origin(g.oz, g.gl, k, g.cs), g.cs);
+  return ((dx * dx + dy * dy) + dz * dz) >= thr2;
+}
+
+__device__ inline void wave_count_add(bool pred, unsigned long long* counter) {
+  const unsigned long long m = __ballot(pred);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(counter, (unsigned long long)__popcll(m));
+}
+
+// particle_displ_over (particle_displ_over.cu:38-77)
+__global__ void k_displ_over(Geom g, uint32_t N, const double4* __restrict__ r, const uint32_t* __restrict__ slot_cell,
+                             const uint32_t* __restrict__ backup, double thr2, unsigned long long* __restrict__ count) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  bool over = false;
+  if (s < N) over = displaced(g, r[s], slot_cell[s], backup, s, thr2);
+  wave_count_add(over, count);
+}
+
+// -------------------------------------------------------------------- integrators
+
+// push_f_v_r (push_vec3_2nd_order.h:33-38): x += v*dt + f*dt2
+__global__ void k_push_f_v_r(uint32_t N, double4* __restrict__ r, const double* __restrict__ vx, const double* __restrict__ vy,
+                             const double* __restrict__ vz, const double* __restrict__ fx, const double* __restrict__ fy,
+                             const double* __restrict__ fz, double dt, double dt2) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  double4 p = r[s];
+  p.x += vx[s] * dt + fx[s] * dt2;
+  p.y += vy[s] * dt + fy[s] * dt2;
+  p.z += vz[s] * dt + fz[s] * dt2;
+  r[s] = p;
+}
+
+// push_f_v (push_vec3_1st_order.h:33-38): v += f*dt
+__global__ void k_push_f_v(uint32_t N, double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+                           const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz, double dt) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  vx[s] += fx[s] * dt;
+  vy[s] += fy[s] * dt;
+  vz[s] += fz[s] * dt;
+}
+
+// verlet_first_half fused: push_f_v_r(1.0) then push_f_v(0.5), then the
+// particle_displ_over test of the new position (same arithmetic as the separate kernels).
+__global__ void k_verlet_first_half(Geom g, uint32_t N, double4* __restrict__ r, double* __restrict__ vx,
+                                    double* __restrict__ vy, double* __restrict__ vz, const double* __restrict__ fx,
+                                    const double* __restrict__ fy, const double* __restrict__ fz, double dt, double dt2,
+                                    double dth, const uint32_t* __restrict__ slot_cell, const uint32_t* __restrict__ backup,
+                                    double thr2, unsigned long long* __restrict__ count) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  bool over = false;
+  if (s < N) {
+    double4 p = r[s];
+    double ux = vx[s], uy = vy[s], uz = vz[s];
+    const double ax = fx[s], ay = fy[s], az = fz[s];
+    p.x += ux * dt + ax * dt2;
+    p.y += uy * dt + ay * dt2;
+    p.z += uz * dt + az * dt2;
+    r[s] = p;
+    ux += ax * dth;
+    uy += ay * dth;
+    uz += az * dth;
+    vx[s] = ux;
+    vy[s] = uy;
+    vz[s] = uz;
+    over = displaced(g, p, slot_cell[s], backup, s, thr2);
+  }
+  wave_count_add(over, count);
+}
+
+__global__ void k_zero_force(uint32_t n, double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
+                             double* __restrict__ ep) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  fx[s] = 0.;
+  fy[s] = 0.;
+  fz[s] = 0.;
+  ep[s] = 0.;
+}
+
+// ------------------------------------------------------------- chunk_neighbors
+// One thread per atom.  Candidate cells are visited in (k, j, i) order and particles in
+// cell order, which is the ascending (cell_enc, p_b) order the reference obtains by
+// std::sort (chunk_neighbors_execute.h:308), so entries come out already sorted.
+// The cutoff+skin test is FP64 on d = r_a - r_b (chunk_neighbors_execute.h:225-239).
+
+__global__ void __launch_bounds__(256) k_nbh_build(Geom g, uint32_t n_atoms, const double4* __restrict__ r,
+                                                   const uint32_t* __restrict__ slot_cell, const uint32_t* __restrict__ cell_off,
+                                                   const uint32_t* __restrict__ cell_n, double max2, int half_symmetric,
+                                                   int skip_ghosts, uint32_t* __restrict__ nbr, uint32_t* __restrict__ nbr_cnt,
+                                                   uint32_t cap, uint32_t* __restrict__ max_cnt) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_atoms) return;
+  const uint32_t ca = slot_cell[s];
+  int ai, aj, ak;
+  cell_ijk(g, ca, ai, aj, ak);
+  const uint32_t pa = s - cell_off[ca];
+  const double4 ra = r[s];
+  const int i0 = max(ai - 1, 0), i1 = min(ai + 1, g.di - 1);
+  const int j0 = max(aj - 1, 0), j1 = min(aj + 1, g.dj - 1);
+  const int k0 = max(ak - 1, 0), k1 = min(ak + 1, g.dk - 1);
+  uint32_t cnt = 0;
+  uint32_t* L = nbr + nbr_index(s, 0, cap);
+  for (int bk = k0; bk <= k1; bk++)
+    for (int bj = j0; bj <= j1; bj++)
+      for (int bi = i0; bi <= i1; bi++) {
+        const uint32_t cb = cell_index(g, bi, bj, bk);
+        const uint32_t off = cell_off[cb], nb = cell_n[cb];
+        const bool bghost = is_ghost(g, bi, bj, bk);
+        for (uint32_t pb = 0; pb < nb; pb++) {
+          const double4 rb = r[off + pb];
+          const double dx = ra.x - rb.x;
+          const double dy = ra.y - rb.y;
+          const double dz = ra.z - rb.z;
+          const double d2 = (dx * dx + dy * dy) + dz * dz;
+          if ((ca != cb || pa != pb) && d2 > 0.0 && d2 <= max2) {
+            if (half_symmetric && (ca < cb || (ca == cb && pa < pb))) continue;
+            if (skip_ghosts && bghost) continue;
+            if (cnt < cap) L[(size_t)cnt << 6] = off + pb;
+            ++cnt;
+          }
+        }
+      }
+  nbr_cnt[s] = cnt;
+  atomicMax(max_cnt, cnt);
+}
+
+// ------------------------------------------------------------------- LJ forces
+// Plain FP64 form: the reference traversal and functor arithmetic, neighbours in stream
+// order, accepted iff 0 < d2 <= rc^2 with d = r_b - r_a
+// (compute_cell_particle_pairs_impl_default.h:183-186), then lj_compute_energy
+// (lennard_jones.cu:46-56) and f += (de * (1/r)) * d (lennard_jones.cu:117-123).
+// The per-atom sum starts from 0 and is added once to f, like the buffer functor
+// (lennard_jones.cu:80-101).
+
+struct LJConst {
+  double sigma, eps4, m24eps, rc2;
+  float sigma2f, eps4f, m24epsf;
+};
+
+template <bool ACCUM>
+__device__ inline void store_force(uint32_t s, double tx, double ty, double tz, double te, double* __restrict__ fx,
+                                   double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ ep) {
+  if (ACCUM) {
+    fx[s] += tx;
+    fy[s] += ty;
+    fz[s] += tz;
+    ep[s] += te;
+  } else {
+    fx[s] = 0.0 + tx;
+    fy[s] = 0.0 + ty;
+    fz[s] = 0.0 + tz;
+    ep[s] = 0.0 + te;
+  }
+}
+
+// ACCUM: add to existing f (lennard_jones_force operator after zero_particle_force);
+// otherwise f is overwritten with 0 + sum (fused zero_particle_force).
+// PUSH: fused verlet_second_half push_f_v(0.5) on the new force.
+template <bool ACCUM, bool PUSH>
+__global__ void __launch_bounds__(256) k_lj_fp64(uint32_t n_atoms, const double4* __restrict__ r, const uint32_t* __restrict__ nbr,
+                                                 const uint32_t* __restrict__ nbr_cnt, uint32_t cap, LJConst lj,
+                                                 double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
+                                                 double* __restrict__ ep, double* __restrict__ vx, double* __restrict__ vy,
+                                                 double* __restrict__ vz, double dth) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_atoms) return;
+  const double4 ra = r[s];
+  const uint32_t n = nbr_cnt[s];
+  const uint32_t* L = nbr + nbr_index(s, 0, cap);
+  double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
+  for (uint32_t q = 0; q < n; q++) {
+    const double4 rb = r[L[(size_t)q << 6]];
+    const double dx = rb.x - ra.x;
+    const double dy = rb.y - ra.y;
+    const double dz = rb.z - ra.z;
+    const double d2 = (dx * dx + dy * dy) + dz * dz;
+    if (d2 > 0.0 && d2 <= lj.rc2) {
+      const double rr = sqrt(d2);
+      const double inv_r = 1.0 / rr;
+      const double ratio = lj.sigma * inv_r;
+      const double ratio2 = ratio * ratio;
+      const double ratio6 = ratio2 * ratio2 * ratio2;
+      const double ratio12 = ratio6 * ratio6;
+      const double e = lj.eps4 * (ratio12 - ratio6);
+      double de = (lj.m24eps * (2. * ratio12 - ratio6)) * inv_r;
+      de *= inv_r;  // interaction_weight / r with weight 1.0: the same correctly rounded 1.0/r
+      tx += de * dx;
+      ty += de * dy;
+      tz += de * dz;
+      te += e;
+    }
+  }
+  store_force<ACCUM>(s, tx, ty, tz, te, fx, fy, fz, ep);
+  if (PUSH) {
+    vx[s] += fx[s] * dth;
+    vy[s] += fy[s] * dth;
+    vz[s] += fz[s] * dth;
+  }
+}
+
+// Mixed form: identical neighbour walk and FP64 displacement; the cutoff test uses the
+// FP64 d2; the pair coefficient de/r = -24 eps (2 s^12 - s^6) / r^2 is evaluated in FP32
+// from (float)d2 with a hardware reciprocal, and accumulated in FP64 as fma(c, d, f).
+template <bool ACCUM, bool PUSH>
+__global__ void __launch_bounds__(256) k_lj_mixed(uint32_t n_atoms, const double4* __restrict__ r, const uint32_t* __restrict__ nbr,
+                                                  const uint32_t* __restrict__ nbr_cnt, uint32_t cap, LJConst lj,
+                                                  double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
+                                                  double* __restrict__ ep, double* __restrict__ vx, double* __restrict__ vy,
+                                                  double* __restrict__ vz, double dth) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_atoms) return;
+  const double4 ra = r[s];
+  const uint32_t n = nbr_cnt[s];
+  const uint32_t* L = nbr + nbr_index(s, 0, cap);
+  double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
+  for (uint32_t q = 0; q < n; q++) {
+    const double4 rb = r[L[(size_t)q << 6]];
+    const double dx = rb.x - ra.x;
+    const double dy = rb.y - ra.y;
+    const double dz = rb.z - ra.z;
+    const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
+    if (d2 > 0.0 && d2 <= lj.rc2) {
+      const float inv_r2 = __builtin_amdgcn_rcpf((float)d2);
+      const float s2 = lj.sigma2f * inv_r2;
+      const float s6 = s2 * s2 * s2;
+      const float s12 = s6 * s6;
+      const float c = lj.m24epsf * (2.0f * s12 - s6) * inv_r2;
+      const double cd = (double)c;
+      tx = fma(cd, dx, tx);
+      ty = fma(cd, dy, ty);
+      tz = fma(cd, dz, tz);
+      te += (double)(lj.eps4f * (s12 - s6));
+    }
+  }
+  store_force<ACCUM>(s, tx, ty, tz, te, fx, fy, fz, ep);
+  if (PUSH) {
+    vx[s] += fx[s] * dth;
+    vy[s] += fy[s] * dth;
+    vz[s] += fz[s] * dth;
+  }
+}
+
+// Multi-material LJ (compute_pair_multimat.h:100-253): accepted iff d2 <= rc(pair)^2, with
+// the per-atom sums grouped by pair id (each group summed in stream order from 0, then added
+// to f in increasing pair id).  At most MM_PAIRS pair ids (up to 4 types).
+static constexpr int MM_PAIRS = 10;
+struct LJMulti {
+  int npairs;
+  double sigma[MM_PAIRS], eps4[MM_PAIRS], m24eps[MM_PAIRS], rc2[MM_PAIRS];
+  float sigma2f[MM_PAIRS], eps4f[MM_PAIRS], m24epsf[MM_PAIRS];
+};
+
+__device__ inline unsigned unique_pair_id(unsigned a, unsigned b) {
+  if (a > b) {
+    unsigned t = a;
+    a = b;
+    b = t;
+  }
+  return (b * (b + 1)) / 2 + a;
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(256) k_lj_multimat(uint32_t n_atoms, const double4* __restrict__ r, const uint32_t* __restrict__ nbr,
+                                                     const uint32_t* __restrict__ nbr_cnt, uint32_t cap, LJMulti lj,
+                                                     double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
+                                                     double* __restrict__ ep) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_atoms) return;
+  const double4 ra = r[s];
+  const unsigned ta = (unsigned)ra.w;
+  const uint32_t n = nbr_cnt[s];
+  const uint32_t* L = nbr + nbr_index(s, 0, cap);
+  double ax = fx[s], ay = fy[s], az = fz[s], ae = ep[s];
+  // group sums in increasing pair id: only pair ids involving type ta can occur
+  for (int pid = 0; pid < lj.npairs; pid++) {
+    double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
+    bool any = false;
+    for (uint32_t q = 0; q < n; q++) {
+      const uint32_t j = L[(size_t)q << 6];
+      if (j == s) continue;
+      const double4 rb = r[j];
+      const unsigned tb = (unsigned)rb.w;
+      if ((int)unique_pair_id(ta, tb) != pid) continue;
+      const double dx = rb.x - ra.x;
+      const double dy = rb.y - ra.y;
+      const double dz = rb.z - ra.z;
+      if (MIXED) {
+        const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        if (d2 <= lj.rc2[pid]) {
+          any = true;
+          const float inv_r2 = __builtin_amdgcn_rcpf((float)d2);
+          const float s2 = lj.sigma2f[pid] * inv_r2;
+          const float s6 = s2 * s2 * s2;
+          const float s12 = s6 * s6;
+          const double cd = (double)(lj.m24epsf[pid] * (2.0f * s12 - s6) * inv_r2);
+          tx = fma(cd, dx, tx);
+          ty = fma(cd, dy, ty);
+          tz = fma(cd, dz, tz);
+          te += (double)(lj.eps4f[pid] * (s12 - s6));
+        }
+      } else {
+        const double d2 = (dx * dx + dy * dy) + dz * dz;
+        if (d2 <= lj.rc2[pid]) {
+          any = true;
+          const double rr = sqrt(d2);
+          const double inv_r = 1.0 / rr;
+          const double ratio = lj.sigma[pid] * inv_r;
+          const double ratio2 = ratio * ratio;
+          const double ratio6 = ratio2 * ratio2 * ratio2;
+          const double ratio12 = ratio6 * ratio6;
+          const double e = lj.eps4[pid] * (ratio12 - ratio6);
+          double de = (lj.m24eps[pid] * (2. * ratio12 - ratio6)) * inv_r;
+          de *= inv_r;
+          tx += de * dx;
+          ty += de * dy;
+          tz += de * dz;
+          te += e;
+        }
+      }
+    }
+    if (any) {
+      ax += tx;
+      ay += ty;
+      az += tz;
+      ae += te;
+    }
+  }
+  fx[s] = ax;
+  fy[s] = ay;
+  fz[s] = az;
+  ep[s] = ae;
+}
+
+// pair statistics of the current list: list entries and in-cutoff pairs (FP64 test)
+__global__ void k_pair_counts(uint32_t n_atoms, const double4* __restrict__ r, const uint32_t* __restrict__ nbr,
+                              const uint32_t* __restrict__ nbr_cnt, uint32_t cap, double rc2,
+                              unsigned long long* __restrict__ out) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t n = 0, m = 0;
+  if (s < n_atoms) {
+    const double4 ra = r[s];
+    n = nbr_cnt[s];
+    const uint32_t* L = nbr + nbr_index(s, 0, cap);
+    for (uint32_t q = 0; q < n; q++) {
+      const double4 rb = r[L[(size_t)q << 6]];
+      const double dx = rb.x - ra.x, dy = rb.y - ra.y, dz = rb.z - ra.z;
+      const double d2 = (dx * dx + dy * dy) + dz * dz;
+      if (d2 > 0.0 && d2 <= rc2) ++m;
+    }
+  }
+  atomicAdd(&out[0], (unsigned long long)n);
+  atomicAdd(&out[1], (unsigned long long)m);
+}
+
+// ----------------------------------------------------------------- stream export
+// GridChunkNeighbors CS=1 stream (chunk_neighbors_execute.h:265-398), derived from the
+// device neighbour list: per particle [ncg][cell_enc, n, p_b ...]..., optional offset tables.
+
+__device__ inline void rel_of(const Geom& g, uint32_t ca, uint32_t cb, int& ri, int& rj, int& rk) {
+  int ai, aj, ak, bi, bj, bk;
+  cell_ijk(g, ca, ai, aj, ak);
+  cell_ijk(g, cb, bi, bj, bk);
+  ri = bi - ai;
+  rj = bj - aj;
+  rk = bk - ak;
+}
+
+// words of one particle's stream (excluding offset tables)
+__global__ void k_stream_plen(uint32_t n_atoms, const uint32_t* __restrict__ slot_cell, const uint32_t* __restrict__ nbr,
+                              const uint32_t* __restrict__ nbr_cnt, uint32_t cap, int random_access,
+                              uint32_t* __restrict__ plen) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_atoms) return;
+  const uint32_t n = nbr_cnt[s];
+  const uint32_t* L = nbr + nbr_index(s, 0, cap);
+  if (random_access) {
+    plen[s] = 2 * n;
+    return;
+  }
+  uint32_t groups = 0, last = NONE;
+  for (uint32_t q = 0; q < n; q++) {
+    const uint32_t cb = slot_cell[L[(size_t)q << 6]];
+    if (cb != last) {
+      ++groups;
+      last = cb;
+    }
+  }
+  plen[s] = 1 + 2 * groups + n;
+}
+
+// per cell: stream length (tables + particle streams) and per-particle start within cell
+__global__ void k_stream_cell_len(Geom g, uint32_t n_atoms, const uint32_t* __restrict__ cell_off,
+                                  const uint32_t* __restrict__ cell_n, const uint32_t* __restrict__ plen, int ntables,
+                                  uint32_t* __restrict__ pstart, uint32_t* __restrict__ clen) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncells) return;
+  const uint32_t off = cell_off[c], n = cell_n[c];
+  if (n == 0 || off + n > n_atoms) {
+    clen[c] = 0;
+    return;
+  }
+  const uint32_t table = ntables >= 1 ? (n + 1) * 2 + (ntables >= 2 ? n * 2 : 0) : 0;
+  uint32_t acc = 0;
+  for (uint32_t q = 0; q < n; q++) {
+    pstart[off + q] = acc;
+    acc += plen[off + q];
+  }
+  clen[c] = table + acc;
+}
+
+__global__ void k_stream_fill(Geom g, uint32_t n_atoms, const uint32_t* __restrict__ slot_cell, const uint32_t* __restrict__ cell_off,
+                              const uint32_t* __restrict__ cell_n, const uint32_t* __restrict__ nbr,
+                              const uint32_t* __restrict__ nbr_cnt, uint32_t cap, const uint32_t* __restrict__ pstart,
+                              const uint32_t* __restrict__ plen, const uint32_t* __restrict__ coff, int ntables, int random_access,
+                              uint16_t* __restrict__ out) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_atoms) return;
+  const uint32_t ca = slot_cell[s];
+  const uint32_t off = cell_off[ca], na = cell_n[ca], pa = s - off;
+  const uint32_t table = ntables >= 1 ? (na + 1) * 2 + (ntables >= 2 ? na * 2 : 0) : 0;
+  uint16_t* cs = out + coff[ca];
+  uint16_t* st = cs + table + pstart[s];
+  const uint32_t n = nbr_cnt[s];
+  const uint32_t* L = nbr + nbr_index(s, 0, cap);
+  uint32_t w = 0, n_sym = 0;
+  uint32_t cg_idx = 0, ch_idx = 0, last = NONE;
+  if (!random_access) st[w++] = 0;
+  for (uint32_t q = 0; q < n; q++) {
+    const uint32_t j = L[(size_t)q << 6];
+    const uint32_t cb = slot_cell[j];
+    const uint32_t pb = j - cell_off[cb];
+    int ri, rj, rk;
+    rel_of(g, ca, cb, ri, rj, rk);
+    const uint16_t enc = encode_cell_rel(ri, rj, rk);
+    if (cb < ca || (cb == ca && pb < pa)) ++n_sym;
+    if (random_access) {
+      st[w++] = enc;
+      st[w++] = (uint16_t)pb;
+    } else {
+      if (cb != last) {
+        ++cg_idx;
+        last = cb;
+        st[w++] = enc;
+        ch_idx = w;
+        st[w++] = 0;
+      }
+      st[ch_idx] = (uint16_t)(st[ch_idx] + 1);
+      st[w++] = (uint16_t)pb;
+    }
+  }
+  if (!random_access) st[0] = (uint16_t)cg_idx;
+  if (ntables >= 1) {
+    const uint32_t o = pstart[s] + (uint32_t)ntables;
+    cs[pa * 2 + 0] = (uint16_t)o;
+    cs[pa * 2 + 1] = (uint16_t)(o >> 16);
+    if (pa == na - 1) {
+      const uint32_t e = pstart[s] + plen[s] + (uint32_t)ntables;
+      cs[na * 2 + 0] = (uint16_t)e;
+      cs[na * 2 + 1] = (uint16_t)(e >> 16);
+    }
+    if (ntables >= 2) {
+      cs[(na + 1 + pa) * 2 + 0] = (uint16_t)n_sym;
+      cs[(na + 1 + pa) * 2 + 1] = (uint16_t)(n_sym >> 16);
+    }
+  }
+}
+
+// --------------------------------------------------------------------- energies
+// Deterministic two-pass reduction: fixed per-block shuffle tree, then one block sums the
+// block partials in index order.
+static constexpr int RED_THREADS = 256;
+
+__device__ inline double block_sum(double v) {
+  __shared__ double ws[RED_THREADS / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (lane == 0) ws[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < RED_THREADS / 64; i++) t += ws[i];
+  __syncthreads();
+  return t;
+}
+
+__global__ void __launch_bounds__(RED_THREADS) k_energy_partials(uint32_t N, const double* __restrict__ vx, const double* __restrict__ vy,
+                                                                const double* __restrict__ vz, const double* __restrict__ ep,
+                                                                double* __restrict__ part) {
+  double ke = 0.0, pe = 0.0;
+  for (uint32_t s = blockIdx.x * RED_THREADS + threadIdx.x; s < N; s += gridDim.x * RED_THREADS) {
+    ke += 0.5 * ((vx[s] * vx[s] + vy[s] * vy[s]) + vz[s] * vz[s]);
+    pe += 0.5 * ep[s];
+  }
+  const double k = block_sum(ke);
+  const double p = block_sum(pe);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = k;
+    part[2 * blockIdx.x + 1] = p;
+  }
+}
+
+__global__ void k_energy_final(uint32_t nb, const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double k = 0.0, p = 0.0;
+    for (uint32_t b = 0; b < nb; b++) {
+      k += part[2 * b];
+      p += part[2 * b + 1];
+    }
+    out[0] = k;
+    out[1] = p;
+  }
+}
+
+// ---------------------------------------------------------------------- launchers
+
+static inline uint32_t nblk(uint32_t n, uint32_t t) { return n == 0 ? 1 : (n + t - 1) / t; }
+
+void launch_move_flag(const Geom& g, uint32_t N, const double4* r, const uint32_t* slot_cell, uint32_t* mv_dst, uint32_t* cnt_out,
+                      uint32_t* cnt_in, uint32_t* err, hipStream_t st) {
+  k_move_flag<<<nblk(N, 256), 256, 0, st>>>(g, N, r, slot_cell, mv_dst, cnt_out, cnt_in, err);
+}
+void launch_move_pack(const Geom& g, const uint32_t* cell_off, const uint32_t* cell_n, const uint32_t* cnt_out,
+                      const uint32_t* cnt_in, const uint32_t* mv_dst, uint32_t* pk, uint32_t* newloc, uint32_t* new_n,
+                      hipStream_t st) {
+  k_move_pack<<<nblk(g.ncells, 128), 128, 0, st>>>(g, cell_off, cell_n, cnt_out, cnt_in, mv_dst, pk, newloc, new_n);
+}
+void launch_move_scatter(uint32_t N, const PState& A, const PState& B, const uint32_t* slot_cell, const uint32_t* mv_dst,
+                         const uint32_t* newloc, const uint32_t* new_off, const uint32_t* cell_n, const uint32_t* cnt_out,
+                         uint32_t* in_cur, uint32_t* in_list, uint32_t* slot_cell_B, hipStream_t st) {
+  k_move_scatter<<<nblk(N, 256), 256, 0, st>>>(N, A, B, slot_cell, mv_dst, newloc, new_off, cell_n, cnt_out, in_cur, in_list,
+                                               slot_cell_B);
+}
+void launch_move_incoming(const Geom& g, const PState& A, const PState& B, const uint32_t* new_off, const uint32_t* cell_n,
+                          const uint32_t* cnt_out, const uint32_t* cnt_in, uint32_t* in_list, uint32_t* slot_cell_B,
+                          hipStream_t st) {
+  k_move_incoming<<<nblk(g.ncells, 128), 128, 0, st>>>(g, A, B, new_off, cell_n, cnt_out, cnt_in, in_list, slot_cell_B);
+}
+void launch_set_inner_cells(const Geom& g, const uint32_t* new_n, const uint32_t* new_off, uint32_t* cell_n, uint32_t* cell_off,
+                            hipStream_t st) {
+  k_set_inner_cells<<<nblk(g.ncells, 256), 256, 0, st>>>(g.ncells, new_n, new_off, g, cell_n, cell_off);
+}
+void launch_ghost_count(const Geom& g, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, uint32_t* gcnt,
+                        hipStream_t st) {
+  k_ghost_count<<<nblk(g.ncells, 128), 128, 0, st>>>(g, r, cell_off, cell_n, gcnt);
+}
+void launch_ghost_fill(const Geom& g, uint32_t N, const PState& B, const uint32_t* goff, uint32_t* cell_off, uint32_t* cell_n,
+                       uint32_t* slot_cell, uint32_t* ghost_src, uint32_t* ghost_shift, hipStream_t st) {
+  k_ghost_fill<<<nblk(g.ncells, 128), 128, 0, st>>>(g, N, B, goff, cell_off, cell_n, slot_cell, ghost_src, ghost_shift);
+}
+void launch_ghost_update_r(const Geom& g, uint32_t N, uint32_t G, double4* r, const uint32_t* ghost_src,
+                           const uint32_t* ghost_shift, hipStream_t st) {
+  if (G == 0) return;
+  k_ghost_update_r<<<nblk(G, 256), 256, 0, st>>>(g, N, G, r, ghost_src, ghost_shift);
+}
+void launch_backup_r(const Geom& g, uint32_t N, const double4* r, const uint32_t* slot_cell, uint32_t* backup, hipStream_t st) {
+  k_backup_r<<<nblk(N, 256), 256, 0, st>>>(g, N, r, slot_cell, backup);
+}
+void launch_displ_over(const Geom& g, uint32_t N, const double4* r, const uint32_t* slot_cell, const uint32_t* backup, double thr2,
+                       unsigned long long* count, hipStream_t st) {
+  k_displ_over<<<nblk(N, 256), 256, 0, st>>>(g, N, r, slot_cell, backup, thr2, count);
+}
+void launch_push_f_v_r(uint32_t N, const PState& A, double dt, double dt2, hipStream_t st) {
+  k_push_f_v_r<<<nblk(N, 256), 256, 0, st>>>(N, A.r, A.vx, A.vy, A.vz, A.fx, A.fy, A.fz, dt, dt2);
+}
+void launch_push_f_v(uint32_t N, const PState& A, double dt, hipStream_t st) {
+  k_push_f_v<<<nblk(N, 256), 256, 0, st>>>(N, A.vx, A.vy, A.vz, A.fx, A.fy, A.fz, dt);
+}
+void launch_verlet_first_half(const Geom& g, uint32_t N, const PState& A, double dt, double dt2, double dth,
+                              const uint32_t* slot_cell, const uint32_t* backup, double thr2, unsigned long long* count,
+                              hipStream_t st) {
+  k_verlet_first_half<<<nblk(N, 256), 256, 0, st>>>(g, N, A.r, A.vx, A.vy, A.vz, A.fx, A.fy, A.fz, dt, dt2, dth, slot_cell, backup,
+                                                    thr2, count);
+}
+void launch_zero_force(uint32_t n, const PState& A, hipStream_t st) {
+  k_zero_force<<<nblk(n, 256), 256, 0, st>>>(n, A.fx, A.fy, A.fz, A.ep);
+}
+void launch_nbh_build(const Geom& g, uint32_t n_atoms, const double4* r, const uint32_t* slot_cell, const uint32_t* cell_off,
+                      const uint32_t* cell_n, double max2, int half_symmetric, int skip_ghosts, uint32_t* nbr, uint32_t* nbr_cnt,
+                      uint32_t cap, uint32_t* max_cnt, hipStream_t st) {
+  k_nbh_build<<<nblk(n_atoms, 256), 256, 0, st>>>(g, n_atoms, r, slot_cell, cell_off, cell_n, max2, half_symmetric, skip_ghosts,
+                                                  nbr, nbr_cnt, cap, max_cnt);
+}
+
+static LJConst make_lj(double eps, double sigma, double rcut) {
+  LJConst c;
+  c.sigma = sigma;
+  c.eps4 = 4. * eps;
+  c.m24eps = -24. * eps;
+  c.rc2 = rcut * rcut;
+  c.sigma2f = (float)(sigma * sigma);
+  c.eps4f = (float)(4. * eps);
+  c.m24epsf = (float)(-24. * eps);
+  return c;
+}
+
+void launch_lj(int precision, bool accum, bool push, uint32_t n_atoms, const PState& A, const uint32_t* nbr,
+               const uint32_t* nbr_cnt, uint32_t cap, double eps, double sigma, double rcut, double dth, hipStream_t st) {
+  const LJConst lj = make_lj(eps, sigma, rcut);
+  const uint32_t nb = nblk(n_atoms, 256);
+#define XNB_LJ_ARGS n_atoms, A.r, nbr, nbr_cnt, cap, lj, A.fx, A.fy, A.fz, A.ep, A.vx, A.vy, A.vz, dth
+  if (precision == 0) {
+    if (accum) {
+      if (push) k_lj_fp64<true, true><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
+      else k_lj_fp64<true, false><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
+    } else {
+      if (push) k_lj_fp64<false, true><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
+      else k_lj_fp64<false, false><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
+    }
+  } else {
+    if (accum) {
+      if (push) k_lj_mixed<true, true><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
+      else k_lj_mixed<true, false><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
+    } else {
+      if (push) k_lj_mixed<false, true><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
+      else k_lj_mixed<false, false><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
+    }
+  }
+#undef XNB_LJ_ARGS
+}
+
+int launch_lj_multimat(int precision, uint32_t n_atoms, const PState& A, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap,
+                       int ntypes, const double* eps, const double* sigma, const double* rcut, hipStream_t st) {
+  LJMulti m;
+  const int np = ntypes * (ntypes + 1) / 2;
+  if (np > MM_PAIRS) return -1;
+  m.npairs = np;
+  for (int q = 0; q < np; q++) {
+    m.sigma[q] = sigma[q];
+    m.eps4[q] = 4. * eps[q];
+    m.m24eps[q] = -24. * eps[q];
+    m.rc2[q] = rcut[q] * rcut[q];
+    m.sigma2f[q] = (float)(sigma[q] * sigma[q]);
+    m.eps4f[q] = (float)(4. * eps[q]);
+    m.m24epsf[q] = (float)(-24. * eps[q]);
+  }
+  const uint32_t nb = nblk(n_atoms, 256);
+  if (precision == 0) k_lj_multimat<false><<<nb, 256, 0, st>>>(n_atoms, A.r, nbr, nbr_cnt, cap, m, A.fx, A.fy, A.fz, A.ep);
+  else k_lj_multimat<true><<<nb, 256, 0, st>>>(n_atoms, A.r, nbr, nbr_cnt, cap, m, A.fx, A.fy, A.fz, A.ep);
+  return 0;
+}
+
+void launch_pair_counts(uint32_t n_atoms, const double4* r, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap, double rc2,
+                        unsigned long long* out, hipStream_t st) {
+  k_pair_counts<<<nblk(n_atoms, 256), 256, 0, st>>>(n_atoms, r, nbr, nbr_cnt, cap, rc2, out);
+}
+
+void launch_stream_plen(uint32_t n_atoms, const uint32_t* slot_cell, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap,
+                        int random_access, uint32_t* plen, hipStream_t st) {
+  k_stream_plen<<<nblk(n_atoms, 256), 256, 0, st>>>(n_atoms, slot_cell, nbr, nbr_cnt, cap, random_access, plen);
+}
+void launch_stream_cell_len(const Geom& g, uint32_t n_atoms, const uint32_t* cell_off, const uint32_t* cell_n, const uint32_t* plen,
+                            int ntables, uint32_t* pstart, uint32_t* clen, hipStream_t st) {
+  k_stream_cell_len<<<nblk(g.ncells, 128), 128, 0, st>>>(g, n_atoms, cell_off, cell_n, plen, ntables, pstart, clen);
+}
+void launch_stream_fill(const Geom& g, uint32_t n_atoms, const uint32_t* slot_cell, const uint32_t* cell_off, const uint32_t* cell_n,
+                        const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap, const uint32_t* pstart, const uint32_t* plen,
+                        const uint32_t* coff, int ntables, int random_access, uint16_t* out, hipStream_t st) {
+  k_stream_fill<<<nblk(n_atoms, 256), 256, 0, st>>>(g, n_atoms, slot_cell, cell_off, cell_n, nbr, nbr_cnt, cap, pstart, plen, coff,
+                                                    ntables, random_access, out);
+}
+
+void launch_energies(uint32_t N, const PState& A, double* part, uint32_t nparts, double* out, hipStream_t st) {
+  k_energy_partials<<<nparts, RED_THREADS, 0, st>>>(N, A.vx, A.vy, A.vz, A.ep, part);
+  k_energy_final<<<1, 64, 0, st>>>(nparts, part, out);
+}
+
+}  // namespace xnb

@@ -1,0 +1,192 @@
+/*
+ * xnb_amd.h -- C ABI of the MI355X (gfx950) short-range particle hot path:
+ * cell grid with periodic ghost layers, chunk_neighbors CS=1 neighbour stream,
+ * Lennard-Jones pair traversal (FP64 plain form and mixed FP32-pair/FP64-accumulate),
+ * velocity-Verlet push operators and move_particles re-binning.
+ *
+ * Conventions
+ *  - Every entry point returns 0 (XNB_OK) or a negative XNB_E_* code; no C++
+ *    exception crosses this boundary.  xnb_last_error() gives a message.
+ *  - Host arrays are caller-owned and copied; the context owns all device memory
+ *    and one HIP stream (default priority).  Calls are synchronous on return
+ *    unless stated otherwise.  One host thread per context.
+ *  - r, v, f host arrays are n x 3 interleaved doubles.
+ *
+ * Each declaration names the reference operator or function it replaces
+ * (paths relative to the exaNBody source tree).
+ */
+#ifndef XNB_AMD_H
+#define XNB_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XNB_OK 0
+#define XNB_E_ARG (-1)
+#define XNB_E_DOMAIN (-2)
+#define XNB_E_OUTSIDE (-3)
+#define XNB_E_STATE (-4)
+#define XNB_E_HIP (-5)
+#define XNB_E_NOMEM (-6)
+#define XNB_E_OVERFLOW (-7)
+#define XNB_E_AMR (-8)
+#define XNB_E_CAPACITY (-9)
+
+#define XNB_PRECISION_FP64 0  /* reference arithmetic, FP64 throughout */
+#define XNB_PRECISION_MIXED 1 /* FP32 pair arithmetic, FP64 positions, cutoff test and accumulation */
+
+typedef struct xnb_ctx xnb_ctx;
+
+/* Domain (src/core/include/exanb/core/domain.h:36-129): bounds, grid_dims, cell_size,
+ * periodic flags.  Identity xform only. */
+typedef struct {
+  double bounds_min[3];
+  double bounds_max[3];
+  int64_t grid_dims[3];
+  double cell_size;
+  int periodic[3];
+} xnb_domain;
+
+/* LennardJonesParms (contribs/md/lennard_jones/lennard_jones.cu:40-44) */
+typedef struct {
+  double epsilon;
+  double sigma;
+} xnb_lj_params;
+
+/* ChunkNeighborsConfig (src/particle_neighbors/include/exanb/particle_neighbors/
+ * chunk_neighbors_config.h:27-39); chunk_size is always 1 (chunk_neighbors.cpp:62). */
+typedef struct {
+  int build_particle_offset;
+  int dual_particle_offset;
+  int random_access;
+  int half_symmetric;
+  int skip_ghosts;
+} xnb_chunk_neighbors_config;
+
+/* ---- library / context ---------------------------------------------------- */
+int xnb_version(void);
+const char* xnb_error_string(int code);
+const char* xnb_last_error(const xnb_ctx* ctx);
+int xnb_device_count(int* count);
+
+/* domain + init_rcb_grid on one rank (src/grid_cell_particles/init_rcb_grid.cpp:32-84),
+ * check_domain (src/core/lib/domain.cpp:78-85). */
+int xnb_create(xnb_ctx** ctx, const xnb_domain* domain, int device);
+int xnb_destroy(xnb_ctx* ctx);
+
+/* nbh_dist (src/particle_neighbors/nbh_dist.cpp:45-72): sets nbh_dist_lab = rcut_max+rcut_inc,
+ * max_displ = rcut_inc/2, ghost_dist = rcut_max+rcut_inc, and the grid's ghost layers
+ * ceil(ghost_dist/cell_size) (src/core/include/exanb/core/grid.h:110). */
+int xnb_nbh_dist(xnb_ctx* ctx, double rcut_max, double rcut_inc, double* nbh_dist_lab,
+                 double* max_displ, double* ghost_dist);
+
+/* Particle input (replaces a reader or lattice operator followed by the first
+ * move_particles binning): particles are appended, in input order, to the cell given by
+ * domain_periodic_location (src/core/include/exanb/core/domain.h:160-191).
+ * v, id, type may be NULL (zero velocity, id = index, type 0). */
+int xnb_set_particles(xnb_ctx* ctx, size_t n, const double* r, const double* v,
+                      const uint64_t* id, const uint8_t* type);
+int xnb_num_particles(const xnb_ctx* ctx, size_t* n_inner, size_t* n_ghost);
+int xnb_grid_info(const xnb_ctx* ctx, int64_t dims[3], int* ghost_layers, size_t* n_cells);
+/* Particles in cell order (ghost cells too if include_ghosts); any pointer may be NULL. */
+int xnb_get_particles(xnb_ctx* ctx, int include_ghosts, uint32_t* cell, double* r, double* v,
+                      double* f, double* ep, uint64_t* id, uint8_t* type);
+int xnb_get_cell_counts(xnb_ctx* ctx, uint32_t* counts);
+
+/* ---- operators (YAML names of the reference) ------------------------------ */
+/* move_particles (src/grid_cell_particles/move_particles.cpp:39-47,
+ * include/exanb/grid_cell_particles/move_particles_across_cells.h:78-235) */
+int xnb_move_particles(xnb_ctx* ctx);
+/* backup_r (src/io/backup_r.cpp:35-116) */
+int xnb_backup_r(xnb_ctx* ctx);
+/* ghost_comm_scheme + ghost_update_all (src/mpi/update_ghosts_comm_scheme.cpp:84-489,
+ * src/mpi/update_ghosts.cu:45-64) on one rank: periodic self images */
+int xnb_ghost_update_all(xnb_ctx* ctx);
+/* ghost_update_r (src/mpi/update_ghosts.cu:58) */
+int xnb_ghost_update_r(xnb_ctx* ctx);
+/* particle_displ_over (src/mpi/particle_displ_over.cu:38-218): count of particles moved
+ * at least threshold since backup_r */
+int xnb_particle_displ_over(xnb_ctx* ctx, double threshold, uint64_t* count);
+/* chunk_neighbors (src/particle_neighbors/chunk_neighbors.cpp:47-90,
+ * include/exanb/particle_neighbors/chunk_neighbors_execute.h:40-423), chunk size 1.
+ * all_cells != 0 also builds lists of ghost cells (the reference builds every cell). */
+int xnb_chunk_neighbors(xnb_ctx* ctx, double nbh_dist_lab, const xnb_chunk_neighbors_config* cfg,
+                        int all_cells);
+/* GridChunkNeighbors stream export (chunk_neighbors.h:42-120; Appendix-A format): per-cell
+ * uint16 streams concatenated; cell_offsets[n_cells+1] in uint16 words.  out may be NULL to
+ * query *total. */
+int xnb_chunk_neighbors_stream(xnb_ctx* ctx, uint16_t* out, size_t cap, uint64_t* cell_offsets,
+                               size_t* total, uint32_t* max_neighbors);
+/* zero_particle_force (src/compute/zero_particle_force.cu:19-55) */
+int xnb_zero_particle_force(xnb_ctx* ctx, int ghost);
+/* lennard_jones_force (contribs/md/lennard_jones/lennard_jones.cu:171-223) over the
+ * compute_cell_particle_pairs traversal (src/compute/include/exanb/compute/
+ * compute_cell_particle_pairs.h:119-243); adds to f and to per-atom energy ep. */
+int xnb_lennard_jones_force(xnb_ctx* ctx, const xnb_lj_params* params, double rcut, int precision,
+                            int ghost);
+/* Multi-material LJ over compute_pair_multimat (src/compute/include/exanb/compute/
+ * compute_pair_multimat.h:256-316): per unique type pair (particle_type_pair.h:39-55)
+ * epsilon, sigma, rcut arrays of ntypes*(ntypes+1)/2 entries. */
+int xnb_lennard_jones_force_multimat(xnb_ctx* ctx, int ntypes, const double* epsilon,
+                                     const double* sigma, const double* rcut, int precision);
+/* push_f_v_r (src/defbox/include/exanb/defbox/push_vec3_2nd_order.h:29-119) */
+int xnb_push_f_v_r(xnb_ctx* ctx, double dt, double dt_scale);
+/* push_f_v (src/defbox/include/exanb/defbox/push_vec3_1st_order.h:29-108) */
+int xnb_push_f_v(xnb_ctx* ctx, double dt, double dt_scale);
+/* kinetic energy (mass 1, src/compute/simulation_stats.cpp:82-101) and PE = 1/2 sum ep */
+int xnb_simulation_energies(xnb_ctx* ctx, double* ke, double* pe);
+
+/* ---- composite step loop --------------------------------------------------- */
+/* first_iteration (data/config/compute-loop.msp:1-7): move_particles, backup_r, ghosts,
+ * chunk_neighbors, zero_particle_force + lennard_jones_force. */
+int xnb_init(xnb_ctx* ctx, const xnb_lj_params* params, double rcut, double rcut_inc, int precision);
+
+typedef struct {
+  double dt;
+  xnb_lj_params lj;
+  double rcut;
+  double rcut_inc;
+  int precision;
+  int rebuild_every; /* <= 0: displacement trigger (reference); > 0: fixed interval */
+  int64_t steps;
+  /* outputs */
+  int64_t n_rebuilds;
+  int64_t n_violations; /* interval mode: steps with an atom past max_displ and no rebuild */
+} xnb_run_params;
+/* numerical_scheme loop (data/config/numerical-scheme.msp:8-25, update-particles.msp:42-79):
+ * push_f_v_r, push_f_v, trigger, full update or ghost_update_r, forces, push_f_v. */
+int xnb_run(xnb_ctx* ctx, xnb_run_params* params);
+
+/* ---- measurement ----------------------------------------------------------- */
+#define XNB_TIMER_FORCE 0
+#define XNB_TIMER_NBH 1
+#define XNB_TIMER_PUSH 2
+#define XNB_TIMER_GHOST 3
+#define XNB_TIMER_REBIN 4
+#define XNB_TIMER_TRIGGER 5
+#define XNB_TIMER_COUNT 6
+/* HIP-event timing of each kernel class on the context's stream. */
+int xnb_timer_enable(xnb_ctx* ctx, int enable);
+int xnb_timer_reset(xnb_ctx* ctx);
+int xnb_timer_read(xnb_ctx* ctx, int timer, double* total_ms, int64_t* launches);
+/* Pairs seen by the last force call: list entries and in-cutoff pairs. */
+int xnb_pair_counts(xnb_ctx* ctx, uint64_t* list_entries, uint64_t* cutoff_pairs);
+int xnb_synchronize(xnb_ctx* ctx);
+
+/* ---- input generation (host only, no GPU call) ------------------------------ */
+/* FCC lattice of ncell^3 unit cells at reduced density rho, first n_atoms sites in lattice
+ * order (n_atoms <= 4 ncell^3; pass 4 ncell^3 for a full lattice), each coordinate jittered
+ * uniformly in [-jitter, jitter], Maxwell-Boltzmann velocities at temperature T (mass 1,
+ * zero net momentum, exact rescale with 3N-3 degrees of freedom), from a counter-based
+ * SplitMix64 stream keyed by seed.  Box edge L = ncell * a, a = (4/rho)^(1/3). */
+int xnb_generate_fcc(int64_t ncell, double rho, int64_t n_atoms, double jitter, double temperature,
+                     uint64_t seed, double* r, double* v, uint64_t* id, double* box_length);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

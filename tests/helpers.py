@@ -1,0 +1,44 @@
+"""Shared test helpers: paired oracle / GPU boxes built from the same seeded inputs."""
+import numpy as np
+
+from exanbody_amd import Box, cubic_domain, generate_fcc
+from oracle.oracle import Oracle
+
+RHO = 0.8442
+RCUT = 2.5
+SKIN = 0.3
+
+
+def fcc_inputs(ncell=8, rho=RHO, jitter=0.01, temperature=0.72, seed=1, n_atoms=0):
+    r, v, ids, L = generate_fcc(ncell, rho, n_atoms=n_atoms, jitter=jitter, temperature=temperature, seed=seed)
+    return r, v, ids, L
+
+
+def make_pair(r, v, ids, L, types=None, min_cell=RCUT + SKIN, rcut=RCUT, skin=SKIN, gpu=True, periodic=(1, 1, 1)):
+    """Oracle and (optionally) GPU box over the same particles; nbh_dist applied to both."""
+    dom = cubic_domain(L, min_cell, periodic=periodic)
+    o = Oracle(dom["bounds_min"], dom["bounds_max"], dom["grid_dims"], dom["cell_size"], periodic=periodic,
+               ghost_dist=rcut + skin)
+    o.set_particles(r, v, ids, types)
+    b = None
+    if gpu:
+        b = Box(dom["bounds_min"], dom["bounds_max"], dom["grid_dims"], dom["cell_size"], periodic=periodic)
+        b.nbh_dist(rcut, skin)
+        b.set_particles(r, v, ids, types)
+    return o, b, dom
+
+
+def assert_same_layout(o_exp, g_exp, what=""):
+    """Same particles in the same cells and the same within-cell order (bitwise positions)."""
+    assert o_exp["cell"].shape == g_exp["cell"].shape, f"{what}: particle counts differ"
+    np.testing.assert_array_equal(o_exp["cell"], g_exp["cell"], err_msg=f"{what}: cell of each slot")
+    np.testing.assert_array_equal(o_exp["id"], g_exp["id"], err_msg=f"{what}: ids in cell order")
+    np.testing.assert_array_equal(o_exp["r"].view(np.uint64), g_exp["r"].view(np.uint64),
+                                  err_msg=f"{what}: positions (bitwise)")
+
+
+def rel_err(a, b):
+    """max and RMS of |a-b| per atom relative to the RMS norm of b."""
+    d = np.linalg.norm(a - b, axis=1)
+    ref = np.sqrt(np.mean(np.sum(b * b, axis=1)))
+    return d.max() / ref, np.sqrt(np.mean(d * d)) / ref

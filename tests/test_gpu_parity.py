@@ -1,0 +1,75 @@
+"""GPU (HIP library) vs CPU oracle on the same seeded inputs.
+
+Bar: bit-exact for cell layout, ids, ghost images, neighbour streams and the FP64 trajectory;
+mixed precision within the stated force tolerance.
+"""
+import numpy as np
+import pytest
+
+from exanbody_amd import PRECISION_FP64, PRECISION_MIXED
+from tests.helpers import RCUT, SKIN, assert_same_layout, fcc_inputs, make_pair, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _init_both(ncell=8, **kw):
+    r, v, ids, L = fcc_inputs(ncell, **kw)
+    o, b, dom = make_pair(r, v, ids, L)
+    o.init(1.0, 1.0, RCUT, SKIN)
+    b.init(1.0, 1.0, RCUT, SKIN, PRECISION_FP64)
+    return o, b
+
+
+def test_init_layout_ghosts_streams_forces_bitwise():
+    o, b = _init_both(8)
+    assert_same_layout(o.export(), b.export(), "inner")
+    og, gg = o.export(ghosts=True), b.export(ghosts=True)
+    assert_same_layout(og, gg, "inner+ghost")
+    # neighbour streams of inner cells: bit-exact CS=1 format
+    gs, gmax = b.streams()
+    os_ = o.streams()
+    dims, gl, nc = b.grid_info()
+    counts = b.cell_counts()
+    assert gmax == 78  # FCC at rho=0.8442: exactly 78 neighbours within rc+skin
+    checked = 0
+    for c in range(nc):
+        k, rem = divmod(c, dims[0] * dims[1])
+        j, i = divmod(rem, dims[0])
+        inner = all(gl <= x < d - gl for x, d in zip((i, j, k), dims))
+        if not inner:
+            continue
+        np.testing.assert_array_equal(gs[c], os_[c], err_msg=f"stream of cell {c}")
+        checked += counts[c]
+    assert checked == 2048
+    oe, ge = o.export(), b.export()
+    np.testing.assert_array_equal(oe["f"].view(np.uint64), ge["f"].view(np.uint64), err_msg="FP64 forces bitwise")
+    np.testing.assert_array_equal(oe["ep"].view(np.uint64), ge["ep"].view(np.uint64), err_msg="FP64 energies bitwise")
+
+
+@pytest.mark.parametrize("rebuild_every", [0, 10])
+def test_fp64_trajectory_bitwise(rebuild_every):
+    o, b = _init_both(8, temperature=1.5, jitter=0.05, seed=7)
+    steps = 40
+    ro = o.run(steps, 0.005, 1.0, 1.0, RCUT, SKIN, rebuild_every=rebuild_every)
+    rg = b.run(steps, 0.005, 1.0, 1.0, RCUT, SKIN, precision=PRECISION_FP64, rebuild_every=rebuild_every)
+    assert ro == rg
+    oe, ge = o.export(), b.export()
+    assert_same_layout(oe, ge, "after run")
+    for k in ("v", "f"):
+        np.testing.assert_array_equal(oe[k].view(np.uint64), ge[k].view(np.uint64), err_msg=k)
+    ko, po = o.energies()
+    kg, pg = b.energies()
+    assert abs(ko - kg) <= 1e-12 * abs(ko) and abs(po - pg) <= 1e-12 * abs(po)
+
+
+def test_mixed_force_error():
+    o, b = _init_both(8, temperature=0.72, seed=3)
+    b.run(50, 0.005, 1.0, 1.0, RCUT, SKIN, precision=PRECISION_FP64, rebuild_every=10)
+    ref = b.export()
+    b.zero_particle_force()
+    b.lennard_jones_force(1.0, 1.0, RCUT, precision=PRECISION_MIXED)
+    mix = b.export()
+    mx, rms = rel_err(mix["f"], ref["f"])
+    assert rms <= 1e-5, (mx, rms)
+    pe_rel = np.abs(mix["ep"] - ref["ep"]).max() / np.abs(ref["ep"]).max()
+    assert pe_rel <= 1e-5
