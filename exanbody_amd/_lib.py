@@ -82,6 +82,7 @@ _SIGS = {
     "xnb_timer_read": (C.c_int, [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "xnb_pair_counts": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "xnb_synchronize": (C.c_int, [P]),
+    "xnb_copy_probe": (C.c_int, [P, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "xnb_generate_fcc": (C.c_int, [C.c_int64, C.c_double, C.c_int64, C.c_double, C.c_double, C.c_uint64, P, P, P,
                                    C.POINTER(C.c_double)]),
 }

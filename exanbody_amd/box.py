@@ -204,5 +204,10 @@ class Box:
         self._check(self.L.xnb_pair_counts(self._h, C.byref(a), C.byref(b)), "pair_counts")
         return a.value, b.value
 
+    def copy_probe(self, nbytes, iters=10):
+        g = C.c_double()
+        self._check(self.L.xnb_copy_probe(self._h, int(nbytes), int(iters), C.byref(g)), "copy_probe")
+        return g.value
+
     def synchronize(self):
         self._check(self.L.xnb_synchronize(self._h), "synchronize")

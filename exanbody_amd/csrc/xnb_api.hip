@@ -947,6 +947,37 @@ int xnb_pair_counts(xnb_ctx* c, uint64_t* list_entries, uint64_t* cutoff_pairs) 
   return XNB_OK;
 }
 
+int xnb_copy_probe(xnb_ctx* c, size_t bytes, int iters, double* gbps) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (bytes < 16 || iters <= 0 || !gbps) return fail(c, XNB_E_ARG, "bad copy probe arguments");
+  bytes &= ~(size_t)15;
+  void *a = nullptr, *b = nullptr;
+  HIPCHK(c, hipMalloc(&a, bytes));
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    hipFree(a);
+    return fail(c, XNB_E_NOMEM, "copy probe allocation");
+  }
+  hipMemsetAsync(a, 1, bytes, c->st);
+  hipEvent_t t0, t1;
+  hipEventCreate(&t0);
+  hipEventCreate(&t1);
+  launch_copy_probe(a, b, bytes, c->st);  // warm-up
+  hipEventRecord(t0, c->st);
+  for (int q = 0; q < iters; q++) launch_copy_probe(a, b, bytes, c->st);
+  hipEventRecord(t1, c->st);
+  hipError_t he = hipEventSynchronize(t1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, t0, t1);
+  hipEventDestroy(t0);
+  hipEventDestroy(t1);
+  hipFree(a);
+  hipFree(b);
+  if (he != hipSuccess) return fail(c, XNB_E_HIP, "copy probe failed");
+  *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+  return XNB_OK;
+}
+
 int xnb_synchronize(xnb_ctx* c) {
   int e;
   if ((e = check_ctx(c))) return e;

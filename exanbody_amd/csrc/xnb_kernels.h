@@ -64,6 +64,7 @@ void launch_stream_cell_len(const Geom& g, uint32_t n_atoms, const uint32_t* cel
 void launch_stream_fill(const Geom& g, uint32_t n_atoms, const uint32_t* slot_cell, const uint32_t* cell_off, const uint32_t* cell_n,
                         const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap, const uint32_t* pstart, const uint32_t* plen,
                         const uint32_t* coff, int ntables, int random_access, uint16_t* out, hipStream_t st);
+void launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t st);
 void launch_energies(uint32_t N, const PState& A, double* part, uint32_t nparts, double* out, hipStream_t st);
 
 }  // namespace xnb

@@ -881,6 +881,18 @@ __global__ void k_energy_final(uint32_t nb, const double* __restrict__ part, dou
   }
 }
 
+// --------------------------------------------------------------------- copy probe
+// Streams n 16-byte words from src to dst: the reference copy kernel used to calibrate HBM
+// traffic counters (bytes read = bytes written = 16 n).
+__global__ void __launch_bounds__(256) k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+  for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) dst[q] = src[q];
+}
+
+void launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t st) {
+  const size_t n = bytes / 16;
+  k_copy_probe<<<4096, 256, 0, st>>>((const uint4*)src, (uint4*)dst, n);
+}
+
 // ---------------------------------------------------------------------- launchers
 
 static inline uint32_t nblk(uint32_t n, uint32_t t) { return n == 0 ? 1 : (n + t - 1) / t; }

@@ -176,6 +176,9 @@ int xnb_timer_read(xnb_ctx* ctx, int timer, double* total_ms, int64_t* launches)
 /* Pairs seen by the last force call: list entries and in-cutoff pairs. */
 int xnb_pair_counts(xnb_ctx* ctx, uint64_t* list_entries, uint64_t* cutoff_pairs);
 int xnb_synchronize(xnb_ctx* ctx);
+/* Device copy of `bytes` (read + write), iters times, timed with HIP events: achieved GB/s
+ * counting read and write bytes.  Also the known-size kernel that calibrates HBM counters. */
+int xnb_copy_probe(xnb_ctx* ctx, size_t bytes, int iters, double* gbps);
 
 /* ---- input generation (host only, no GPU call) ------------------------------ */
 /* FCC lattice of ncell^3 unit cells at reduced density rho, first n_atoms sites in lattice

@@ -1,0 +1,91 @@
+"""CPU checks of the C ABI library: it loads, exports every function include/xnb_amd.h declares,
+and its host-only generator has the stated properties.  No GPU compute call is made."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import exanbody_amd
+from exanbody_amd import _lib, generate_fcc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "xnb_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(xnb_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(_lib.LIB_PATH)
+    names = header_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the ctypes table covers the header exactly
+    assert sorted(_lib.exported_symbols()) == names
+
+
+def test_package_refuses_to_run_without_library(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
+    with pytest.raises(ImportError):
+        _lib.load()
+
+
+def test_error_strings_and_no_device_path():
+    L = exanbody_amd.load()
+    assert L.xnb_error_string(0) == b"ok"
+    assert L.xnb_error_string(_lib.XNB_E_OVERFLOW) == b"neighbour list overflow"
+    if exanbody_amd.device_count() == 0:
+        d = _lib.Domain()
+        for a in range(3):
+            d.bounds_max[a] = 9.0
+            d.grid_dims[a] = 3
+            d.periodic[a] = 1
+        d.cell_size = 3.0
+        h = C.c_void_p()
+        assert L.xnb_create(C.byref(h), C.byref(d), 0) == _lib.XNB_E_HIP
+        assert not h.value
+
+
+def test_domain_validation_before_device():
+    L = exanbody_amd.load()
+    d = _lib.Domain()
+    for a in range(3):
+        d.bounds_max[a] = 10.0
+        d.grid_dims[a] = 3
+    d.cell_size = 3.0  # 3 * 3 != 10: check_domain fails before any device call
+    h = C.c_void_p()
+    assert L.xnb_create(C.byref(h), C.byref(d), 0) == _lib.XNB_E_DOMAIN
+
+
+def test_generator_properties():
+    r, v, ids, L = generate_fcc(6, 0.8442, jitter=0.01, temperature=0.72, seed=5)
+    n = len(r)
+    assert n == 4 * 6 ** 3
+    assert L == pytest.approx(6 * (4 / 0.8442) ** (1 / 3), rel=1e-15)
+    assert np.all(r >= 0) and np.all(r < L)
+    a = L / 6
+    base = np.array([[0, 0, 0], [.5, .5, 0], [.5, 0, .5], [0, .5, .5]])
+    g = np.stack(np.meshgrid(np.arange(6), np.arange(6), np.arange(6), indexing="ij"), -1).reshape(-1, 3)[:, ::-1]
+    sites = ((g[:, None, :] + base[None]) * a).reshape(-1, 3)
+    d = r - sites
+    d -= L * np.round(d / L)
+    assert np.abs(d).max() <= 0.01 + 1e-12
+    assert np.abs(v.sum(0)).max() < 1e-12
+    assert (v * v).sum() / (3 * n - 3) == pytest.approx(0.72, rel=1e-13)
+    assert np.array_equal(ids, np.arange(n))
+    r2, v2, _, _ = generate_fcc(6, 0.8442, jitter=0.01, temperature=0.72, seed=5)
+    assert np.array_equal(r, r2) and np.array_equal(v, v2)
+    r3, _, _, _ = generate_fcc(6, 0.8442, jitter=0.01, temperature=0.72, seed=6)
+    assert not np.array_equal(r, r3)
+
+
+def test_generator_partial_fill_density():
+    r, v, ids, L = generate_fcc(5, 0.8, n_atoms=300, jitter=0.0, temperature=1.0, seed=1)
+    assert len(r) == 300 and L ** 3 * 0.8 == pytest.approx(300, rel=1e-12)
+    assert len({tuple(x) for x in np.round(r, 9)}) == 300
