@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -53,7 +54,7 @@ struct xnb_ctx {
   DArr<uint32_t> ghost_src, ghost_shift, backup;
   // neighbour lists
   DArr<uint32_t> nbr, nbr_cnt;
-  uint32_t nbr_cap = 128;
+  uint32_t nbr_cap = 96;
   uint32_t list_atoms = 0;
   uint32_t max_neighbors = 0;
   xnb_chunk_neighbors_config list_cfg{1, 0, 0, 0, 0};
@@ -76,6 +77,16 @@ struct xnb_ctx {
   int64_t t_n[XNB_TIMER_COUNT] = {};
   int t_cur = -1;
   hipEvent_t t_a = nullptr;
+
+  // brick-tiled production lists (xnb_brick.hip)
+  int list_kind = 0;  // 0: generic per-slot u32 lists in nbr; 1: brick u16 lists in blist
+  bool use_bricks = true;
+  BrickParams bp{};
+  int brick_dims[3] = {0, 0, 0};  // 0: choose automatically
+  DArr<uint16_t> blist;
+  DArr<uint32_t> bcnt;
+  bool slot_lists_current = false;  // nbr/nbr_cnt mirror the brick lists
+  uint32_t lds_limit = 65536;
 
   double last_rcut = 0.;
   std::string err;
@@ -346,17 +357,103 @@ int op_backup(xnb_ctx* c) {
   return XNB_OK;
 }
 
-int op_nbh(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& cfg, int all_cells) {
-  const uint32_t n_atoms = all_cells ? c->N + c->G : c->N;
+constexpr int NOT_APPLICABLE = 1;
+
+inline uint32_t list_granule() { return std::max<uint32_t>(16u, std::max(lj_list_granule(), brick_list_granule())); }
+inline uint32_t grow_cap(uint32_t mx) {
+  const uint32_t gran = list_granule();  // force kernels load rows in batches of the granule
+  return (mx + mx / 8 + gran - 1) / gran * gran;
+}
+
+// Brick dims, T and Rmax for the current cell occupancy; NOT_APPLICABLE if nothing fits LDS.
+int choose_bricks(xnb_ctx* c, uint32_t cap, BrickParams& out) {
+  static const int cand[][3] = {{4, 2, 2}, {2, 2, 2}, {4, 2, 1}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
+  const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
+  int e;
+  for (int q = -1; q < ncand; q++) {
+    int dims[3];
+    if (q < 0) {
+      if (c->brick_dims[0] <= 0) continue;
+      dims[0] = c->brick_dims[0];
+      dims[1] = c->brick_dims[1];
+      dims[2] = c->brick_dims[2];
+    } else {
+      dims[0] = cand[q][0];
+      dims[1] = cand[q][1];
+      dims[2] = cand[q][2];
+    }
+    BrickParams bp = brick_params(c->g, dims[0], dims[1], dims[2]);
+    HIPCHK(c, hipMemsetAsync(c->u32s.p + 3, 0, 2 * sizeof(uint32_t), c->st));
+    launch_brick_stats(c->g, bp, c->cell_n.p, c->u32s.p + 3, c->st);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->h_u32, c->u32s.p + 3, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const uint32_t nb_max = c->h_u32[0], r_max = c->h_u32[1];
+    uint32_t T = std::max<uint32_t>({nb_max, (uint32_t)bp.nrc, (uint32_t)bp.nbc, 64u});
+    T = (T + 63) / 64 * 64;
+    if (T > 1024 || r_max >= 65535u) continue;
+    bp.T = T;
+    bp.Rmax = std::max<uint32_t>(r_max, 1u);
+    bp.cap = cap;
+    if (brick_layout(bp, true) > c->lds_limit) continue;
+    out = bp;
+    return XNB_OK;
+  }
+  (void)e;
+  return NOT_APPLICABLE;
+}
+
+int op_nbh_brick(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& cfg) {
   const double max2 = nbh_dist * nbh_dist;
   int e;
+  uint32_t cap = (c->nbr_cap + list_granule() - 1) / list_granule() * list_granule();
+  for (int attempt = 0; attempt < 3; attempt++) {
+    BrickParams bp;
+    if ((e = choose_bricks(c, cap, bp))) return e;
+    if ((e = dalloc(c, c->blist, (size_t)bp.nbricks * bp.cap * bp.T))) return e;
+    if ((e = dalloc(c, c->bcnt, (size_t)bp.nbricks * bp.T))) return e;
+    HIPCHK(c, hipMemsetAsync(c->u32s.p + 2, 0, sizeof(uint32_t), c->st));
+    tstart(c, XNB_TIMER_NBH);
+    hipError_t he = launch_nbh_brick(c->g, bp, c->A.r, c->cell_off.p, c->cell_n.p, max2, cfg.half_symmetric, cfg.skip_ghosts,
+                                     c->blist.p, c->bcnt.p, c->u32s.p + 2, c->st);
+    tstop(c);
+    if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick neighbour build: ") + hipGetErrorString(he));
+    uint32_t mx = 0;
+    if ((e = read_u32(c, c->u32s.p + 2, &mx))) return e;
+    c->max_neighbors = mx;
+    if (mx <= bp.cap) {
+      c->bp = bp;
+      c->nbr_cap = bp.cap;
+      c->list_kind = 1;
+      c->slot_lists_current = false;
+      c->list_atoms = c->N;
+      c->list_cfg = cfg;
+      c->lists_valid = true;
+      return XNB_OK;
+    }
+    cap = grow_cap(mx);
+  }
+  return NOT_APPLICABLE;
+}
+
+int op_nbh(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& cfg, int all_cells) {
+  const int lay = (int)std::ceil(nbh_dist / c->g.cs);  // cell_layers (amr_grid_algorithm.h:451)
+  if (lay > 15) return fail(c, XNB_E_ARG, "neighbour distance spans more than 15 cells");
+  int e;
+  if (!all_cells && lay == 1 && c->g.gl >= 1 && c->use_bricks && c->N > 0) {
+    e = op_nbh_brick(c, nbh_dist, cfg);
+    if (e != NOT_APPLICABLE) return e;
+  }
+  const uint32_t n_atoms = all_cells ? c->N + c->G : c->N;
+  const double max2 = nbh_dist * nbh_dist;
+  c->nbr_cap = (c->nbr_cap + list_granule() - 1) / list_granule() * list_granule();
   for (int attempt = 0; attempt < 3; attempt++) {
     const size_t rows = ((size_t)n_atoms + 63) / 64 * 64;
     if ((e = dalloc(c, c->nbr, rows * c->nbr_cap))) return e;
     if ((e = dalloc(c, c->nbr_cnt, rows))) return e;
     HIPCHK(c, hipMemsetAsync(c->u32s.p + 2, 0, sizeof(uint32_t), c->st));
     tstart(c, XNB_TIMER_NBH);
-    launch_nbh_build(c->g, n_atoms, c->A.r, c->slot_cell_A.p, c->cell_off.p, c->cell_n.p, max2, cfg.half_symmetric,
+    launch_nbh_build(c->g, n_atoms, c->A.r, c->slot_cell_A.p, c->cell_off.p, c->cell_n.p, max2, lay, cfg.half_symmetric,
                      cfg.skip_ghosts, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, c->u32s.p + 2, c->st);
     tstop(c);
     HIPCHK(c, hipGetLastError());
@@ -364,14 +461,46 @@ int op_nbh(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& cfg, i
     if ((e = read_u32(c, c->u32s.p + 2, &mx))) return e;
     c->max_neighbors = mx;
     if (mx <= c->nbr_cap) {
+      c->list_kind = 0;
+      c->slot_lists_current = true;
       c->list_atoms = n_atoms;
       c->list_cfg = cfg;
       c->lists_valid = true;
       return XNB_OK;
     }
-    c->nbr_cap = (uint32_t)((mx + mx / 8 + 15) & ~15u);
+    c->nbr_cap = grow_cap(mx);
   }
   return fail(c, XNB_E_OVERFLOW, "neighbour list capacity");
+}
+
+// Generic per-slot lists (nbr, nbr_cnt) for the stream export and generic traversals.
+int ensure_slot_lists(xnb_ctx* c) {
+  if (c->list_kind == 0 || c->slot_lists_current) return XNB_OK;
+  const size_t rows = ((size_t)c->N + 63) / 64 * 64;
+  int e;
+  if ((e = dalloc(c, c->nbr, rows * c->bp.cap))) return e;
+  if ((e = dalloc(c, c->nbr_cnt, rows))) return e;
+  hipError_t he = launch_brick_to_slots(c->g, c->bp, c->cell_off.p, c->cell_n.p, c->blist.p, c->bcnt.p, c->nbr.p, c->nbr_cnt.p,
+                                        c->bp.cap, c->st);
+  if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick list conversion: ") + hipGetErrorString(he));
+  c->nbr_cap = c->bp.cap;
+  c->slot_lists_current = true;
+  return XNB_OK;
+}
+
+// LJ force over the current lists: brick kernel for brick lists, generic kernel otherwise.
+int launch_force(xnb_ctx* c, int precision, bool accum, bool push, bool ghost, double eps, double sigma, double rcut, double dth) {
+  if (c->list_kind == 1 && !ghost) {
+    hipError_t he = launch_lj_brick(precision, accum, push, c->g, c->bp, c->A, c->cell_off.p, c->cell_n.p, c->blist.p, c->bcnt.p, eps,
+                                    sigma, rcut, dth, c->st);
+    if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick LJ kernel: ") + hipGetErrorString(he));
+    return XNB_OK;
+  }
+  const uint32_t n_atoms = ghost ? c->N + c->G : c->N;
+  if (c->list_kind != 0 || c->list_atoms < n_atoms) return fail(c, XNB_E_STATE, "neighbour lists do not cover the requested cells");
+  launch_lj(precision, accum, push, n_atoms, c->A, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, eps, sigma, rcut, dth, c->st);
+  HIPCHK(c, hipGetLastError());
+  return XNB_OK;
 }
 
 int full_update(xnb_ctx* c) {
@@ -455,6 +584,13 @@ int xnb_create(xnb_ctx** out, const xnb_domain* d, int device) {
     xnb_destroy(c);
     return fail(nullptr, XNB_E_NOMEM, "pinned allocation failed");
   }
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+    c->lds_limit = (uint32_t)lds;
+  if (const char* v = std::getenv("XNB_BRICKS")) {  // "0" disables the brick path, "i,j,k" fixes brick dims
+    if (v[0] == '0' && v[1] == 0) c->use_bricks = false;
+    else std::sscanf(v, "%d,%d,%d", &c->brick_dims[0], &c->brick_dims[1], &c->brick_dims[2]);
+  }
   HIPCHK(c, hipMemsetAsync(c->cell_n.p, 0, c->cell_n.n * sizeof(uint32_t), c->st));
   HIPCHK(c, hipMemsetAsync(c->cell_off.p, 0, c->cell_off.n * sizeof(uint32_t), c->st));
   *out = c;
@@ -477,6 +613,7 @@ int xnb_destroy(xnb_ctx* c) {
                             &c->clen, &c->coff, &c->mv_dst, &c->pk, &c->newloc, &c->in_list, &c->plen, &c->pstart, &c->ghost_src,
                             &c->ghost_shift, &c->backup, &c->nbr, &c->nbr_cnt, &c->bsum, &c->u32s})
     dfree(*a);
+  dfree(c->blist); dfree(c->bcnt);
   dfree(c->stream_buf); dfree(c->u64s); dfree(c->step_counts); dfree(c->red_part); dfree(c->red_out);
   if (c->h_u32) hipHostFree(c->h_u32);
   if (c->h_u64) hipHostFree(c->h_u64);
@@ -734,6 +871,7 @@ int xnb_chunk_neighbors_stream(xnb_ctx* c, uint16_t* out, size_t cap, uint64_t* 
   int e;
   if ((e = check_ctx(c))) return e;
   if (!c->lists_valid) return fail(c, XNB_E_STATE, "no neighbour lists (chunk_neighbors)");
+  if ((e = ensure_slot_lists(c))) return e;
   const Geom& g = c->g;
   const int ntables = c->list_cfg.build_particle_offset ? (c->list_cfg.dual_particle_offset ? 2 : 1) : 0;
   launch_stream_plen(c->list_atoms, c->slot_cell_A.p, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, c->list_cfg.random_access, c->plen.p,
@@ -774,13 +912,12 @@ int xnb_lennard_jones_force(xnb_ctx* c, const xnb_lj_params* p, double rcut, int
   int e;
   if ((e = check_ctx(c))) return e;
   if (!p || (precision != XNB_PRECISION_FP64 && precision != XNB_PRECISION_MIXED)) return fail(c, XNB_E_ARG, "bad LJ arguments");
-  const uint32_t n_atoms = ghost ? c->N + c->G : c->N;
-  if (!c->lists_valid || c->list_atoms < n_atoms) return fail(c, XNB_E_STATE, "neighbour lists do not cover the requested cells");
+  if (!c->lists_valid) return fail(c, XNB_E_STATE, "no neighbour lists (chunk_neighbors)");
   c->last_rcut = rcut;
   tstart(c, XNB_TIMER_FORCE);
-  launch_lj(precision, true, false, n_atoms, c->A, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, p->epsilon, p->sigma, rcut, 0.0, c->st);
+  e = launch_force(c, precision, true, false, ghost != 0, p->epsilon, p->sigma, rcut, 0.0);
   tstop(c);
-  HIPCHK(c, hipGetLastError());
+  if (e) return e;
   HIPCHK(c, hipStreamSynchronize(c->st));
   return XNB_OK;
 }
@@ -791,6 +928,7 @@ int xnb_lennard_jones_force_multimat(xnb_ctx* c, int ntypes, const double* eps, 
   if ((e = check_ctx(c))) return e;
   if (ntypes <= 0 || ntypes > 4 || !eps || !sigma || !rcut) return fail(c, XNB_E_ARG, "bad multimat arguments (1..4 types)");
   if (!c->lists_valid) return fail(c, XNB_E_STATE, "no neighbour lists");
+  if ((e = ensure_slot_lists(c))) return e;
   tstart(c, XNB_TIMER_FORCE);
   if (launch_lj_multimat(precision, c->N, c->A, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, ntypes, eps, sigma, rcut, c->st))
     return fail(c, XNB_E_ARG, "too many type pairs");
@@ -846,9 +984,9 @@ int xnb_init(xnb_ctx* c, const xnb_lj_params* p, double rcut, double rcut_inc, i
   if ((e = full_update(c))) return e;
   c->last_rcut = rcut;
   tstart(c, XNB_TIMER_FORCE);
-  launch_lj(precision, false, false, c->N, c->A, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, p->epsilon, p->sigma, rcut, 0.0, c->st);
+  e = launch_force(c, precision, false, false, false, p->epsilon, p->sigma, rcut, 0.0);
   tstop(c);
-  HIPCHK(c, hipGetLastError());
+  if (e) return e;
   HIPCHK(c, hipStreamSynchronize(c->st));
   return XNB_OK;
 }
@@ -893,9 +1031,9 @@ int xnb_run(xnb_ctx* c, xnb_run_params* p) {
       tstop(c);
     }
     tstart(c, XNB_TIMER_FORCE);
-    launch_lj(p->precision, false, true, c->N, c->A, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, p->lj.epsilon, p->lj.sigma, p->rcut, dth,
-              c->st);
+    e = launch_force(c, p->precision, false, true, false, p->lj.epsilon, p->lj.sigma, p->rcut, dth);
     tstop(c);
+    if (e) return e;
   }
   HIPCHK(c, hipGetLastError());
   if (interval && p->steps > 0) {
@@ -937,6 +1075,7 @@ int xnb_pair_counts(xnb_ctx* c, uint64_t* list_entries, uint64_t* cutoff_pairs) 
   int e;
   if ((e = check_ctx(c))) return e;
   if (!c->lists_valid) return fail(c, XNB_E_STATE, "no neighbour lists");
+  if ((e = ensure_slot_lists(c))) return e;
   HIPCHK(c, hipMemsetAsync(c->u64s.p + 1, 0, 2 * sizeof(unsigned long long), c->st));
   launch_pair_counts(c->N, c->A.r, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, c->last_rcut * c->last_rcut, c->u64s.p + 1, c->st);
   HIPCHK(c, hipGetLastError());

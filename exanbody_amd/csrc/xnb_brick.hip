@@ -1,0 +1,447 @@
+// xnb_brick.hip -- LDS-tiled neighbour build and LJ force for gfx950.
+//
+// The inner domain is tiled by bricks of Bi x Bj x Bk cells; one workgroup owns one brick.
+// It stages the positions of the brick's region (the brick plus one cell on every side,
+// ghost cells included) into LDS, region cells in k-major order, so a neighbour is addressed
+// by its u16 LDS index.  Lanes are the brick's atoms in (cell k-major, particle) order.
+//
+// Neighbour lists are u16 LDS indices, one slab of cap rows x T lanes per brick, written as
+// whole coalesced rows.  Because region cells are k-major and candidate cells are visited in
+// (dk, dj, di) order with particles in cell order, every list is in ascending (cell_enc, p_b)
+// order -- the order of the reference stream (chunk_neighbors_execute.h:308) -- and the
+// forces are summed in that order.
+//
+// Requires cell_layers == 1 (cell edge >= nbh_dist) and ghost layers >= 1.
+#include "xnb_kernels.h"
+
+namespace xnb {
+
+// inclusive-exclusive scan across the whole workgroup (blockDim.x multiple of 64, <= 1024)
+__device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  uint32_t woff = 0, tot = 0;
+  for (int i = 0; i < nw; i++) {
+    const uint32_t s = wsum[i];
+    if (i < w) woff += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return x - v + woff;
+}
+
+__device__ inline uint32_t wave_max_b(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
+// largest k in [0, n) with pref[k] <= x (pref non-decreasing, pref[0] = 0 <= x)
+__device__ inline uint32_t upper_index(const uint32_t* pref, uint32_t n, uint32_t x) {
+  uint32_t lo = 0, hi = n;  // invariant: pref[lo] <= x, answer in [lo, hi)
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pref[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+struct BrickLds {
+  uint32_t *rcnt, *rpref, *roff, *bpref;
+  double *px, *py, *pz;
+  uint16_t* stage;
+};
+
+__device__ inline BrickLds brick_lds(const BrickParams& bp, char* lds) {
+  BrickLds L;
+  L.rcnt = (uint32_t*)(lds + bp.off_rcnt);
+  L.rpref = (uint32_t*)(lds + bp.off_rpref);
+  L.roff = (uint32_t*)(lds + bp.off_roff);
+  L.bpref = (uint32_t*)(lds + bp.off_bpref);
+  L.px = (double*)(lds + bp.off_px);
+  L.py = (double*)(lds + bp.off_py);
+  L.pz = (double*)(lds + bp.off_pz);
+  L.stage = (uint16_t*)(lds + bp.off_stage);
+  return L;
+}
+
+struct BrickPos {
+  int gi0, gj0, gk0;  // grid coordinates of the brick's first cell
+  int ni, nj, nk;     // brick extent clipped to the inner domain
+};
+
+__device__ inline BrickPos brick_pos(const Geom& g, const BrickParams& bp, uint32_t b) {
+  BrickPos P;
+  const int bx = (int)(b % (uint32_t)bp.nbx);
+  const int by = (int)((b / (uint32_t)bp.nbx) % (uint32_t)bp.nby);
+  const int bz = (int)(b / ((uint32_t)bp.nbx * (uint32_t)bp.nby));
+  P.gi0 = g.gl + bx * bp.Bi;
+  P.gj0 = g.gl + by * bp.Bj;
+  P.gk0 = g.gl + bz * bp.Bk;
+  P.ni = min(bp.Bi, g.Di - bx * bp.Bi);
+  P.nj = min(bp.Bj, g.Dj - by * bp.Bj);
+  P.nk = min(bp.Bk, g.Dk - bz * bp.Bk);
+  return P;
+}
+
+// Region tables (+ positions when load_pos) into LDS.  Needs nrc <= blockDim.x and
+// nbc <= blockDim.x.  Ends with a workgroup barrier.
+__device__ inline void brick_setup(const Geom& g, const BrickParams& bp, const BrickPos& P, const BrickLds& L,
+                                   const double4* __restrict__ r, const uint32_t* __restrict__ cell_off,
+                                   const uint32_t* __restrict__ cell_n, bool load_pos) {
+  const uint32_t t = threadIdx.x;
+  uint32_t cnt = 0;
+  if (t < (uint32_t)bp.nrc) {
+    const int ri = (int)(t % (uint32_t)bp.RX), rj = (int)((t / (uint32_t)bp.RX) % (uint32_t)bp.RY),
+              rk = (int)(t / ((uint32_t)bp.RX * (uint32_t)bp.RY));
+    const int gi = P.gi0 - 1 + ri, gj = P.gj0 - 1 + rj, gk = P.gk0 - 1 + rk;
+    const bool in = ri <= P.ni + 1 && rj <= P.nj + 1 && rk <= P.nk + 1 && gi < g.di && gj < g.dj && gk < g.dk;
+    uint32_t off = 0;
+    if (in) {
+      const uint32_t c = cell_index(g, gi, gj, gk);
+      cnt = cell_n[c];
+      off = cell_off[c];
+    }
+    L.rcnt[t] = cnt;
+    L.roff[t] = off;
+  }
+  uint32_t tot;
+  const uint32_t ex = block_scan_excl(cnt, &tot);
+  if (t < (uint32_t)bp.nrc) L.rpref[t] = ex;
+  if (t == 0) L.rpref[bp.nrc] = tot;
+  __syncthreads();
+  // brick cells: bc = (ck*Bj + cj)*Bi + ci, region cell ((ck+1)*RY + cj+1)*RX + ci+1
+  uint32_t bcnt = 0;
+  if (t < (uint32_t)bp.nbc) {
+    const int ci = (int)(t % (uint32_t)bp.Bi), cj = (int)((t / (uint32_t)bp.Bi) % (uint32_t)bp.Bj),
+              ck = (int)(t / ((uint32_t)bp.Bi * (uint32_t)bp.Bj));
+    if (ci < P.ni && cj < P.nj && ck < P.nk) bcnt = L.rcnt[((ck + 1) * bp.RY + cj + 1) * bp.RX + ci + 1];
+  }
+  const uint32_t bex = block_scan_excl(bcnt, &tot);
+  if (t < (uint32_t)bp.nbc) L.bpref[t] = bex;
+  if (t == 0) L.bpref[bp.nbc] = tot;
+  __syncthreads();
+  if (load_pos) {
+    const uint32_t R = L.rpref[bp.nrc];
+    for (uint32_t idx = t; idx < R; idx += blockDim.x) {
+      const uint32_t rc = upper_index(L.rpref, (uint32_t)bp.nrc, idx);
+      const double4 p = r[L.roff[rc] + (idx - L.rpref[rc])];
+      L.px[idx] = p.x;
+      L.py[idx] = p.y;
+      L.pz[idx] = p.z;
+    }
+    __syncthreads();
+  }
+}
+
+struct LaneAtom {
+  bool live;
+  uint32_t rc;    // region cell
+  uint32_t p;     // index in cell
+  uint32_t ia;    // LDS index
+  uint32_t slot;  // global slot
+};
+
+__device__ inline LaneAtom lane_atom(const BrickParams& bp, const BrickLds& L) {
+  LaneAtom A;
+  const uint32_t t = threadIdx.x;
+  A.live = t < L.bpref[bp.nbc];
+  if (!A.live) {
+    A.rc = 0;
+    A.p = 0;
+    A.ia = 0;
+    A.slot = 0;
+    return A;
+  }
+  const uint32_t bc = upper_index(L.bpref, (uint32_t)bp.nbc, t);
+  const int ci = (int)(bc % (uint32_t)bp.Bi), cj = (int)((bc / (uint32_t)bp.Bi) % (uint32_t)bp.Bj),
+            ck = (int)(bc / ((uint32_t)bp.Bi * (uint32_t)bp.Bj));
+  A.rc = (uint32_t)(((ck + 1) * bp.RY + cj + 1) * bp.RX + ci + 1);
+  A.p = t - L.bpref[bc];
+  A.ia = L.rpref[A.rc] + A.p;
+  A.slot = L.roff[A.rc] + A.p;
+  return A;
+}
+
+// ------------------------------------------------------------------ statistics
+// per brick: atoms in the brick and in its region -> maxima
+__global__ void k_brick_stats(Geom g, BrickParams bp, const uint32_t* __restrict__ cell_n, uint32_t* __restrict__ out) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= bp.nbricks) return;
+  const BrickPos P = brick_pos(g, bp, b);
+  uint32_t R = 0, nb = 0;
+  for (int rk = 0; rk <= P.nk + 1; rk++)
+    for (int rj = 0; rj <= P.nj + 1; rj++)
+      for (int ri = 0; ri <= P.ni + 1; ri++) {
+        const int gi = P.gi0 - 1 + ri, gj = P.gj0 - 1 + rj, gk = P.gk0 - 1 + rk;
+        if (gi >= g.di || gj >= g.dj || gk >= g.dk) continue;
+        const uint32_t n = cell_n[cell_index(g, gi, gj, gk)];
+        R += n;
+        if (ri >= 1 && ri <= P.ni && rj >= 1 && rj <= P.nj && rk >= 1 && rk <= P.nk) nb += n;
+      }
+  atomicMax(&out[0], nb);
+  atomicMax(&out[1], R);
+}
+
+// ------------------------------------------------------------------ build
+// chunk_neighbors (chunk_neighbors_execute.h:221-239): d = r_a - r_b, FP64,
+// kept iff (a != b) && 0 < d2 <= nbh_dist^2, then the NeighborFilterHalfSymGhost filters.
+__global__ void __launch_bounds__(1024) k_nbh_brick(Geom g, BrickParams bp, const double4* __restrict__ r,
+                                                    const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
+                                                    double max2, int half_symmetric, int skip_ghosts, uint16_t* __restrict__ blist,
+                                                    uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt) {
+  extern __shared__ __align__(16) char lds[];
+  const uint32_t b = blockIdx.x;
+  const BrickLds L = brick_lds(bp, lds);
+  const BrickPos P = brick_pos(g, bp, b);
+  brick_setup(g, bp, P, L, r, cell_off, cell_n, true);
+  const LaneAtom A = lane_atom(bp, L);
+  const uint32_t t = threadIdx.x, T = blockDim.x;
+  double ax = 0., ay = 0., az = 0.;
+  uint32_t ca = 0;
+  if (A.live) {
+    ax = L.px[A.ia];
+    ay = L.py[A.ia];
+    az = L.pz[A.ia];
+    const int ri = (int)(A.rc % (uint32_t)bp.RX), rj = (int)((A.rc / (uint32_t)bp.RX) % (uint32_t)bp.RY),
+              rk = (int)(A.rc / ((uint32_t)bp.RX * (uint32_t)bp.RY));
+    ca = cell_index(g, P.gi0 - 1 + ri, P.gj0 - 1 + rj, P.gk0 - 1 + rk);
+  }
+  const uint32_t cap = bp.cap;
+  uint32_t cnt = 0;
+  for (int dk = -1; dk <= 1; dk++)
+    for (int dj = -1; dj <= 1; dj++)
+      for (int di = -1; di <= 1; di++) {
+        const int rcb = (int)A.rc + (dk * bp.RY + dj) * bp.RX + di;
+        const uint32_t nb = A.live ? L.rcnt[rcb] : 0u;
+        const uint32_t base = A.live ? L.rpref[rcb] : 0u;
+        const uint32_t nbmax = wave_max_b(nb);
+        const uint32_t cb = ca + (uint32_t)((dk * g.dj + dj) * g.di + di);
+        bool bghost = false;
+        if (skip_ghosts && A.live) {
+          int ii, jj, kk;
+          cell_ijk(g, cb, ii, jj, kk);
+          bghost = is_ghost(g, ii, jj, kk);
+        }
+        for (uint32_t p = 0; p < nbmax; p++) {
+          if (p < nb) {
+            const uint32_t ib = base + p;
+            const double dx = ax - L.px[ib];
+            const double dy = ay - L.py[ib];
+            const double dz = az - L.pz[ib];
+            const double d2 = (dx * dx + dy * dy) + dz * dz;
+            bool keep = ib != A.ia && d2 > 0.0 && d2 <= max2;
+            if (half_symmetric && (ca < cb || (ca == cb && A.p < p))) keep = false;
+            if (skip_ghosts && bghost) keep = false;
+            if (keep) {
+              if (cnt < cap) L.stage[(size_t)cnt * T + t] = (uint16_t)ib;
+              ++cnt;
+            }
+          }
+        }
+      }
+  // coalesced write-out: row q of the slab holds entry q of every lane
+  const uint32_t keep_n = min(cnt, cap);
+  const uint32_t rows = wave_max_b(keep_n);
+  uint16_t* out = blist + (size_t)b * cap * T + t;
+  for (uint32_t q = 0; q < rows; q++)
+    if (q < keep_n) out[(size_t)q * T] = L.stage[(size_t)q * T + t];
+  bcnt[(size_t)b * T + t] = A.live ? cnt : 0u;
+  const uint32_t wmax = wave_max_b(cnt);
+  if ((t & 63) == 0 && wmax > 0) atomicMax(max_cnt, wmax);
+}
+
+// ------------------------------------------------------------------ force
+template <bool MIXED, bool ACCUM, bool PUSH, int U>
+__global__ void __launch_bounds__(1024) k_lj_brick(Geom g, BrickParams bp, const double4* __restrict__ r,
+                                                   const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
+                                                   const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt,
+                                                   LJConst lj, double* __restrict__ fx, double* __restrict__ fy,
+                                                   double* __restrict__ fz, double* __restrict__ ep, double* __restrict__ vx,
+                                                   double* __restrict__ vy, double* __restrict__ vz, double dth) {
+  extern __shared__ __align__(16) char lds[];
+  const uint32_t b = blockIdx.x;
+  const BrickLds L = brick_lds(bp, lds);
+  const BrickPos P = brick_pos(g, bp, b);
+  brick_setup(g, bp, P, L, r, cell_off, cell_n, true);
+  const LaneAtom A = lane_atom(bp, L);
+  const uint32_t t = threadIdx.x, T = blockDim.x;
+  const uint32_t n = A.live ? bcnt[(size_t)b * T + t] : 0u;
+  const uint32_t nmax = wave_max_b(n);
+  double4 ra;
+  ra.x = L.px[A.ia];
+  ra.y = L.py[A.ia];
+  ra.z = L.pz[A.ia];
+  ra.w = 0.;
+  const uint16_t* lst = blist + (size_t)b * bp.cap * T + t;
+  double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
+  for (uint32_t q0 = 0; q0 < nmax; q0 += U) {
+    uint32_t j[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) j[u] = lst[(size_t)(q0 + u) * T];  // rows < cap: in bounds
+#pragma unroll
+    for (int u = 0; u < U; u++) j[u] = (q0 + u < n) ? j[u] : A.ia;  // own index: rejected by d2 > 0
+    double4 rb[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      rb[u].x = L.px[j[u]];
+      rb[u].y = L.py[j[u]];
+      rb[u].z = L.pz[j[u]];
+      rb[u].w = 0.;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (MIXED) lj_pair_mixed(lj, ra, rb[u], tx, ty, tz, te);
+      else lj_pair_fp64(lj, ra, rb[u], tx, ty, tz, te);
+    }
+  }
+  if (!A.live) return;
+  const uint32_t s = A.slot;
+  store_force<ACCUM>(s, tx, ty, tz, te, fx, fy, fz, ep);
+  if (PUSH) {
+    vx[s] += fx[s] * dth;
+    vy[s] += fy[s] * dth;
+    vz[s] += fz[s] * dth;
+  }
+}
+
+// ------------------------------------------------------------------ conversion
+// brick list -> generic per-slot u32 list (slot indices), for stream export and the generic
+// traversals.
+__global__ void __launch_bounds__(1024) k_brick_to_slots(Geom g, BrickParams bp, const uint32_t* __restrict__ cell_off,
+                                                         const uint32_t* __restrict__ cell_n, const uint16_t* __restrict__ blist,
+                                                         const uint32_t* __restrict__ bcnt, uint32_t* __restrict__ nbr,
+                                                         uint32_t* __restrict__ nbr_cnt, uint32_t cap) {
+  extern __shared__ __align__(16) char lds[];
+  const uint32_t b = blockIdx.x;
+  const BrickLds L = brick_lds(bp, lds);
+  const BrickPos P = brick_pos(g, bp, b);
+  brick_setup(g, bp, P, L, nullptr, cell_off, cell_n, false);
+  const LaneAtom A = lane_atom(bp, L);
+  if (!A.live) return;
+  const uint32_t t = threadIdx.x, T = blockDim.x;
+  const uint32_t n = min(bcnt[(size_t)b * T + t], bp.cap);
+  const uint16_t* lst = blist + (size_t)b * bp.cap * T + t;
+  uint32_t* outp = nbr + nbr_index(A.slot, 0, cap);
+  for (uint32_t q = 0; q < n; q++) {
+    const uint32_t idx = lst[(size_t)q * T];
+    const uint32_t rc = upper_index(L.rpref, (uint32_t)bp.nrc, idx);
+    outp[(size_t)q << 6] = L.roff[rc] + (idx - L.rpref[rc]);
+  }
+  nbr_cnt[A.slot] = n;
+}
+
+// ------------------------------------------------------------------ launchers
+BrickParams brick_params(const Geom& g, int Bi, int Bj, int Bk) {
+  BrickParams bp{};
+  bp.Bi = Bi;
+  bp.Bj = Bj;
+  bp.Bk = Bk;
+  bp.nbx = (g.Di + Bi - 1) / Bi;
+  bp.nby = (g.Dj + Bj - 1) / Bj;
+  bp.nbz = (g.Dk + Bk - 1) / Bk;
+  bp.nbricks = (uint32_t)bp.nbx * bp.nby * bp.nbz;
+  bp.RX = Bi + 2;
+  bp.RY = Bj + 2;
+  bp.RZ = Bk + 2;
+  bp.nrc = bp.RX * bp.RY * bp.RZ;
+  bp.nbc = Bi * Bj * Bk;
+  return bp;
+}
+
+static inline uint32_t align8(uint32_t x) { return (x + 7u) & ~7u; }
+
+// LDS layout; returns bytes for the build (with_stage) or force kernel
+uint32_t brick_layout(BrickParams& bp, bool with_stage) {
+  uint32_t o = 0;
+  bp.off_rcnt = o;
+  o = align8(o + 4u * bp.nrc);
+  bp.off_rpref = o;
+  o = align8(o + 4u * (bp.nrc + 1));
+  bp.off_roff = o;
+  o = align8(o + 4u * bp.nrc);
+  bp.off_bpref = o;
+  o = align8(o + 4u * (bp.nbc + 1));
+  bp.off_px = o;
+  o += 8u * bp.Rmax;
+  bp.off_py = o;
+  o += 8u * bp.Rmax;
+  bp.off_pz = o;
+  o += 8u * bp.Rmax;
+  bp.off_stage = o;
+  if (with_stage) o += 2u * bp.cap * bp.T;
+  return align8(o);
+}
+
+void launch_brick_stats(const Geom& g, const BrickParams& bp, const uint32_t* cell_n, uint32_t* out, hipStream_t st) {
+  k_brick_stats<<<(bp.nbricks + 255) / 256, 256, 0, st>>>(g, bp, cell_n, out);
+}
+
+static hipError_t set_lds(const void* fn, uint32_t bytes) {
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, double max2,
+                            int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt, uint32_t* max_cnt, hipStream_t st) {
+  const uint32_t lds = brick_layout(bp, true);
+  hipError_t e = set_lds((const void*)k_nbh_brick, lds);
+  if (e != hipSuccess) return e;
+  k_nbh_brick<<<bp.nbricks, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist, bcnt, max_cnt);
+  return hipGetLastError();
+}
+
+static constexpr int LJB_UNROLL = 8;
+
+template <bool MIXED, bool ACCUM, bool PUSH>
+static hipError_t ljb_launch(const Geom& g, const BrickParams& bp, uint32_t lds, const double4* r, const uint32_t* cell_off,
+                             const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const LJConst& lj, const PState& A,
+                             double dth, hipStream_t st) {
+  hipError_t e = set_lds((const void*)k_lj_brick<MIXED, ACCUM, PUSH, LJB_UNROLL>, lds);
+  if (e != hipSuccess) return e;
+  k_lj_brick<MIXED, ACCUM, PUSH, LJB_UNROLL><<<bp.nbricks, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, blist, bcnt, lj, A.fx, A.fy,
+                                                                          A.fz, A.ep, A.vx, A.vy, A.vz, dth);
+  return hipGetLastError();
+}
+
+uint32_t brick_list_granule() { return LJB_UNROLL; }
+
+hipError_t launch_lj_brick(int precision, bool accum, bool push, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
+                           const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, double eps, double sigma, double rcut,
+                           double dth, hipStream_t st) {
+  const LJConst lj = make_lj_const(eps, sigma, rcut);
+  const uint32_t lds = brick_layout(bp, false);
+  const int key = (precision ? 4 : 0) | (accum ? 2 : 0) | (push ? 1 : 0);
+#define XNB_LJB_ARGS g, bp, lds, A.r, cell_off, cell_n, blist, bcnt, lj, A, dth, st
+  switch (key) {
+    case 0: return ljb_launch<false, false, false>(XNB_LJB_ARGS);
+    case 1: return ljb_launch<false, false, true>(XNB_LJB_ARGS);
+    case 2: return ljb_launch<false, true, false>(XNB_LJB_ARGS);
+    case 3: return ljb_launch<false, true, true>(XNB_LJB_ARGS);
+    case 4: return ljb_launch<true, false, false>(XNB_LJB_ARGS);
+    case 5: return ljb_launch<true, false, true>(XNB_LJB_ARGS);
+    case 6: return ljb_launch<true, true, false>(XNB_LJB_ARGS);
+    default: return ljb_launch<true, true, true>(XNB_LJB_ARGS);
+  }
+#undef XNB_LJB_ARGS
+}
+
+hipError_t launch_brick_to_slots(const Geom& g, BrickParams bp, const uint32_t* cell_off, const uint32_t* cell_n, const uint16_t* blist,
+                                 const uint32_t* bcnt, uint32_t* nbr, uint32_t* nbr_cnt, uint32_t cap, hipStream_t st) {
+  const uint32_t lds = brick_layout(bp, false);
+  hipError_t e = set_lds((const void*)k_brick_to_slots, lds);
+  if (e != hipSuccess) return e;
+  k_brick_to_slots<<<bp.nbricks, bp.T, lds, st>>>(g, bp, cell_off, cell_n, blist, bcnt, nbr, nbr_cnt, cap);
+  return hipGetLastError();
+}
+
+}  // namespace xnb

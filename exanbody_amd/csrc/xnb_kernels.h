@@ -5,8 +5,22 @@
 #include <stdint.h>
 
 #include "xnb_device.h"
+#include "xnb_pair.h"
 
 namespace xnb {
+
+// Brick tiling of the inner domain (see xnb_brick.hip) and the LDS layout of one workgroup.
+struct BrickParams {
+  int Bi, Bj, Bk;     // brick size in cells
+  int nbx, nby, nbz;  // bricks per axis
+  int RX, RY, RZ;     // region (brick + 1 cell each side)
+  int nrc, nbc;       // region cells, brick cells
+  uint32_t nbricks;
+  uint32_t T;     // threads per workgroup (>= brick atoms, nrc, nbc; multiple of 64)
+  uint32_t Rmax;  // largest region atom count
+  uint32_t cap;   // list rows per brick slab
+  uint32_t off_rcnt, off_rpref, off_roff, off_bpref, off_px, off_py, off_pz, off_stage;  // LDS byte offsets
+};
 
 // Particle arrays of one storage buffer (slots [0, N) inner, [N, N+G) ghosts).
 struct PState {
@@ -49,8 +63,9 @@ void launch_verlet_first_half(const Geom& g, uint32_t N, const PState& A, double
                               hipStream_t st);
 void launch_zero_force(uint32_t n, const PState& A, hipStream_t st);
 void launch_nbh_build(const Geom& g, uint32_t n_atoms, const double4* r, const uint32_t* slot_cell, const uint32_t* cell_off,
-                      const uint32_t* cell_n, double max2, int half_symmetric, int skip_ghosts, uint32_t* nbr, uint32_t* nbr_cnt,
-                      uint32_t cap, uint32_t* max_cnt, hipStream_t st);
+                      const uint32_t* cell_n, double max2, int lay, int half_symmetric, int skip_ghosts, uint32_t* nbr,
+                      uint32_t* nbr_cnt, uint32_t cap, uint32_t* max_cnt, hipStream_t st);
+uint32_t lj_list_granule();
 void launch_lj(int precision, bool accum, bool push, uint32_t n_atoms, const PState& A, const uint32_t* nbr,
                const uint32_t* nbr_cnt, uint32_t cap, double eps, double sigma, double rcut, double dth, hipStream_t st);
 int launch_lj_multimat(int precision, uint32_t n_atoms, const PState& A, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap,
@@ -65,6 +80,18 @@ void launch_stream_fill(const Geom& g, uint32_t n_atoms, const uint32_t* slot_ce
                         const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap, const uint32_t* pstart, const uint32_t* plen,
                         const uint32_t* coff, int ntables, int random_access, uint16_t* out, hipStream_t st);
 void launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t st);
+
+BrickParams brick_params(const Geom& g, int Bi, int Bj, int Bk);
+uint32_t brick_layout(BrickParams& bp, bool with_stage);
+uint32_t brick_list_granule();
+void launch_brick_stats(const Geom& g, const BrickParams& bp, const uint32_t* cell_n, uint32_t* out, hipStream_t st);
+hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, double max2,
+                            int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt, uint32_t* max_cnt, hipStream_t st);
+hipError_t launch_lj_brick(int precision, bool accum, bool push, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
+                           const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, double eps, double sigma, double rcut,
+                           double dth, hipStream_t st);
+hipError_t launch_brick_to_slots(const Geom& g, BrickParams bp, const uint32_t* cell_off, const uint32_t* cell_n, const uint16_t* blist,
+                                 const uint32_t* bcnt, uint32_t* nbr, uint32_t* nbr_cnt, uint32_t cap, hipStream_t st);
 void launch_energies(uint32_t N, const PState& A, double* part, uint32_t nparts, double* out, hipStream_t st);
 
 }  // namespace xnb

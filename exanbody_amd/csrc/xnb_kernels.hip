@@ -454,7 +454,7 @@ __global__ void k_zero_force(uint32_t n, double* __restrict__ fx, double* __rest
 
 __global__ void __launch_bounds__(256) k_nbh_build(Geom g, uint32_t n_atoms, const double4* __restrict__ r,
                                                    const uint32_t* __restrict__ slot_cell, const uint32_t* __restrict__ cell_off,
-                                                   const uint32_t* __restrict__ cell_n, double max2, int half_symmetric,
+                                                   const uint32_t* __restrict__ cell_n, double max2, int lay, int half_symmetric,
                                                    int skip_ghosts, uint32_t* __restrict__ nbr, uint32_t* __restrict__ nbr_cnt,
                                                    uint32_t cap, uint32_t* __restrict__ max_cnt) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -464,9 +464,10 @@ __global__ void __launch_bounds__(256) k_nbh_build(Geom g, uint32_t n_atoms, con
   cell_ijk(g, ca, ai, aj, ak);
   const uint32_t pa = s - cell_off[ca];
   const double4 ra = r[s];
-  const int i0 = max(ai - 1, 0), i1 = min(ai + 1, g.di - 1);
-  const int j0 = max(aj - 1, 0), j1 = min(aj + 1, g.dj - 1);
-  const int k0 = max(ak - 1, 0), k1 = min(ak + 1, g.dk - 1);
+  // candidate cells [a - lay, a + lay] clamped to the grid (chunk_neighbors_execute.h:134-139)
+  const int i0 = max(ai - lay, 0), i1 = min(ai + lay, g.di - 1);
+  const int j0 = max(aj - lay, 0), j1 = min(aj + lay, g.dj - 1);
+  const int k0 = max(ak - lay, 0), k1 = min(ak + lay, g.dk - 1);
   uint32_t cnt = 0;
   uint32_t* L = nbr + nbr_index(s, 0, cap);
   for (int bk = k0; bk <= k1; bk++)
@@ -494,113 +495,43 @@ __global__ void __launch_bounds__(256) k_nbh_build(Geom g, uint32_t n_atoms, con
 }
 
 // ------------------------------------------------------------------- LJ forces
-// Plain FP64 form: the reference traversal and functor arithmetic, neighbours in stream
-// order, accepted iff 0 < d2 <= rc^2 with d = r_b - r_a
-// (compute_cell_particle_pairs_impl_default.h:183-186), then lj_compute_energy
-// (lennard_jones.cu:46-56) and f += (de * (1/r)) * d (lennard_jones.cu:117-123).
-// The per-atom sum starts from 0 and is added once to f, like the buffer functor
-// (lennard_jones.cu:80-101).
+// Pair arithmetic (lj_pair_fp64 / lj_pair_mixed) lives in xnb_pair.h.
 
-struct LJConst {
-  double sigma, eps4, m24eps, rc2;
-  float sigma2f, eps4f, m24epsf;
-};
-
-template <bool ACCUM>
-__device__ inline void store_force(uint32_t s, double tx, double ty, double tz, double te, double* __restrict__ fx,
-                                   double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ ep) {
-  if (ACCUM) {
-    fx[s] += tx;
-    fy[s] += ty;
-    fz[s] += tz;
-    ep[s] += te;
-  } else {
-    fx[s] = 0.0 + tx;
-    fy[s] = 0.0 + ty;
-    fz[s] = 0.0 + tz;
-    ep[s] = 0.0 + te;
-  }
-}
-
-// ACCUM: add to existing f (lennard_jones_force operator after zero_particle_force);
-// otherwise f is overwritten with 0 + sum (fused zero_particle_force).
-// PUSH: fused verlet_second_half push_f_v(0.5) on the new force.
-template <bool ACCUM, bool PUSH>
-__global__ void __launch_bounds__(256) k_lj_fp64(uint32_t n_atoms, const double4* __restrict__ r, const uint32_t* __restrict__ nbr,
-                                                 const uint32_t* __restrict__ nbr_cnt, uint32_t cap, LJConst lj,
-                                                 double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
-                                                 double* __restrict__ ep, double* __restrict__ vx, double* __restrict__ vy,
-                                                 double* __restrict__ vz, double dth) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n_atoms) return;
+// Thread per atom, neighbours in list (= stream) order.  The trip count is the wave's largest
+// list; lanes past their own count read their own slot, which the d2 > 0 test rejects exactly
+// as the reference rejects self pairs.  U list rows are loaded, then U positions, then U pairs
+// are evaluated, so each wave keeps 3U loads in flight.  Requires cap % U == 0.
+// ACCUM: add to existing f (lennard_jones_force after zero_particle_force); otherwise
+// f = 0 + sum (fused zero_particle_force).  PUSH: fused push_f_v(0.5) on the new force.
+template <bool MIXED, bool ACCUM, bool PUSH, int U>
+__global__ void __launch_bounds__(256) k_lj(uint32_t n_atoms, const double4* __restrict__ r, const uint32_t* __restrict__ nbr,
+                                            const uint32_t* __restrict__ nbr_cnt, uint32_t cap, LJConst lj, double* __restrict__ fx,
+                                            double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ ep,
+                                            double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz, double dth) {
+  const uint32_t s0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = s0 < n_atoms;
+  const uint32_t s = live ? s0 : n_atoms - 1;
+  const uint32_t n = live ? nbr_cnt[s] : 0u;
+  const uint32_t nmax = wave_max_u32(n);
   const double4 ra = r[s];
-  const uint32_t n = nbr_cnt[s];
   const uint32_t* L = nbr + nbr_index(s, 0, cap);
   double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
-  for (uint32_t q = 0; q < n; q++) {
-    const double4 rb = r[L[(size_t)q << 6]];
-    const double dx = rb.x - ra.x;
-    const double dy = rb.y - ra.y;
-    const double dz = rb.z - ra.z;
-    const double d2 = (dx * dx + dy * dy) + dz * dz;
-    if (d2 > 0.0 && d2 <= lj.rc2) {
-      const double rr = sqrt(d2);
-      const double inv_r = 1.0 / rr;
-      const double ratio = lj.sigma * inv_r;
-      const double ratio2 = ratio * ratio;
-      const double ratio6 = ratio2 * ratio2 * ratio2;
-      const double ratio12 = ratio6 * ratio6;
-      const double e = lj.eps4 * (ratio12 - ratio6);
-      double de = (lj.m24eps * (2. * ratio12 - ratio6)) * inv_r;
-      de *= inv_r;  // interaction_weight / r with weight 1.0: the same correctly rounded 1.0/r
-      tx += de * dx;
-      ty += de * dy;
-      tz += de * dz;
-      te += e;
+  for (uint32_t q0 = 0; q0 < nmax; q0 += U) {
+    uint32_t j[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) j[u] = L[(size_t)(q0 + u) << 6];  // rows < cap: always in bounds
+#pragma unroll
+    for (int u = 0; u < U; u++) j[u] = (q0 + u < n) ? j[u] : s;
+    double4 rb[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) rb[u] = r[j[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (MIXED) lj_pair_mixed(lj, ra, rb[u], tx, ty, tz, te);
+      else lj_pair_fp64(lj, ra, rb[u], tx, ty, tz, te);
     }
   }
-  store_force<ACCUM>(s, tx, ty, tz, te, fx, fy, fz, ep);
-  if (PUSH) {
-    vx[s] += fx[s] * dth;
-    vy[s] += fy[s] * dth;
-    vz[s] += fz[s] * dth;
-  }
-}
-
-// Mixed form: identical neighbour walk and FP64 displacement; the cutoff test uses the
-// FP64 d2; the pair coefficient de/r = -24 eps (2 s^12 - s^6) / r^2 is evaluated in FP32
-// from (float)d2 with a hardware reciprocal, and accumulated in FP64 as fma(c, d, f).
-template <bool ACCUM, bool PUSH>
-__global__ void __launch_bounds__(256) k_lj_mixed(uint32_t n_atoms, const double4* __restrict__ r, const uint32_t* __restrict__ nbr,
-                                                  const uint32_t* __restrict__ nbr_cnt, uint32_t cap, LJConst lj,
-                                                  double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
-                                                  double* __restrict__ ep, double* __restrict__ vx, double* __restrict__ vy,
-                                                  double* __restrict__ vz, double dth) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n_atoms) return;
-  const double4 ra = r[s];
-  const uint32_t n = nbr_cnt[s];
-  const uint32_t* L = nbr + nbr_index(s, 0, cap);
-  double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
-  for (uint32_t q = 0; q < n; q++) {
-    const double4 rb = r[L[(size_t)q << 6]];
-    const double dx = rb.x - ra.x;
-    const double dy = rb.y - ra.y;
-    const double dz = rb.z - ra.z;
-    const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
-    if (d2 > 0.0 && d2 <= lj.rc2) {
-      const float inv_r2 = __builtin_amdgcn_rcpf((float)d2);
-      const float s2 = lj.sigma2f * inv_r2;
-      const float s6 = s2 * s2 * s2;
-      const float s12 = s6 * s6;
-      const float c = lj.m24epsf * (2.0f * s12 - s6) * inv_r2;
-      const double cd = (double)c;
-      tx = fma(cd, dx, tx);
-      ty = fma(cd, dy, ty);
-      tz = fma(cd, dz, tz);
-      te += (double)(lj.eps4f * (s12 - s6));
-    }
-  }
+  if (!live) return;
   store_force<ACCUM>(s, tx, ty, tz, te, fx, fy, fz, ep);
   if (PUSH) {
     vx[s] += fx[s] * dth;
@@ -957,47 +888,39 @@ void launch_zero_force(uint32_t n, const PState& A, hipStream_t st) {
   k_zero_force<<<nblk(n, 256), 256, 0, st>>>(n, A.fx, A.fy, A.fz, A.ep);
 }
 void launch_nbh_build(const Geom& g, uint32_t n_atoms, const double4* r, const uint32_t* slot_cell, const uint32_t* cell_off,
-                      const uint32_t* cell_n, double max2, int half_symmetric, int skip_ghosts, uint32_t* nbr, uint32_t* nbr_cnt,
-                      uint32_t cap, uint32_t* max_cnt, hipStream_t st) {
-  k_nbh_build<<<nblk(n_atoms, 256), 256, 0, st>>>(g, n_atoms, r, slot_cell, cell_off, cell_n, max2, half_symmetric, skip_ghosts,
+                      const uint32_t* cell_n, double max2, int lay, int half_symmetric, int skip_ghosts, uint32_t* nbr,
+                      uint32_t* nbr_cnt, uint32_t cap, uint32_t* max_cnt, hipStream_t st) {
+  k_nbh_build<<<nblk(n_atoms, 256), 256, 0, st>>>(g, n_atoms, r, slot_cell, cell_off, cell_n, max2, lay, half_symmetric, skip_ghosts,
                                                   nbr, nbr_cnt, cap, max_cnt);
 }
 
-static LJConst make_lj(double eps, double sigma, double rcut) {
-  LJConst c;
-  c.sigma = sigma;
-  c.eps4 = 4. * eps;
-  c.m24eps = -24. * eps;
-  c.rc2 = rcut * rcut;
-  c.sigma2f = (float)(sigma * sigma);
-  c.eps4f = (float)(4. * eps);
-  c.m24epsf = (float)(-24. * eps);
-  return c;
+
+static constexpr int LJ_UNROLL = 8;
+
+template <bool MIXED, bool ACCUM, bool PUSH>
+static void lj_launch(uint32_t n_atoms, const PState& A, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap, const LJConst& lj,
+                      double dth, hipStream_t st) {
+  k_lj<MIXED, ACCUM, PUSH, LJ_UNROLL><<<nblk(n_atoms, 256), 256, 0, st>>>(n_atoms, A.r, nbr, nbr_cnt, cap, lj, A.fx, A.fy, A.fz, A.ep,
+                                                                        A.vx, A.vy, A.vz, dth);
 }
+
+uint32_t lj_list_granule() { return LJ_UNROLL; }
 
 void launch_lj(int precision, bool accum, bool push, uint32_t n_atoms, const PState& A, const uint32_t* nbr,
                const uint32_t* nbr_cnt, uint32_t cap, double eps, double sigma, double rcut, double dth, hipStream_t st) {
-  const LJConst lj = make_lj(eps, sigma, rcut);
-  const uint32_t nb = nblk(n_atoms, 256);
-#define XNB_LJ_ARGS n_atoms, A.r, nbr, nbr_cnt, cap, lj, A.fx, A.fy, A.fz, A.ep, A.vx, A.vy, A.vz, dth
-  if (precision == 0) {
-    if (accum) {
-      if (push) k_lj_fp64<true, true><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
-      else k_lj_fp64<true, false><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
-    } else {
-      if (push) k_lj_fp64<false, true><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
-      else k_lj_fp64<false, false><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
-    }
-  } else {
-    if (accum) {
-      if (push) k_lj_mixed<true, true><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
-      else k_lj_mixed<true, false><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
-    } else {
-      if (push) k_lj_mixed<false, true><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
-      else k_lj_mixed<false, false><<<nb, 256, 0, st>>>(XNB_LJ_ARGS);
-    }
+  if (n_atoms == 0) return;
+  const LJConst lj = make_lj_const(eps, sigma, rcut);
+  const int key = (precision ? 4 : 0) | (accum ? 2 : 0) | (push ? 1 : 0);
+  switch (key) {
+    case 0: lj_launch<false, false, false>(n_atoms, A, nbr, nbr_cnt, cap, lj, dth, st); break;
+    case 1: lj_launch<false, false, true>(n_atoms, A, nbr, nbr_cnt, cap, lj, dth, st); break;
+    case 2: lj_launch<false, true, false>(n_atoms, A, nbr, nbr_cnt, cap, lj, dth, st); break;
+    case 3: lj_launch<false, true, true>(n_atoms, A, nbr, nbr_cnt, cap, lj, dth, st); break;
+    case 4: lj_launch<true, false, false>(n_atoms, A, nbr, nbr_cnt, cap, lj, dth, st); break;
+    case 5: lj_launch<true, false, true>(n_atoms, A, nbr, nbr_cnt, cap, lj, dth, st); break;
+    case 6: lj_launch<true, true, false>(n_atoms, A, nbr, nbr_cnt, cap, lj, dth, st); break;
+    default: lj_launch<true, true, true>(n_atoms, A, nbr, nbr_cnt, cap, lj, dth, st); break;
   }
-#undef XNB_LJ_ARGS
 }
 
 int launch_lj_multimat(int precision, uint32_t n_atoms, const PState& A, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap,

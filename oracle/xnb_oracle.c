@@ -605,6 +605,9 @@ int op_chunk_neighbors(op_sim* s, double nbh_dist_lab, const op_nbh_config* cfg)
   if (!cfg) cfg = &dflt;
   const int build_offset = cfg->build_particle_offset || cfg->dual_particle_offset;
   const double max_dist2 = nbh_dist_lab * nbh_dist_lab;
+  /* cell_layers = ceil(max_dist / cell_size) (amr_grid_algorithm.h:451) */
+  const int64_t lay = (int64_t)ceil(nbh_dist_lab / s->cell_size);
+  if (lay > 15) return OP_E_ARG;
   free_streams(s);
   for (size_t c = 0; c < s->ncells; c++)
     if (s->cells[c].n >= 65535u) return OP_E_OVERFLOW;
@@ -634,9 +637,10 @@ int op_chunk_neighbors(op_sim* s, double nbh_dist_lab, const op_nbh_config* cfg)
         palloc = na;
       }
       for (uint32_t q = 0; q < na; q++) pn[q] = 0;
-      const int64_t bi0 = ai - 1 > 0 ? ai - 1 : 0, bi1 = ai + 1 < d[0] - 1 ? ai + 1 : d[0] - 1;
-      const int64_t bj0 = aj - 1 > 0 ? aj - 1 : 0, bj1 = aj + 1 < d[1] - 1 ? aj + 1 : d[1] - 1;
-      const int64_t bk0 = ak - 1 > 0 ? ak - 1 : 0, bk1 = ak + 1 < d[2] - 1 ? ak + 1 : d[2] - 1;
+      /* candidate cells [a - lay, a + lay] clamped to the grid (chunk_neighbors_execute.h:134-139) */
+      const int64_t bi0 = ai - lay > 0 ? ai - lay : 0, bi1 = ai + lay < d[0] - 1 ? ai + lay : d[0] - 1;
+      const int64_t bj0 = aj - lay > 0 ? aj - lay : 0, bj1 = aj + lay < d[1] - 1 ? aj + lay : d[1] - 1;
+      const int64_t bk0 = ak - lay > 0 ? ak - lay : 0, bk1 = ak + lay < d[2] - 1 ? ak + lay : d[2] - 1;
       for (int64_t bk = bk0; bk <= bk1; bk++)
         for (int64_t bj = bj0; bj <= bj1; bj++)
           for (int64_t bi = bi0; bi <= bi1; bi++) {

@@ -73,3 +73,94 @@ def test_mixed_force_error():
     assert rms <= 1e-5, (mx, rms)
     pe_rel = np.abs(mix["ep"] - ref["ep"]).max() / np.abs(ref["ep"]).max()
     assert pe_rel <= 1e-5
+
+
+def _with_bricks(monkeypatch, value):
+    if value is None:
+        monkeypatch.delenv("XNB_BRICKS", raising=False)
+    else:
+        monkeypatch.setenv("XNB_BRICKS", value)
+
+
+@pytest.mark.parametrize("bricks", ["0", "1,1,1", "2,2,1", "4,2,2", "3,2,2"])
+def test_list_paths_bitwise(monkeypatch, bricks):
+    """Generic (per-slot) and brick-tiled lists of several shapes give the oracle's streams
+    and the oracle's FP64 forces bit for bit."""
+    _with_bricks(monkeypatch, bricks)
+    r, v, ids, L = fcc_inputs(7, temperature=1.2, jitter=0.1, seed=21)
+    o, b, dom = make_pair(r, v, ids, L)
+    o.init(1.0, 1.0, RCUT, SKIN)
+    b.init(1.0, 1.0, RCUT, SKIN, PRECISION_FP64)
+    dims, gl, nc = b.grid_info()
+    gs, _ = b.streams()
+    for c in range(nc):
+        k, rem = divmod(c, dims[0] * dims[1])
+        j, i = divmod(rem, dims[0])
+        if all(gl <= x < d - gl for x, d in zip((i, j, k), dims)):
+            np.testing.assert_array_equal(gs[c], o.stream(c), err_msg=f"cell {c}")
+    oe, ge = o.export(), b.export()
+    np.testing.assert_array_equal(oe["f"].view(np.uint64), ge["f"].view(np.uint64))
+    ro = o.run(30, 0.005, 1.0, 1.0, RCUT, SKIN, rebuild_every=0)
+    rg = b.run(30, 0.005, 1.0, 1.0, RCUT, SKIN, precision=PRECISION_FP64, rebuild_every=0)
+    assert ro == rg
+    oe, ge = o.export(), b.export()
+    assert_same_layout(oe, ge, "after run")
+    np.testing.assert_array_equal(oe["v"].view(np.uint64), ge["v"].view(np.uint64))
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(build_particle_offset=0), dict(dual_particle_offset=1),
+                                  dict(random_access=1), dict(half_symmetric=1), dict(skip_ghosts=1)])
+def test_stream_options_all_cells(opts):
+    """Every ChunkNeighborsConfig option, ghost cells included (all-cells generic build)."""
+    r, v, ids, L = fcc_inputs(6, jitter=0.05, seed=4)
+    o, b, dom = make_pair(r, v, ids, L)
+    for x in (o, b):
+        x.move_particles()
+        x.backup_r()
+        x.ghost_update_all()
+    o.chunk_neighbors(RCUT + SKIN, **opts)
+    b.chunk_neighbors(RCUT + SKIN, all_cells=True, **opts)
+    gs, gmax = b.streams()
+    os_ = o.streams()
+    assert len(gs) == len(os_)
+    for c in range(len(gs)):
+        np.testing.assert_array_equal(gs[c], os_[c], err_msg=f"cell {c} {opts}")
+    assert gmax == o.max_neighbors()
+
+
+@pytest.mark.parametrize("opts", [dict(half_symmetric=1), dict(skip_ghosts=1), dict(dual_particle_offset=1)])
+def test_stream_options_brick_path(opts):
+    r, v, ids, L = fcc_inputs(6, jitter=0.05, seed=5)
+    o, b, dom = make_pair(r, v, ids, L)
+    for x in (o, b):
+        x.move_particles()
+        x.backup_r()
+        x.ghost_update_all()
+    o.chunk_neighbors(RCUT + SKIN, **opts)
+    b.chunk_neighbors(RCUT + SKIN, **opts)
+    dims, gl, nc = b.grid_info()
+    gs, _ = b.streams()
+    for c in range(nc):
+        k, rem = divmod(c, dims[0] * dims[1])
+        j, i = divmod(rem, dims[0])
+        if all(gl <= x < d - gl for x, d in zip((i, j, k), dims)):
+            np.testing.assert_array_equal(gs[c], o.stream(c), err_msg=f"cell {c} {opts}")
+
+
+def test_stream_fixture_gpu():
+    import json
+    import os
+    from exanbody_amd import Box
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "stream_fixture.json")))
+    d = fx["domain"]
+    for name, case in fx["cases"].items():
+        b = Box(d["origin"], d["extent"], d["grid_dims"], d["cell_size"])
+        b.nbh_dist(2.5, 0.3)
+        b.set_particles(np.array(fx["positions"]))
+        b.move_particles()
+        b.ghost_update_all()
+        b.chunk_neighbors(fx["nbh_dist"], **case["config"])
+        gs, _ = b.streams()
+        for c, want in case["streams"].items():
+            assert gs[int(c)].tolist() == want, (name, c)
+        b.close()
