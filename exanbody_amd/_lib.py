@@ -83,6 +83,15 @@ _SIGS = {
     "xnb_pair_counts": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "xnb_synchronize": (C.c_int, [P]),
     "xnb_copy_probe": (C.c_int, [P, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    "xnb_deck_create": (C.c_int, [C.POINTER(P)]),
+    "xnb_deck_destroy": (C.c_int, [P]),
+    "xnb_deck_load": (C.c_int, [P, C.c_char_p]),
+    "xnb_deck_set_particles": (C.c_int, [P, C.c_size_t, P, P, P, P]),
+    "xnb_deck_run": (C.c_int, [P, C.c_char_p]),
+    "xnb_deck_context": (P, [P]),
+    "xnb_deck_error": (C.c_char_p, [P]),
+    "xnb_deck_log": (C.c_char_p, [P]),
+    "xnb_deck_slot": (C.c_int, [P, C.c_char_p, C.c_char_p, C.c_size_t]),
     "xnb_generate_fcc": (C.c_int, [C.c_int64, C.c_double, C.c_int64, C.c_double, C.c_double, C.c_uint64, P, P, P,
                                    C.POINTER(C.c_double)]),
 }

@@ -67,6 +67,9 @@ class Box:
         return rc
 
     def close(self):
+        if getattr(self, "_borrowed", False):  # owned by a Deck
+            self._h = None
+            return
         if getattr(self, "_h", None) is not None and self._h.value:
             self.L.xnb_destroy(self._h)
             self._h = None

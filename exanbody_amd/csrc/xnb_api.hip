@@ -86,6 +86,7 @@ struct xnb_ctx {
   DArr<uint16_t> blist;
   DArr<uint32_t> bcnt;
   bool slot_lists_current = false;  // nbr/nbr_cnt mirror the brick lists
+  bool build_stage = true;          // brick build stages lists in LDS before writing rows
   uint32_t lds_limit = 65536;
 
   double last_rcut = 0.;
@@ -395,7 +396,7 @@ int choose_bricks(xnb_ctx* c, uint32_t cap, BrickParams& out) {
     bp.T = T;
     bp.Rmax = std::max<uint32_t>(r_max, 1u);
     bp.cap = cap;
-    if (brick_layout(bp, true) > c->lds_limit) continue;
+    if (brick_layout(bp, c->build_stage) > c->lds_limit) continue;
     out = bp;
     return XNB_OK;
   }
@@ -414,7 +415,7 @@ int op_nbh_brick(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& 
     if ((e = dalloc(c, c->bcnt, (size_t)bp.nbricks * bp.T))) return e;
     HIPCHK(c, hipMemsetAsync(c->u32s.p + 2, 0, sizeof(uint32_t), c->st));
     tstart(c, XNB_TIMER_NBH);
-    hipError_t he = launch_nbh_brick(c->g, bp, c->A.r, c->cell_off.p, c->cell_n.p, max2, cfg.half_symmetric, cfg.skip_ghosts,
+    hipError_t he = launch_nbh_brick(c->g, bp, c->build_stage, c->A.r, c->cell_off.p, c->cell_n.p, max2, cfg.half_symmetric, cfg.skip_ghosts,
                                      c->blist.p, c->bcnt.p, c->u32s.p + 2, c->st);
     tstop(c);
     if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick neighbour build: ") + hipGetErrorString(he));
@@ -591,6 +592,7 @@ int xnb_create(xnb_ctx** out, const xnb_domain* d, int device) {
     if (v[0] == '0' && v[1] == 0) c->use_bricks = false;
     else std::sscanf(v, "%d,%d,%d", &c->brick_dims[0], &c->brick_dims[1], &c->brick_dims[2]);
   }
+  if (const char* v = std::getenv("XNB_BUILD_STAGE")) c->build_stage = v[0] != '0';
   HIPCHK(c, hipMemsetAsync(c->cell_n.p, 0, c->cell_n.n * sizeof(uint32_t), c->st));
   HIPCHK(c, hipMemsetAsync(c->cell_off.p, 0, c->cell_off.n * sizeof(uint32_t), c->st));
   *out = c;

@@ -196,6 +196,10 @@ __global__ void k_brick_stats(Geom g, BrickParams bp, const uint32_t* __restrict
 // ------------------------------------------------------------------ build
 // chunk_neighbors (chunk_neighbors_execute.h:221-239): d = r_a - r_b, FP64,
 // kept iff (a != b) && 0 < d2 <= nbh_dist^2, then the NeighborFilterHalfSymGhost filters.
+// Candidates are tested UB at a time (independent LDS reads and FP64 chains) and appended in
+// order.  STAGE: entries go to an LDS slab first and leave as whole rows; otherwise each
+// accepted entry is stored straight into its row of the global slab.
+template <bool STAGE, int UB>
 __global__ void __launch_bounds__(1024) k_nbh_brick(Geom g, BrickParams bp, const double4* __restrict__ r,
                                                     const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
                                                     double max2, int half_symmetric, int skip_ghosts, uint16_t* __restrict__ blist,
@@ -207,6 +211,7 @@ __global__ void __launch_bounds__(1024) k_nbh_brick(Geom g, BrickParams bp, cons
   brick_setup(g, bp, P, L, r, cell_off, cell_n, true);
   const LaneAtom A = lane_atom(bp, L);
   const uint32_t t = threadIdx.x, T = blockDim.x;
+  uint16_t* out = blist + (size_t)b * bp.cap * T + t;
   double ax = 0., ay = 0., az = 0.;
   uint32_t ca = 0;
   if (A.live) {
@@ -233,29 +238,40 @@ __global__ void __launch_bounds__(1024) k_nbh_brick(Geom g, BrickParams bp, cons
           cell_ijk(g, cb, ii, jj, kk);
           bghost = is_ghost(g, ii, jj, kk);
         }
-        for (uint32_t p = 0; p < nbmax; p++) {
-          if (p < nb) {
-            const uint32_t ib = base + p;
+        for (uint32_t p0 = 0; p0 < nbmax; p0 += UB) {
+          bool kp[UB];
+#pragma unroll
+          for (int u = 0; u < UB; u++) {
+            const uint32_t p = p0 + u;
+            const bool ok = p < nb;
+            const uint32_t ib = ok ? base + p : A.ia;
             const double dx = ax - L.px[ib];
             const double dy = ay - L.py[ib];
             const double dz = az - L.pz[ib];
             const double d2 = (dx * dx + dy * dy) + dz * dz;
-            bool keep = ib != A.ia && d2 > 0.0 && d2 <= max2;
+            bool keep = ok && ib != A.ia && d2 > 0.0 && d2 <= max2;
             if (half_symmetric && (ca < cb || (ca == cb && A.p < p))) keep = false;
             if (skip_ghosts && bghost) keep = false;
-            if (keep) {
-              if (cnt < cap) L.stage[(size_t)cnt * T + t] = (uint16_t)ib;
+            kp[u] = keep;
+          }
+#pragma unroll
+          for (int u = 0; u < UB; u++) {
+            if (kp[u]) {
+              if (cnt < cap) {
+                if (STAGE) L.stage[(size_t)cnt * T + t] = (uint16_t)(base + p0 + u);
+                else out[(size_t)cnt * T] = (uint16_t)(base + p0 + u);
+              }
               ++cnt;
             }
           }
         }
       }
-  // coalesced write-out: row q of the slab holds entry q of every lane
-  const uint32_t keep_n = min(cnt, cap);
-  const uint32_t rows = wave_max_b(keep_n);
-  uint16_t* out = blist + (size_t)b * cap * T + t;
-  for (uint32_t q = 0; q < rows; q++)
-    if (q < keep_n) out[(size_t)q * T] = L.stage[(size_t)q * T + t];
+  if (STAGE) {  // coalesced write-out: row q of the slab holds entry q of every lane
+    const uint32_t keep_n = min(cnt, cap);
+    const uint32_t rows = wave_max_b(keep_n);
+    for (uint32_t q = 0; q < rows; q++)
+      if (q < keep_n) out[(size_t)q * T] = L.stage[(size_t)q * T + t];
+  }
   bcnt[(size_t)b * T + t] = A.live ? cnt : 0u;
   const uint32_t wmax = wave_max_b(cnt);
   if ((t & 63) == 0 && wmax > 0) atomicMax(max_cnt, wmax);
@@ -391,12 +407,24 @@ static hipError_t set_lds(const void* fn, uint32_t bytes) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, double max2,
-                            int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt, uint32_t* max_cnt, hipStream_t st) {
-  const uint32_t lds = brick_layout(bp, true);
-  hipError_t e = set_lds((const void*)k_nbh_brick, lds);
-  if (e != hipSuccess) return e;
-  k_nbh_brick<<<bp.nbricks, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist, bcnt, max_cnt);
+static constexpr int NBB_UNROLL = 4;
+
+hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, bool stage, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n,
+                            double max2, int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt, uint32_t* max_cnt,
+                            hipStream_t st) {
+  const uint32_t lds = brick_layout(bp, stage);
+  hipError_t e;
+  if (stage) {
+    e = set_lds((const void*)k_nbh_brick<true, NBB_UNROLL>, lds);
+    if (e != hipSuccess) return e;
+    k_nbh_brick<true, NBB_UNROLL><<<bp.nbricks, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist,
+                                                                 bcnt, max_cnt);
+  } else {
+    e = set_lds((const void*)k_nbh_brick<false, NBB_UNROLL>, lds);
+    if (e != hipSuccess) return e;
+    k_nbh_brick<false, NBB_UNROLL><<<bp.nbricks, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist,
+                                                                  bcnt, max_cnt);
+  }
   return hipGetLastError();
 }
 

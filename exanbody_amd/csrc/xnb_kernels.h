@@ -85,7 +85,7 @@ BrickParams brick_params(const Geom& g, int Bi, int Bj, int Bk);
 uint32_t brick_layout(BrickParams& bp, bool with_stage);
 uint32_t brick_list_granule();
 void launch_brick_stats(const Geom& g, const BrickParams& bp, const uint32_t* cell_n, uint32_t* out, hipStream_t st);
-hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, double max2,
+hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, bool stage, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, double max2,
                             int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt, uint32_t* max_cnt, hipStream_t st);
 hipError_t launch_lj_brick(int precision, bool accum, bool push, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
                            const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, double eps, double sigma, double rcut,

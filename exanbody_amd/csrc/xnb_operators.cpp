@@ -1,0 +1,680 @@
+// xnb_operators.cpp -- host side mirroring exaNBody's operator interface for the short-range
+// path: operators registered by their YAML names, slots resolved by name from the deck and
+// from a global slot store, batches with condition / loop / rebind, and a YAML-subset reader
+// for the deck files.  Every particle operation goes through the C ABI (include/xnb_amd.h).
+//
+// Operator names and slot names follow the reference (data/config/numerical-scheme.msp,
+// update-particles.msp, compute-loop.msp; operator sources cited per operator below).
+#include <algorithm>
+#include <cctype>
+#include <cstdlib>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/xnb_amd.h"
+
+namespace xnbdeck {
+
+// ------------------------------------------------------------------ YAML subset
+struct Node {
+  enum Kind { Null, Scalar, Map, Seq } kind = Null;
+  std::string s;
+  std::vector<std::pair<std::string, Node>> m;
+  std::vector<Node> q;
+
+  const Node* get(const std::string& k) const {
+    for (auto& kv : m)
+      if (kv.first == k) return &kv.second;
+    return nullptr;
+  }
+  Node* get(const std::string& k) {
+    for (auto& kv : m)
+      if (kv.first == k) return &kv.second;
+    return nullptr;
+  }
+  void set(const std::string& k, Node v) {
+    if (Node* p = get(k)) *p = std::move(v);
+    else m.emplace_back(k, std::move(v));
+  }
+  static Node scalar(const std::string& v) {
+    Node n;
+    n.kind = Scalar;
+    n.s = v;
+    return n;
+  }
+};
+
+struct DeckError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+static std::string trim(const std::string& x) {
+  size_t a = 0, b = x.size();
+  while (a < b && std::isspace((unsigned char)x[a])) a++;
+  while (b > a && std::isspace((unsigned char)x[b - 1])) b--;
+  return x.substr(a, b - a);
+}
+
+static std::string strip_comment(const std::string& line) {
+  bool sq = false, dq = false;
+  for (size_t i = 0; i < line.size(); i++) {
+    const char c = line[i];
+    if (c == '\'' && !dq) sq = !sq;
+    else if (c == '"' && !sq) dq = !dq;
+    else if (c == '#' && !sq && !dq && (i == 0 || std::isspace((unsigned char)line[i - 1]))) return line.substr(0, i);
+  }
+  return line;
+}
+
+// flow-style value: {a: 1, b: [x, y]}, [1, 2], scalar
+struct Flow {
+  const std::string& t;
+  size_t i = 0;
+  explicit Flow(const std::string& s) : t(s) {}
+  void ws() {
+    while (i < t.size() && std::isspace((unsigned char)t[i])) i++;
+  }
+  Node value() {
+    ws();
+    if (i < t.size() && t[i] == '{') return map();
+    if (i < t.size() && t[i] == '[') return seq();
+    return Node::scalar(word());
+  }
+  std::string word() {
+    ws();
+    if (i < t.size() && (t[i] == '"' || t[i] == '\'')) {
+      const char qc = t[i++];
+      std::string w;
+      while (i < t.size() && t[i] != qc) w += t[i++];
+      i++;
+      return w;
+    }
+    std::string w;
+    int depth = 0;
+    while (i < t.size()) {
+      const char c = t[i];
+      if (depth == 0 && (c == ',' || c == '}' || c == ']')) break;
+      if (c == '[' || c == '{') depth++;
+      if (c == ']' || c == '}') depth--;
+      w += c;
+      i++;
+    }
+    return trim(w);
+  }
+  std::string key() {
+    ws();
+    std::string k;
+    while (i < t.size() && t[i] != ':') k += t[i++];
+    if (i >= t.size()) throw DeckError("flow map: missing ':' in " + t);
+    i++;
+    return trim(k);
+  }
+  Node map() {
+    Node n;
+    n.kind = Node::Map;
+    i++;
+    ws();
+    if (i < t.size() && t[i] == '}') {
+      i++;
+      return n;
+    }
+    while (i < t.size()) {
+      std::string k = key();
+      n.m.emplace_back(k, value());
+      ws();
+      if (i < t.size() && t[i] == ',') {
+        i++;
+        continue;
+      }
+      if (i < t.size() && t[i] == '}') {
+        i++;
+        break;
+      }
+      throw DeckError("flow map: expected ',' or '}' in " + t);
+    }
+    return n;
+  }
+  Node seq() {
+    Node n;
+    n.kind = Node::Seq;
+    i++;
+    ws();
+    if (i < t.size() && t[i] == ']') {
+      i++;
+      return n;
+    }
+    while (i < t.size()) {
+      n.q.push_back(value());
+      ws();
+      if (i < t.size() && t[i] == ',') {
+        i++;
+        continue;
+      }
+      if (i < t.size() && t[i] == ']') {
+        i++;
+        break;
+      }
+      throw DeckError("flow seq: expected ',' or ']' in " + t);
+    }
+    return n;
+  }
+};
+
+static Node parse_inline(const std::string& v) {
+  Flow f(v);
+  return f.value();
+}
+
+struct Line {
+  int indent;
+  std::string text;
+};
+
+// block-structure parser over pre-split lines
+struct Block {
+  std::vector<Line> L;
+  size_t p = 0;
+
+  Node parse(int indent) {
+    if (p >= L.size()) return Node();
+    if (L[p].text.rfind("- ", 0) == 0 || L[p].text == "-") return seq(L[p].indent);
+    return map(L[p].indent);
+  }
+
+  Node value_after(const std::string& rest, int parent_indent) {
+    if (!rest.empty()) return parse_inline(rest);
+    if (p < L.size() && L[p].indent > parent_indent) return parse(L[p].indent);
+    if (p < L.size() && L[p].indent == parent_indent && L[p].text.rfind("- ", 0) == 0) return seq(parent_indent);
+    return Node();
+  }
+
+  Node map(int indent) {
+    Node n;
+    n.kind = Node::Map;
+    while (p < L.size() && L[p].indent == indent && L[p].text.rfind("- ", 0) != 0) {
+      const std::string& t = L[p].text;
+      const size_t c = find_colon(t);
+      if (c == std::string::npos) throw DeckError("expected 'key:' at: " + t);
+      std::string k = trim(t.substr(0, c));
+      std::string rest = trim(t.substr(c + 1));
+      p++;
+      n.m.emplace_back(k, value_after(rest, indent));
+    }
+    return n;
+  }
+
+  Node seq(int indent) {
+    Node n;
+    n.kind = Node::Seq;
+    while (p < L.size() && L[p].indent == indent && (L[p].text.rfind("- ", 0) == 0 || L[p].text == "-")) {
+      std::string rest = L[p].text == "-" ? "" : trim(L[p].text.substr(2));
+      const int item_indent = indent + 2;
+      p++;
+      if (rest.empty()) {
+        n.q.push_back(p < L.size() && L[p].indent > indent ? parse(L[p].indent) : Node());
+      } else if (rest[0] == '{' || rest[0] == '[' || find_colon(rest) == std::string::npos) {
+        n.q.push_back(parse_inline(rest));
+      } else {
+        // "- key: value" possibly continued by more keys at item_indent
+        L.insert(L.begin() + (long)p, Line{item_indent, rest});
+        n.q.push_back(map(item_indent));
+      }
+    }
+    return n;
+  }
+
+  static size_t find_colon(const std::string& t) {
+    int depth = 0;
+    bool sq = false, dq = false;
+    for (size_t i = 0; i < t.size(); i++) {
+      const char c = t[i];
+      if (c == '\'' && !dq) sq = !sq;
+      if (c == '"' && !sq) dq = !dq;
+      if (sq || dq) continue;
+      if (c == '{' || c == '[') depth++;
+      if (c == '}' || c == ']') depth--;
+      if (c == ':' && depth == 0 && (i + 1 == t.size() || std::isspace((unsigned char)t[i + 1]))) return i;
+    }
+    return std::string::npos;
+  }
+};
+
+Node parse_yaml(const std::string& text) {
+  Block b;
+  std::istringstream in(text);
+  std::string line;
+  while (std::getline(in, line)) {
+    std::string s = strip_comment(line);
+    if (trim(s).empty()) continue;
+    int ind = 0;
+    while (ind < (int)s.size() && s[ind] == ' ') ind++;
+    b.L.push_back(Line{ind, trim(s)});
+  }
+  if (b.L.empty()) {
+    Node n;
+    n.kind = Node::Map;
+    return n;
+  }
+  return b.parse(b.L[0].indent);
+}
+
+// ------------------------------------------------------------------ values
+static double as_double(const Node& n, const std::string& what) {
+  if (n.kind != Node::Scalar) throw DeckError(what + ": expected a number");
+  char* end = nullptr;
+  const double v = std::strtod(n.s.c_str(), &end);
+  if (end == n.s.c_str()) throw DeckError(what + ": not a number: " + n.s);
+  return v;
+}
+static bool as_bool(const Node& n, const std::string& what) {
+  if (n.kind != Node::Scalar) throw DeckError(what + ": expected a boolean");
+  if (n.s == "true" || n.s == "True" || n.s == "1") return true;
+  if (n.s == "false" || n.s == "False" || n.s == "0") return false;
+  throw DeckError(what + ": not a boolean: " + n.s);
+}
+static std::string fmt(double v) {
+  char b[64];
+  std::snprintf(b, sizeof b, "%.17g", v);
+  return b;
+}
+
+// ------------------------------------------------------------------ runner
+struct Runner;
+using OpFn = std::function<void(Runner&, const Node& params)>;
+
+struct Runner {
+  Node deck;                           // merged deck definitions
+  std::map<std::string, Node> slots;   // global slot store
+  std::vector<std::map<std::string, std::string>> rebinds;
+  std::map<std::string, OpFn> ops;
+  xnb_ctx* ctx = nullptr;
+  std::ostringstream log;
+  std::vector<double> in_r, in_v;
+  std::vector<uint64_t> in_id;
+  std::vector<uint8_t> in_type;
+  int depth = 0;
+
+  std::string resolve(const std::string& name) const {
+    for (auto it = rebinds.rbegin(); it != rebinds.rend(); ++it) {
+      auto f = it->find(name);
+      if (f != it->end()) return f->second;
+    }
+    return name;
+  }
+  // slot lookup: operator parameters first, then the (rebound) global slot
+  const Node* slot(const Node& params, const std::string& name) const {
+    if (params.kind == Node::Map)
+      if (const Node* v = params.get(name)) return v;
+    auto it = slots.find(resolve(name));
+    return it == slots.end() ? nullptr : &it->second;
+  }
+  double num(const Node& params, const std::string& name, double dflt, bool required = false) const {
+    const Node* v = slot(params, name);
+    if (!v) {
+      if (required) throw DeckError("required slot '" + name + "' is not set");
+      return dflt;
+    }
+    return as_double(*v, name);
+  }
+  bool flag(const Node& params, const std::string& name, bool dflt) const {
+    const Node* v = slot(params, name);
+    return v ? as_bool(*v, name) : dflt;
+  }
+  void out(const std::string& name, const Node& v) { slots[resolve(name)] = v; }
+  void out_num(const std::string& name, double v) { out(name, Node::scalar(fmt(v))); }
+  void out_bool(const std::string& name, bool v) { out(name, Node::scalar(v ? "true" : "false")); }
+
+  void check(int rc, const char* what) {
+    if (rc != XNB_OK)
+      throw DeckError(std::string(what) + " failed (" + xnb_error_string(rc) + "): " + (ctx ? xnb_last_error(ctx) : xnb_last_error(nullptr)));
+  }
+  xnb_ctx* need_ctx(const char* what) {
+    if (!ctx) throw DeckError(std::string(what) + ": no domain/grid (run 'domain' first)");
+    return ctx;
+  }
+
+  bool eval_condition(const Node& c) {
+    if (c.kind != Node::Scalar) throw DeckError("condition must be a slot name");
+    std::string s = trim(c.s);
+    bool neg = false;
+    if (s.rfind("not ", 0) == 0) {
+      neg = true;
+      s = trim(s.substr(4));
+    }
+    if (s == "true") return !neg;
+    if (s == "false") return neg;
+    auto it = slots.find(resolve(s));
+    const bool v = it != slots.end() && as_bool(it->second, s);
+    return neg ? !v : v;
+  }
+
+  // run a named entry: deck definition, or a registered operator with the deck's parameters
+  void run_name(const std::string& name, const Node* inline_params) {
+    if (++depth > 64) throw DeckError("operator nesting too deep at '" + name + "'");
+    const Node* def = deck.get(name);
+    auto op = ops.find(name);
+    if (def && def->kind == Node::Scalar && def->s != name) {
+      if (def->s == "nop") {
+        depth--;
+        return;
+      }
+      run_name(def->s, inline_params);
+    } else if (def && (def->kind == Node::Seq || (def->kind == Node::Map && def->get("body")))) {
+      run_block(*def);
+    } else if (op != ops.end()) {
+      Node params;
+      params.kind = Node::Map;
+      if (def && def->kind == Node::Map) params = *def;
+      if (inline_params && inline_params->kind == Node::Map)
+        for (auto& kv : inline_params->m) params.set(kv.first, kv.second);
+      op->second(*this, params);
+    } else if (name == "nop") {
+    } else {
+      throw DeckError("unknown operator '" + name + "'");
+    }
+    depth--;
+  }
+
+  void run_item(const Node& item) {
+    if (item.kind == Node::Scalar) run_name(item.s, nullptr);
+    else if (item.kind == Node::Map && item.m.size() == 1 && !item.get("body")) run_name(item.m[0].first, &item.m[0].second);
+    else if (item.kind == Node::Map) run_block(item);
+    else if (item.kind == Node::Seq) run_block(item);
+  }
+
+  void run_block(const Node& b) {
+    if (b.kind == Node::Seq) {
+      for (auto& it : b.q) run_item(it);
+      return;
+    }
+    const Node* body = b.get("body");
+    const Node* cond = b.get("condition");
+    const Node* loop = b.get("loop");
+    const Node* rb = b.get("rebind");
+    std::map<std::string, std::string> alias;
+    if (rb && rb->kind == Node::Map)
+      for (auto& kv : rb->m) alias[kv.first] = resolve(kv.second.s);
+    rebinds.push_back(alias);
+    const bool is_loop = loop && as_bool(*loop, "loop");
+    long guard = 0;
+    while (true) {
+      if (cond && !eval_condition(*cond)) break;
+      if (body) {
+        if (body->kind == Node::Seq) run_block(*body);
+        else run_item(*body);
+      }
+      if (!is_loop) break;
+      if (++guard > 100000000) throw DeckError("loop does not terminate");
+    }
+    rebinds.pop_back();
+  }
+};
+
+// ------------------------------------------------------------------ operators
+static void register_ops(Runner& R) {
+  // domain (src/io/domain_init.cpp; Domain, src/core/include/exanb/core/domain.h:36-129) + init_rcb_grid
+  R.ops["domain"] = [](Runner& r, const Node& p) {
+    xnb_domain d{};
+    const Node* cs = p.get("cell_size");
+    const Node* gd = p.get("grid_dims");
+    const Node* bd = p.get("bounds");
+    const Node* pr = p.get("periodic");
+    if (!cs || !gd || !bd) throw DeckError("domain: cell_size, grid_dims and bounds are required");
+    d.cell_size = as_double(*cs, "cell_size");
+    for (int a = 0; a < 3; a++) {
+      d.grid_dims[a] = (int64_t)as_double(gd->q.at(a), "grid_dims");
+      d.bounds_min[a] = as_double(bd->q.at(0).q.at(a), "bounds");
+      d.bounds_max[a] = as_double(bd->q.at(1).q.at(a), "bounds");
+      d.periodic[a] = pr ? as_bool(pr->q.at(a), "periodic") : 1;
+    }
+    if (r.ctx) xnb_destroy(r.ctx);
+    r.ctx = nullptr;
+    r.check(xnb_create(&r.ctx, &d, 0), "domain");
+    r.slots["domain"] = p;
+  };
+  R.ops["init_rcb_grid"] = [](Runner& r, const Node&) { r.need_ctx("init_rcb_grid"); };
+  // nbh_dist (src/particle_neighbors/nbh_dist.cpp:45-72)
+  R.ops["nbh_dist"] = [](Runner& r, const Node& p) {
+    double lab, md, gd;
+    r.check(xnb_nbh_dist(r.need_ctx("nbh_dist"), r.num(p, "rcut_max", 0.0), r.num(p, "rcut_inc", 0.0), &lab, &md, &gd), "nbh_dist");
+    r.out_num("nbh_dist_lab", lab);
+    r.out_num("nbh_dist", lab);
+    r.out_num("max_displ", md);
+    r.out_num("max_displ_lab", md);
+    r.out_num("ghost_dist", gd);
+  };
+  // particles handed over through xnb_deck_set_particles (stands for a reader / lattice operator)
+  R.ops["particle_input"] = [](Runner& r, const Node&) {
+    const size_t n = r.in_id.size();
+    r.check(xnb_set_particles(r.need_ctx("particle_input"), n, r.in_r.data(), r.in_v.empty() ? nullptr : r.in_v.data(),
+                              r.in_id.data(), r.in_type.empty() ? nullptr : r.in_type.data()),
+            "particle_input");
+  };
+  // seeded FCC lattice of this library (xnb_generate_fcc)
+  R.ops["fcc_lattice"] = [](Runner& r, const Node& p) {
+    const int64_t nc = (int64_t)r.num(p, "cells", 0, true);
+    const int64_t n = 4 * nc * nc * nc;
+    r.in_r.assign(3 * (size_t)n, 0.);
+    r.in_v.assign(3 * (size_t)n, 0.);
+    r.in_id.assign((size_t)n, 0);
+    double L = 0.;
+    r.check(xnb_generate_fcc(nc, r.num(p, "density", 0, true), n, r.num(p, "jitter", 0.0), r.num(p, "temperature", 0.0),
+                             (uint64_t)r.num(p, "seed", 1.0), r.in_r.data(), r.in_v.data(), r.in_id.data(), &L),
+            "fcc_lattice");
+    r.in_type.clear();
+    r.out_num("box_length", L);
+  };
+  // move_particles (src/grid_cell_particles/move_particles.cpp:39-47)
+  R.ops["move_particles"] = [](Runner& r, const Node&) { r.check(xnb_move_particles(r.need_ctx("move_particles")), "move_particles"); };
+  // backup_r (src/io/backup_r.cpp:35-116)
+  R.ops["backup_r"] = [](Runner& r, const Node&) { r.check(xnb_backup_r(r.need_ctx("backup_r")), "backup_r"); };
+  // ghost_comm_scheme is built together with the ghost copy in ghost_update_all on one rank
+  R.ops["ghost_comm_scheme"] = [](Runner& r, const Node&) { r.need_ctx("ghost_comm_scheme"); };
+  R.ops["ghost_update_all"] = [](Runner& r, const Node&) {
+    r.check(xnb_ghost_update_all(r.need_ctx("ghost_update_all")), "ghost_update_all");
+  };
+  R.ops["ghost_update_r"] = [](Runner& r, const Node&) { r.check(xnb_ghost_update_r(r.need_ctx("ghost_update_r")), "ghost_update_r"); };
+  // particle_displ_over (src/mpi/particle_displ_over.cu:115-179): result = count > 0
+  R.ops["particle_displ_over"] = [](Runner& r, const Node& p) {
+    uint64_t cnt = 0;
+    r.check(xnb_particle_displ_over(r.need_ctx("particle_displ_over"), r.num(p, "threshold", 0.0), &cnt), "particle_displ_over");
+    r.out_bool("result", cnt > 0);
+  };
+  // chunk_neighbors (src/particle_neighbors/chunk_neighbors.cpp:47-90), config slot as in
+  // chunk_neighbors_config.h:61-83
+  R.ops["chunk_neighbors"] = [](Runner& r, const Node& p) {
+    xnb_chunk_neighbors_config c{0, 0, 0, 0, 0};
+    if (const Node* cf = r.slot(p, "config")) {
+      if (const Node* v = cf->get("build_particle_offset")) c.build_particle_offset = as_bool(*v, "build_particle_offset");
+      if (const Node* v = cf->get("dual_particle_offset")) c.dual_particle_offset = as_bool(*v, "dual_particle_offset");
+      if (const Node* v = cf->get("random_access")) c.random_access = as_bool(*v, "random_access");
+      if (const Node* v = cf->get("half_symmetric")) c.half_symmetric = as_bool(*v, "half_symmetric");
+      if (const Node* v = cf->get("skip_ghosts")) c.skip_ghosts = as_bool(*v, "skip_ghosts");
+    }
+    r.check(xnb_chunk_neighbors(r.need_ctx("chunk_neighbors"), r.num(p, "nbh_dist_lab", 0, true), &c, 0), "chunk_neighbors");
+  };
+  // single-rank no-ops of update_particles_full / parallel_update_particles
+  for (const char* nm : {"amr_grid_pairs", "rebuild_amr", "resize_particle_locks", "migrate_cell_particles", "extend_domain",
+                         "simple_cost_model", "load_balance_rcb", "grid_stats", "chunk_neighbors_stats", "print_grid"})
+    R.ops[nm] = [](Runner&, const Node&) {};
+  R.ops["trigger_load_balance"] = [](Runner& r, const Node&) { r.out_bool("trigger_load_balance", false); };
+  // zero_particle_force (src/compute/zero_particle_force.cu:19-55)
+  R.ops["zero_particle_force"] = [](Runner& r, const Node& p) {
+    r.check(xnb_zero_particle_force(r.need_ctx("zero_particle_force"), r.flag(p, "ghost", false)), "zero_particle_force");
+  };
+  // lennard_jones_force (contribs/md/lennard_jones/lennard_jones.cu:171-223); slot 'mixed' selects
+  // the FP32-pair arithmetic
+  R.ops["lennard_jones_force"] = [](Runner& r, const Node& p) {
+    const Node* cf = r.slot(p, "config");
+    if (!cf || cf->kind != Node::Map) throw DeckError("lennard_jones_force: config {epsilon, sigma} is required");
+    xnb_lj_params lj{as_double(*cf->get("epsilon"), "epsilon"), as_double(*cf->get("sigma"), "sigma")};
+    const double rcut = r.num(p, "rcut", 0.0);
+    r.out_num("rcut_max", std::max(rcut, r.num(p, "rcut_max", 0.0)));
+    r.check(xnb_lennard_jones_force(r.need_ctx("lennard_jones_force"), &lj, rcut,
+                                    r.flag(p, "mixed", false) ? XNB_PRECISION_MIXED : XNB_PRECISION_FP64, r.flag(p, "ghost", false)),
+            "lennard_jones_force");
+  };
+  // push_f_v_r / push_f_v (src/defbox/include/exanb/defbox/push_vec3_2nd_order.h:85-117,
+  // push_vec3_1st_order.h:79-106), identity xform
+  R.ops["push_f_v_r"] = [](Runner& r, const Node& p) {
+    r.check(xnb_push_f_v_r(r.need_ctx("push_f_v_r"), r.num(p, "dt", 0, true), r.num(p, "dt_scale", 0, true)), "push_f_v_r");
+  };
+  R.ops["push_f_v"] = [](Runner& r, const Node& p) {
+    r.check(xnb_push_f_v(r.need_ctx("push_f_v"), r.num(p, "dt", 0, true), r.num(p, "dt_scale", 0, true)), "push_f_v");
+  };
+  // control operators of compute-loop.msp
+  R.ops["nth_timestep"] = [](Runner& r, const Node& p) {
+    const long ts = (long)r.num(p, "timestep", 0.0);
+    const long fq = (long)r.num(p, "freq", 1.0);
+    const bool first = r.flag(p, "first", true);
+    r.out_bool("result", fq > 0 && ts % fq == 0 && (first || ts > 0));
+  };
+  R.ops["sim_continue"] = [](Runner& r, const Node& p) {
+    r.out_bool("result", r.num(p, "timestep", 0.0) < r.num(p, "end_at", 0, true));
+  };
+  R.ops["next_time_step"] = [](Runner& r, const Node& p) {
+    r.out_num("timestep", r.num(p, "timestep", 0.0) + 1.0);
+    r.out_num("physical_time", r.num(p, "physical_time", 0.0) + r.num(p, "dt", 0.0));
+  };
+  R.ops["simulation_stats"] = [](Runner& r, const Node&) {
+    double ke = 0., pe = 0.;
+    r.check(xnb_simulation_energies(r.need_ctx("simulation_stats"), &ke, &pe), "simulation_stats");
+    r.out_num("kinetic_energy", ke);
+    r.out_num("potential_energy", pe);
+  };
+  R.ops["print_simulation_stats"] = [](Runner& r, const Node& p) {
+    r.log << "step " << (long)r.num(p, "timestep", 0.0) << " ke " << fmt(r.num(p, "kinetic_energy", 0.0)) << " pe "
+          << fmt(r.num(p, "potential_energy", 0.0)) << "\n";
+  };
+  R.ops["message"] = [](Runner& r, const Node& p) {
+    if (const Node* m = p.get("text")) r.log << m->s << "\n";
+  };
+}
+
+// merge a parsed deck into the runner: "name:" replaces, "+name:" appends to a sequence
+static void merge_deck(Runner& R, const Node& doc) {
+  if (doc.kind != Node::Map) throw DeckError("deck must be a mapping at top level");
+  for (auto& kv : doc.m) {
+    if (!kv.first.empty() && kv.first[0] == '+') {
+      const std::string nm = kv.first.substr(1);
+      Node* cur = R.deck.get(nm);
+      if (!cur) {
+        R.deck.set(nm, kv.second);
+        continue;
+      }
+      if (cur->kind != Node::Seq) {
+        Node s;
+        s.kind = Node::Seq;
+        s.q.push_back(Node::scalar(nm + "__base"));
+        R.deck.set(nm + "__base", *cur);
+        *cur = s;
+      }
+      if (kv.second.kind == Node::Seq)
+        for (auto& x : kv.second.q) cur->q.push_back(x);
+      else
+        cur->q.push_back(kv.second);
+    } else if (kv.first == "global" && kv.second.kind == Node::Map) {
+      for (auto& g : kv.second.m) R.slots[g.first] = g.second;
+      R.deck.set(kv.first, kv.second);
+    } else {
+      R.deck.set(kv.first, kv.second);
+    }
+  }
+}
+
+struct Deck {
+  Runner R;
+  Deck() {
+    R.deck.kind = Node::Map;
+    register_ops(R);
+  }
+  ~Deck() {
+    if (R.ctx) xnb_destroy(R.ctx);
+  }
+};
+
+}  // namespace xnbdeck
+
+// ------------------------------------------------------------------ C ABI
+using xnbdeck::Deck;
+
+struct xnb_deck {
+  Deck d;
+  std::string err;
+  std::string log;
+};
+
+extern "C" {
+
+int xnb_deck_create(xnb_deck** out) {
+  if (!out) return XNB_E_ARG;
+  *out = new xnb_deck();
+  return XNB_OK;
+}
+
+int xnb_deck_destroy(xnb_deck* k) {
+  delete k;
+  return XNB_OK;
+}
+
+int xnb_deck_load(xnb_deck* k, const char* yaml_text) {
+  if (!k || !yaml_text) return XNB_E_ARG;
+  try {
+    xnbdeck::merge_deck(k->d.R, xnbdeck::parse_yaml(yaml_text));
+  } catch (const std::exception& e) {
+    k->err = e.what();
+    return XNB_E_ARG;
+  }
+  return XNB_OK;
+}
+
+int xnb_deck_set_particles(xnb_deck* k, size_t n, const double* r, const double* v, const uint64_t* id, const uint8_t* type) {
+  if (!k || (n && !r)) return XNB_E_ARG;
+  auto& R = k->d.R;
+  R.in_r.assign(r, r + 3 * n);
+  if (v) R.in_v.assign(v, v + 3 * n);
+  else R.in_v.clear();
+  R.in_id.resize(n);
+  for (size_t q = 0; q < n; q++) R.in_id[q] = id ? id[q] : (uint64_t)q;
+  if (type) R.in_type.assign(type, type + n);
+  else R.in_type.clear();
+  return XNB_OK;
+}
+
+int xnb_deck_run(xnb_deck* k, const char* entry) {
+  if (!k) return XNB_E_ARG;
+  try {
+    k->d.R.run_name(entry ? entry : "simulation", nullptr);
+  } catch (const std::exception& e) {
+    k->err = e.what();
+    k->log = k->d.R.log.str();
+    return XNB_E_STATE;
+  }
+  k->log = k->d.R.log.str();
+  return XNB_OK;
+}
+
+xnb_ctx* xnb_deck_context(xnb_deck* k) { return k ? k->d.R.ctx : nullptr; }
+
+const char* xnb_deck_error(const xnb_deck* k) { return k ? k->err.c_str() : ""; }
+
+const char* xnb_deck_log(const xnb_deck* k) { return k ? k->log.c_str() : ""; }
+
+int xnb_deck_slot(xnb_deck* k, const char* name, char* out, size_t cap) {
+  if (!k || !name || !out || cap == 0) return XNB_E_ARG;
+  auto it = k->d.R.slots.find(name);
+  if (it == k->d.R.slots.end() || it->second.kind != xnbdeck::Node::Scalar) return XNB_E_STATE;
+  std::snprintf(out, cap, "%s", it->second.s.c_str());
+  return XNB_OK;
+}
+
+}  // extern "C"

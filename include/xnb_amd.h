@@ -180,6 +180,25 @@ int xnb_synchronize(xnb_ctx* ctx);
  * counting read and write bytes.  Also the known-size kernel that calibrates HBM counters. */
 int xnb_copy_probe(xnb_ctx* ctx, size_t bytes, int iters, double* gbps);
 
+/* ---- operator decks ---------------------------------------------------------- */
+/* YAML-subset decks with the reference's operator names and slots (the .msp structure of
+ * data/config/numerical-scheme.msp, update-particles.msp, compute-loop.msp): named
+ * definitions, batches with condition / loop / rebind, "+name" appends, a `global` slot
+ * block.  Each registered operator forwards to the entry point of this header that
+ * replaces it.  Loading and slot queries make no GPU call. */
+typedef struct xnb_deck xnb_deck;
+int xnb_deck_create(xnb_deck** deck);
+int xnb_deck_destroy(xnb_deck* deck);
+int xnb_deck_load(xnb_deck* deck, const char* yaml_text); /* merge a deck text */
+/* particles consumed by the deck's `particle_input` operator (copied) */
+int xnb_deck_set_particles(xnb_deck* deck, size_t n, const double* r, const double* v,
+                           const uint64_t* id, const uint8_t* type);
+int xnb_deck_run(xnb_deck* deck, const char* entry); /* entry NULL: "simulation" */
+xnb_ctx* xnb_deck_context(xnb_deck* deck);          /* the box built by the `domain` operator */
+const char* xnb_deck_error(const xnb_deck* deck);
+const char* xnb_deck_log(const xnb_deck* deck);
+int xnb_deck_slot(xnb_deck* deck, const char* name, char* out, size_t cap);
+
 /* ---- input generation (host only, no GPU call) ------------------------------ */
 /* FCC lattice of ncell^3 unit cells at reduced density rho, first n_atoms sites in lattice
  * order (n_atoms <= 4 ncell^3; pass 4 ncell^3 for a full lattice), each coordinate jittered
