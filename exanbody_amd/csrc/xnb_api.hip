@@ -64,7 +64,7 @@ struct xnb_ctx {
   // scan / scalar scratch
   DArr<uint32_t> bsum, u32s;            // u32s: [0] err, [1] scan total, [2] max count
   DArr<unsigned long long> u64s;        // [0] displacement count, [1..2] pair counts
-  DArr<unsigned long long> step_counts; // per-step displacement counts (interval mode)
+  DArr<unsigned long long> step_counts; // per-step displacement counts of xnb_run
   DArr<double> red_part, red_out;
   uint32_t* h_u32 = nullptr;            // pinned
   unsigned long long* h_u64 = nullptr;  // pinned
@@ -86,7 +86,7 @@ struct xnb_ctx {
   DArr<uint16_t> blist;
   DArr<uint32_t> bcnt;
   bool slot_lists_current = false;  // nbr/nbr_cnt mirror the brick lists
-  bool build_stage = true;          // brick build stages lists in LDS before writing rows
+  bool fuse_push = true;            // force kernel performs the next step's first half (brick lists)
   uint32_t lds_limit = 65536;
 
   double last_rcut = 0.;
@@ -368,7 +368,7 @@ inline uint32_t grow_cap(uint32_t mx) {
 
 // Brick dims, T and Rmax for the current cell occupancy; NOT_APPLICABLE if nothing fits LDS.
 int choose_bricks(xnb_ctx* c, uint32_t cap, BrickParams& out) {
-  static const int cand[][3] = {{4, 2, 2}, {2, 2, 2}, {4, 2, 1}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
+  static const int cand[][3] = {{4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {4, 2, 1}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
   const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
   int e;
   for (int q = -1; q < ncand; q++) {
@@ -396,7 +396,7 @@ int choose_bricks(xnb_ctx* c, uint32_t cap, BrickParams& out) {
     bp.T = T;
     bp.Rmax = std::max<uint32_t>(r_max, 1u);
     bp.cap = cap;
-    if (brick_layout(bp, c->build_stage) > c->lds_limit) continue;
+    if (brick_layout(bp) > c->lds_limit) continue;
     out = bp;
     return XNB_OK;
   }
@@ -415,7 +415,7 @@ int op_nbh_brick(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& 
     if ((e = dalloc(c, c->bcnt, (size_t)bp.nbricks * bp.T))) return e;
     HIPCHK(c, hipMemsetAsync(c->u32s.p + 2, 0, sizeof(uint32_t), c->st));
     tstart(c, XNB_TIMER_NBH);
-    hipError_t he = launch_nbh_brick(c->g, bp, c->build_stage, c->A.r, c->cell_off.p, c->cell_n.p, max2, cfg.half_symmetric, cfg.skip_ghosts,
+    hipError_t he = launch_nbh_brick(c->g, bp, c->A.r, c->cell_off.p, c->cell_n.p, max2, cfg.half_symmetric, cfg.skip_ghosts,
                                      c->blist.p, c->bcnt.p, c->u32s.p + 2, c->st);
     tstop(c);
     if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick neighbour build: ") + hipGetErrorString(he));
@@ -490,13 +490,18 @@ int ensure_slot_lists(xnb_ctx* c) {
 }
 
 // LJ force over the current lists: brick kernel for brick lists, generic kernel otherwise.
-int launch_force(xnb_ctx* c, int precision, bool accum, bool push, bool ghost, double eps, double sigma, double rcut, double dth) {
+// With next (brick lists only, no accum), the kernel also advances the next step's first
+// half into buffer B's position array, which the caller then swaps in.
+int launch_force(xnb_ctx* c, int precision, bool accum, bool push, bool ghost, double eps, double sigma, double rcut, double dth,
+                 const NextStep* next = nullptr) {
   if (c->list_kind == 1 && !ghost) {
-    hipError_t he = launch_lj_brick(precision, accum, push, c->g, c->bp, c->A, c->cell_off.p, c->cell_n.p, c->blist.p, c->bcnt.p, eps,
-                                    sigma, rcut, dth, c->st);
+    const int epi = next ? EPI_NEXT : (push ? EPI_KICK : EPI_NONE);
+    hipError_t he = launch_lj_brick(precision, accum, epi, c->g, c->bp, c->A, c->cell_off.p, c->cell_n.p, c->blist.p, c->bcnt.p, eps,
+                                    sigma, rcut, dth, next, c->st);
     if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick LJ kernel: ") + hipGetErrorString(he));
     return XNB_OK;
   }
+  if (next) return fail(c, XNB_E_STATE, "fused next-step push needs brick lists");
   const uint32_t n_atoms = ghost ? c->N + c->G : c->N;
   if (c->list_kind != 0 || c->list_atoms < n_atoms) return fail(c, XNB_E_STATE, "neighbour lists do not cover the requested cells");
   launch_lj(precision, accum, push, n_atoms, c->A, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, eps, sigma, rcut, dth, c->st);
@@ -592,7 +597,7 @@ int xnb_create(xnb_ctx** out, const xnb_domain* d, int device) {
     if (v[0] == '0' && v[1] == 0) c->use_bricks = false;
     else std::sscanf(v, "%d,%d,%d", &c->brick_dims[0], &c->brick_dims[1], &c->brick_dims[2]);
   }
-  if (const char* v = std::getenv("XNB_BUILD_STAGE")) c->build_stage = v[0] != '0';
+  if (const char* v = std::getenv("XNB_FUSE_PUSH")) c->fuse_push = v[0] != '0';
   HIPCHK(c, hipMemsetAsync(c->cell_n.p, 0, c->cell_n.n * sizeof(uint32_t), c->st));
   HIPCHK(c, hipMemsetAsync(c->cell_off.p, 0, c->cell_off.n * sizeof(uint32_t), c->st));
   *out = c;
@@ -993,6 +998,11 @@ int xnb_init(xnb_ctx* c, const xnb_lj_params* p, double rcut, double rcut_inc, i
   return XNB_OK;
 }
 
+// Velocity-Verlet loop (numerical-scheme.msp: push_f_v_r, push_f_v, check_and_update,
+// compute_force, push_f_v).  When the lists are brick lists, the force kernel of step s < steps
+// also performs step s+1's first half and displacement count, writing positions to buffer B
+// whose position array is then swapped with A's; the arithmetic is the same as the separate
+// push kernel's.  Displacement counts go to step_counts[step - 1].
 int xnb_run(xnb_ctx* c, xnb_run_params* p) {
   int e;
   if ((e = check_ctx(c))) return e;
@@ -1004,22 +1014,24 @@ int xnb_run(xnb_ctx* c, xnb_run_params* p) {
   c->last_rcut = p->rcut;
   p->n_rebuilds = 0;
   p->n_violations = 0;
-  if (interval && p->steps > 0) {
+  if (p->steps > 0) {
     if ((e = dalloc(c, c->step_counts, (size_t)p->steps))) return e;
     HIPCHK(c, hipMemsetAsync(c->step_counts.p, 0, (size_t)p->steps * sizeof(unsigned long long), c->st));
   }
   std::vector<char> rebuilt(interval ? (size_t)p->steps : 0, 0);
+  bool first_half_done = false;  // step's first half already done by the previous force kernel
   for (int64_t step = 1; step <= p->steps; step++) {
-    unsigned long long* cnt = interval ? c->step_counts.p + (step - 1) : c->u64s.p;
-    if (!interval) HIPCHK(c, hipMemsetAsync(c->u64s.p, 0, sizeof(unsigned long long), c->st));
-    tstart(c, XNB_TIMER_PUSH);
-    launch_verlet_first_half(c->g, c->N, c->A, dt, dt2, dth, c->slot_cell_A.p, c->backup.p, thr2, cnt, c->st);
-    tstop(c);
+    unsigned long long* cnt = c->step_counts.p + (step - 1);
+    if (!first_half_done) {
+      tstart(c, XNB_TIMER_PUSH);
+      launch_verlet_first_half(c->g, c->N, c->A, dt, dt2, dth, c->slot_cell_A.p, c->backup.p, thr2, cnt, c->st);
+      tstop(c);
+    }
     bool trigger;
     if (interval) {
       trigger = (step % p->rebuild_every) == 0;
     } else {
-      HIPCHK(c, hipMemcpyAsync(c->h_u64, c->u64s.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(c, hipMemcpyAsync(c->h_u64, cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
       HIPCHK(c, hipStreamSynchronize(c->st));
       trigger = c->h_u64[0] > 0;
     }
@@ -1032,10 +1044,17 @@ int xnb_run(xnb_ctx* c, xnb_run_params* p) {
       launch_ghost_update_r(c->g, c->N, c->G, c->A.r, c->ghost_src.p, c->ghost_shift.p, c->st);
       tstop(c);
     }
+    const bool fuse = c->fuse_push && step < p->steps && c->list_kind == 1;
+    NextStep nx{c->rB.p, c->slot_cell_A.p, c->backup.p, dt, dt2, thr2, cnt + 1};
     tstart(c, XNB_TIMER_FORCE);
-    e = launch_force(c, p->precision, false, true, false, p->lj.epsilon, p->lj.sigma, p->rcut, dth);
+    e = launch_force(c, p->precision, false, true, false, p->lj.epsilon, p->lj.sigma, p->rcut, dth, fuse ? &nx : nullptr);
     tstop(c);
     if (e) return e;
+    if (fuse) {
+      std::swap(c->rA, c->rB);
+      bind_states(c);
+    }
+    first_half_done = fuse;
   }
   HIPCHK(c, hipGetLastError());
   if (interval && p->steps > 0) {

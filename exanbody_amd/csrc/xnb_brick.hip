@@ -5,10 +5,12 @@
 // ghost cells included) into LDS, region cells in k-major order, so a neighbour is addressed
 // by its u16 LDS index.  Lanes are the brick's atoms in (cell k-major, particle) order.
 //
-// Neighbour lists are u16 LDS indices, one slab of cap rows x T lanes per brick, written as
-// whole coalesced rows.  Because region cells are k-major and candidate cells are visited in
-// (dk, dj, di) order with particles in cell order, every list is in ascending (cell_enc, p_b)
-// order -- the order of the reference stream (chunk_neighbors_execute.h:308) -- and the
+// Neighbour lists are u16 LDS indices, one slab of cap entries x T lanes per brick.  Entries
+// are grouped by four: entry q of lane t sits at ((q / 4) * T + t) * 4 + q % 4, so a lane
+// reads four consecutive entries with one 8-byte load and a wave's loads are contiguous.
+// Because region cells are k-major and candidate cells are visited in (dk, dj, di) order
+// with particles in cell order, every list is in ascending (cell_enc, p_b) order -- the order
+// of the reference stream (chunk_neighbors_execute.h:308) -- and the
 // forces are summed in that order.
 //
 // Requires cell_layers == 1 (cell edge >= nbh_dist) and ghost layers >= 1.
@@ -39,11 +41,7 @@ __device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total) {
   return x - v + woff;
 }
 
-__device__ inline uint32_t wave_max_b(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
-  return v;
-}
+__device__ inline uint32_t wave_max_b(uint32_t v) { return wave_max_u32(v); }
 
 // largest k in [0, n) with pref[k] <= x (pref non-decreasing, pref[0] = 0 <= x)
 __device__ inline uint32_t upper_index(const uint32_t* pref, uint32_t n, uint32_t x) {
@@ -56,10 +54,12 @@ __device__ inline uint32_t upper_index(const uint32_t* pref, uint32_t n, uint32_
   return lo;
 }
 
+// position of entry q of lane t in a brick's list slab (T lanes)
+__device__ inline size_t lpos(uint32_t q, uint32_t t, uint32_t T) { return ((size_t)(q >> 2) * T + t) * 4 + (q & 3u); }
+
 struct BrickLds {
   uint32_t *rcnt, *rpref, *roff, *bpref;
   double *px, *py, *pz;
-  uint16_t* stage;
 };
 
 __device__ inline BrickLds brick_lds(const BrickParams& bp, char* lds) {
@@ -68,10 +68,9 @@ __device__ inline BrickLds brick_lds(const BrickParams& bp, char* lds) {
   L.rpref = (uint32_t*)(lds + bp.off_rpref);
   L.roff = (uint32_t*)(lds + bp.off_roff);
   L.bpref = (uint32_t*)(lds + bp.off_bpref);
-  L.px = (double*)(lds + bp.off_px);
-  L.py = (double*)(lds + bp.off_py);
-  L.pz = (double*)(lds + bp.off_pz);
-  L.stage = (uint16_t*)(lds + bp.off_stage);
+  L.px = (double*)(lds + bp.off_pos);  // interleaved x, y, z: atom i at [3 i, 3 i + 2]
+  L.py = L.px + 1;
+  L.pz = L.px + 2;
   return L;
 }
 
@@ -136,9 +135,9 @@ __device__ inline void brick_setup(const Geom& g, const BrickParams& bp, const B
     for (uint32_t idx = t; idx < R; idx += blockDim.x) {
       const uint32_t rc = upper_index(L.rpref, (uint32_t)bp.nrc, idx);
       const double4 p = r[L.roff[rc] + (idx - L.rpref[rc])];
-      L.px[idx] = p.x;
-      L.py[idx] = p.y;
-      L.pz[idx] = p.z;
+      L.px[3 * idx] = p.x;
+      L.py[3 * idx] = p.y;
+      L.pz[3 * idx] = p.z;
     }
     __syncthreads();
   }
@@ -194,16 +193,18 @@ __global__ void k_brick_stats(Geom g, BrickParams bp, const uint32_t* __restrict
 }
 
 // ------------------------------------------------------------------ build
-// chunk_neighbors (chunk_neighbors_execute.h:221-239): d = r_a - r_b, FP64,
-// kept iff (a != b) && 0 < d2 <= nbh_dist^2, then the NeighborFilterHalfSymGhost filters.
-// Candidates are tested UB at a time (independent LDS reads and FP64 chains) and appended in
-// order.  STAGE: entries go to an LDS slab first and leave as whole rows; otherwise each
-// accepted entry is stored straight into its row of the global slab.
-template <bool STAGE, int UB>
-__global__ void __launch_bounds__(1024) k_nbh_brick(Geom g, BrickParams bp, const double4* __restrict__ r,
-                                                    const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
-                                                    double max2, int half_symmetric, int skip_ghosts, uint16_t* __restrict__ blist,
-                                                    uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt) {
+// chunk_neighbors (chunk_neighbors_execute.h:221-239): d = r_a - r_b, FP64, kept iff
+// (a != b) && 0 < d2 <= nbh_dist^2, then the NeighborFilterHalfSymGhost filters.
+// Candidates are tested in chunks of 32 without branches (bit q of m set iff candidate
+// p0 + q is kept), then each lane expands its mask in ascending bit order, which appends
+// entries in candidate order.  Stores go to row
+// min(cnt, cap - 1): an overflowing list is rebuilt with a larger cap, so its last row may be
+// overwritten.  FILTER enables the half_symmetric / skip_ghosts filters.
+template <bool FILTER>
+__global__ void __launch_bounds__(1024) k_nbh_brick_mask(Geom g, BrickParams bp, const double4* __restrict__ r,
+                                                         const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
+                                                         double max2, int half_symmetric, int skip_ghosts, uint16_t* __restrict__ blist,
+                                                         uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt) {
   extern __shared__ __align__(16) char lds[];
   const uint32_t b = blockIdx.x;
   const BrickLds L = brick_lds(bp, lds);
@@ -211,18 +212,19 @@ __global__ void __launch_bounds__(1024) k_nbh_brick(Geom g, BrickParams bp, cons
   brick_setup(g, bp, P, L, r, cell_off, cell_n, true);
   const LaneAtom A = lane_atom(bp, L);
   const uint32_t t = threadIdx.x, T = blockDim.x;
-  uint16_t* out = blist + (size_t)b * bp.cap * T + t;
-  double ax = 0., ay = 0., az = 0.;
+  uint16_t* out = blist + (size_t)b * bp.cap * T;
+  // every lane, live or not, tests against its own LDS atom when past the candidate cell's
+  // end (A.ia = 0 for lanes without an atom), so d2 == 0 rejects those candidates
+  const double ax = L.px[3 * A.ia], ay = L.py[3 * A.ia], az = L.pz[3 * A.ia];
   uint32_t ca = 0;
   if (A.live) {
-    ax = L.px[A.ia];
-    ay = L.py[A.ia];
-    az = L.pz[A.ia];
-    const int ri = (int)(A.rc % (uint32_t)bp.RX), rj = (int)((A.rc / (uint32_t)bp.RX) % (uint32_t)bp.RY),
-              rk = (int)(A.rc / ((uint32_t)bp.RX * (uint32_t)bp.RY));
-    ca = cell_index(g, P.gi0 - 1 + ri, P.gj0 - 1 + rj, P.gk0 - 1 + rk);
+    if (FILTER) {
+      const int ri = (int)(A.rc % (uint32_t)bp.RX), rj = (int)((A.rc / (uint32_t)bp.RX) % (uint32_t)bp.RY),
+                rk = (int)(A.rc / ((uint32_t)bp.RX * (uint32_t)bp.RY));
+      ca = cell_index(g, P.gi0 - 1 + ri, P.gj0 - 1 + rj, P.gk0 - 1 + rk);
+    }
   }
-  const uint32_t cap = bp.cap;
+  const uint32_t last = bp.cap - 1;
   uint32_t cnt = 0;
   for (int dk = -1; dk <= 1; dk++)
     for (int dj = -1; dj <= 1; dj++)
@@ -231,88 +233,113 @@ __global__ void __launch_bounds__(1024) k_nbh_brick(Geom g, BrickParams bp, cons
         const uint32_t nb = A.live ? L.rcnt[rcb] : 0u;
         const uint32_t base = A.live ? L.rpref[rcb] : 0u;
         const uint32_t nbmax = wave_max_b(nb);
-        const uint32_t cb = ca + (uint32_t)((dk * g.dj + dj) * g.di + di);
-        bool bghost = false;
-        if (skip_ghosts && A.live) {
-          int ii, jj, kk;
-          cell_ijk(g, cb, ii, jj, kk);
-          bghost = is_ghost(g, ii, jj, kk);
-        }
-        for (uint32_t p0 = 0; p0 < nbmax; p0 += UB) {
-          bool kp[UB];
-#pragma unroll
-          for (int u = 0; u < UB; u++) {
-            const uint32_t p = p0 + u;
-            const bool ok = p < nb;
-            const uint32_t ib = ok ? base + p : A.ia;
-            const double dx = ax - L.px[ib];
-            const double dy = ay - L.py[ib];
-            const double dz = az - L.pz[ib];
-            const double d2 = (dx * dx + dy * dy) + dz * dz;
-            bool keep = ok && ib != A.ia && d2 > 0.0 && d2 <= max2;
-            if (half_symmetric && (ca < cb || (ca == cb && A.p < p))) keep = false;
-            if (skip_ghosts && bghost) keep = false;
-            kp[u] = keep;
+        bool drop_cell = false;
+        uint32_t cb = 0;
+        if (FILTER) {
+          cb = ca + (uint32_t)((dk * g.dj + dj) * g.di + di);
+          if (skip_ghosts && A.live) {
+            int ii, jj, kk;
+            cell_ijk(g, cb, ii, jj, kk);
+            drop_cell = is_ghost(g, ii, jj, kk);
           }
+          if (half_symmetric && ca < cb) drop_cell = true;
+        }
+        for (uint32_t p0 = 0; p0 < nbmax; p0 += 32) {
+          const uint32_t pend = min(nbmax - p0, 32u);
+          uint32_t m = 0;
+          for (uint32_t q = 0; q < pend; q += 4) {
 #pragma unroll
-          for (int u = 0; u < UB; u++) {
-            if (kp[u]) {
-              if (cnt < cap) {
-                if (STAGE) L.stage[(size_t)cnt * T + t] = (uint16_t)(base + p0 + u);
-                else out[(size_t)cnt * T] = (uint16_t)(base + p0 + u);
-              }
-              ++cnt;
+            for (uint32_t u = 0; u < 4; u++) {
+              const uint32_t p = p0 + q + u;
+              const uint32_t ib = p < nb ? base + p : A.ia;  // own index: d2 == 0 is rejected
+              const double dx = ax - L.px[3 * ib];
+              const double dy = ay - L.py[3 * ib];
+              const double dz = az - L.pz[3 * ib];
+              const double d2 = (dx * dx + dy * dy) + dz * dz;
+              bool keep = d2 > 0.0 && d2 <= max2;
+              if (FILTER && half_symmetric && ca == cb && A.p < p) keep = false;
+              m |= keep ? (1u << (q + u)) : 0u;
             }
+          }
+          if (FILTER && drop_cell) m = 0;
+          while (m) {
+            const uint32_t bit = (uint32_t)__builtin_ctz(m);
+            m &= m - 1u;
+            out[lpos(min(cnt, last), t, T)] = (uint16_t)(base + p0 + bit);
+            ++cnt;
           }
         }
       }
-  if (STAGE) {  // coalesced write-out: row q of the slab holds entry q of every lane
-    const uint32_t keep_n = min(cnt, cap);
-    const uint32_t rows = wave_max_b(keep_n);
-    for (uint32_t q = 0; q < rows; q++)
-      if (q < keep_n) out[(size_t)q * T] = L.stage[(size_t)q * T + t];
-  }
-  bcnt[(size_t)b * T + t] = A.live ? cnt : 0u;
+  if (!A.live) cnt = 0;
+  bcnt[(size_t)b * T + t] = cnt;
   const uint32_t wmax = wave_max_b(cnt);
   if ((t & 63) == 0 && wmax > 0) atomicMax(max_cnt, wmax);
 }
 
 // ------------------------------------------------------------------ force
-template <bool MIXED, bool ACCUM, bool PUSH, int U>
+// EPI selects the epilogue after the force store (f = 0 + sum, or f += sum with ACCUM):
+//   EPI_NONE   nothing more;
+//   EPI_KICK   v += f dth, the closing push_f_v of the step (numerical-scheme.msp);
+//   EPI_NEXT   EPI_KICK, then the next step's push_f_v_r / push_f_v
+//              (x += v dt + f dt2; v += f dth) into NextStep::r_next, and the
+//              particle_displ_over count of the new positions against the backup.
+// Positions are read from r (staged in LDS) and, with EPI_NEXT, written to a second buffer,
+// so no workgroup reads a position another one has already advanced.
+template <bool MIXED, bool ACCUM, int EPI, int U>
 __global__ void __launch_bounds__(1024) k_lj_brick(Geom g, BrickParams bp, const double4* __restrict__ r,
                                                    const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
                                                    const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt,
-                                                   LJConst lj, double* __restrict__ fx, double* __restrict__ fy,
-                                                   double* __restrict__ fz, double* __restrict__ ep, double* __restrict__ vx,
-                                                   double* __restrict__ vy, double* __restrict__ vz, double dth) {
+                                                   LJConst lj, PState A, double dth, NextStep nx) {
   extern __shared__ __align__(16) char lds[];
   const uint32_t b = blockIdx.x;
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
   brick_setup(g, bp, P, L, r, cell_off, cell_n, true);
-  const LaneAtom A = lane_atom(bp, L);
+  const LaneAtom At = lane_atom(bp, L);
   const uint32_t t = threadIdx.x, T = blockDim.x;
-  const uint32_t n = A.live ? bcnt[(size_t)b * T + t] : 0u;
+  const uint32_t n = At.live ? bcnt[(size_t)b * T + t] : 0u;
   const uint32_t nmax = wave_max_b(n);
   double4 ra;
-  ra.x = L.px[A.ia];
-  ra.y = L.py[A.ia];
-  ra.z = L.pz[A.ia];
+  ra.x = L.px[3 * At.ia];
+  ra.y = L.py[3 * At.ia];
+  ra.z = L.pz[3 * At.ia];
   ra.w = 0.;
-  const uint16_t* lst = blist + (size_t)b * bp.cap * T + t;
+  // U entries per batch = U / 4 eight-byte loads per lane; the next batch's loads are issued
+  // before the current batch is computed.  Batches never pass cap (a multiple of U).
+  static_assert(U % 4 == 0, "batches are whole groups of four entries");
+  constexpr int W = U / 4;
+  const uint2* lst = reinterpret_cast<const uint2*>(blist + (size_t)b * bp.cap * T) + t;
   double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
+  uint2 wn[W];
+  if (nmax > 0) {
+#pragma unroll
+    for (int w = 0; w < W; w++) wn[w] = lst[(size_t)w * T];
+  }
   for (uint32_t q0 = 0; q0 < nmax; q0 += U) {
+    uint2 wc[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) wc[w] = wn[w];
+    if (q0 + U < nmax) {
+      const uint2* nxt = lst + (size_t)((q0 + U) >> 2) * T;
+#pragma unroll
+      for (int w = 0; w < W; w++) wn[w] = nxt[(size_t)w * T];
+    }
     uint32_t j[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) j[u] = lst[(size_t)(q0 + u) * T];  // rows < cap: in bounds
+    for (int w = 0; w < W; w++) {
+      j[4 * w + 0] = wc[w].x & 0xffffu;
+      j[4 * w + 1] = wc[w].x >> 16;
+      j[4 * w + 2] = wc[w].y & 0xffffu;
+      j[4 * w + 3] = wc[w].y >> 16;
+    }
 #pragma unroll
-    for (int u = 0; u < U; u++) j[u] = (q0 + u < n) ? j[u] : A.ia;  // own index: rejected by d2 > 0
+    for (int u = 0; u < U; u++) j[u] = (q0 + u < n) ? j[u] : At.ia;  // own index: rejected by d2 > 0
     double4 rb[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      rb[u].x = L.px[j[u]];
-      rb[u].y = L.py[j[u]];
-      rb[u].z = L.pz[j[u]];
+      rb[u].x = L.px[3 * j[u]];
+      rb[u].y = L.py[3 * j[u]];
+      rb[u].z = L.pz[3 * j[u]];
       rb[u].w = 0.;
     }
 #pragma unroll
@@ -321,14 +348,32 @@ __global__ void __launch_bounds__(1024) k_lj_brick(Geom g, BrickParams bp, const
       else lj_pair_fp64(lj, ra, rb[u], tx, ty, tz, te);
     }
   }
-  if (!A.live) return;
-  const uint32_t s = A.slot;
-  store_force<ACCUM>(s, tx, ty, tz, te, fx, fy, fz, ep);
-  if (PUSH) {
-    vx[s] += fx[s] * dth;
-    vy[s] += fy[s] * dth;
-    vz[s] += fz[s] * dth;
+  bool over = false;
+  if (At.live) {
+    const uint32_t s = At.slot;
+    store_force<ACCUM>(s, tx, ty, tz, te, A.fx, A.fy, A.fz, A.ep);
+    if (EPI != EPI_NONE) {
+      const double ax = A.fx[s], ay = A.fy[s], az = A.fz[s];
+      double ux = A.vx[s] + ax * dth;
+      double uy = A.vy[s] + ay * dth;
+      double uz = A.vz[s] + az * dth;
+      if (EPI == EPI_NEXT) {
+        double4 p = r[s];
+        p.x += ux * nx.dt + ax * nx.dt2;
+        p.y += uy * nx.dt + ay * nx.dt2;
+        p.z += uz * nx.dt + az * nx.dt2;
+        nx.r_next[s] = p;
+        ux += ax * dth;
+        uy += ay * dth;
+        uz += az * dth;
+        over = displaced(g, p, nx.slot_cell[s], nx.backup, s, nx.thr2);
+      }
+      A.vx[s] = ux;
+      A.vy[s] = uy;
+      A.vz[s] = uz;
+    }
   }
+  if (EPI == EPI_NEXT) wave_count_add(over, nx.count);
 }
 
 // ------------------------------------------------------------------ conversion
@@ -347,10 +392,10 @@ __global__ void __launch_bounds__(1024) k_brick_to_slots(Geom g, BrickParams bp,
   if (!A.live) return;
   const uint32_t t = threadIdx.x, T = blockDim.x;
   const uint32_t n = min(bcnt[(size_t)b * T + t], bp.cap);
-  const uint16_t* lst = blist + (size_t)b * bp.cap * T + t;
+  const uint16_t* lst = blist + (size_t)b * bp.cap * T;
   uint32_t* outp = nbr + nbr_index(A.slot, 0, cap);
   for (uint32_t q = 0; q < n; q++) {
-    const uint32_t idx = lst[(size_t)q * T];
+    const uint32_t idx = lst[lpos(q, t, T)];
     const uint32_t rc = upper_index(L.rpref, (uint32_t)bp.nrc, idx);
     outp[(size_t)q << 6] = L.roff[rc] + (idx - L.rpref[rc]);
   }
@@ -377,8 +422,8 @@ BrickParams brick_params(const Geom& g, int Bi, int Bj, int Bk) {
 
 static inline uint32_t align8(uint32_t x) { return (x + 7u) & ~7u; }
 
-// LDS layout; returns bytes for the build (with_stage) or force kernel
-uint32_t brick_layout(BrickParams& bp, bool with_stage) {
+// LDS layout of the build, force and conversion kernels; returns its bytes
+uint32_t brick_layout(BrickParams& bp) {
   uint32_t o = 0;
   bp.off_rcnt = o;
   o = align8(o + 4u * bp.nrc);
@@ -388,14 +433,8 @@ uint32_t brick_layout(BrickParams& bp, bool with_stage) {
   o = align8(o + 4u * bp.nrc);
   bp.off_bpref = o;
   o = align8(o + 4u * (bp.nbc + 1));
-  bp.off_px = o;
-  o += 8u * bp.Rmax;
-  bp.off_py = o;
-  o += 8u * bp.Rmax;
-  bp.off_pz = o;
-  o += 8u * bp.Rmax;
-  bp.off_stage = o;
-  if (with_stage) o += 2u * bp.cap * bp.T;
+  bp.off_pos = o;
+  o += 24u * bp.Rmax;
   return align8(o);
 }
 
@@ -407,65 +446,60 @@ static hipError_t set_lds(const void* fn, uint32_t bytes) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-static constexpr int NBB_UNROLL = 4;
-
-hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, bool stage, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n,
+hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n,
                             double max2, int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt, uint32_t* max_cnt,
                             hipStream_t st) {
-  const uint32_t lds = brick_layout(bp, stage);
+  const uint32_t lds = brick_layout(bp);
   hipError_t e;
-  if (stage) {
-    e = set_lds((const void*)k_nbh_brick<true, NBB_UNROLL>, lds);
+  if (half_symmetric || skip_ghosts) {
+    e = set_lds((const void*)k_nbh_brick_mask<true>, lds);
     if (e != hipSuccess) return e;
-    k_nbh_brick<true, NBB_UNROLL><<<bp.nbricks, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist,
-                                                                 bcnt, max_cnt);
+    k_nbh_brick_mask<true><<<bp.nbricks, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist, bcnt,
+                                                          max_cnt);
   } else {
-    e = set_lds((const void*)k_nbh_brick<false, NBB_UNROLL>, lds);
+    e = set_lds((const void*)k_nbh_brick_mask<false>, lds);
     if (e != hipSuccess) return e;
-    k_nbh_brick<false, NBB_UNROLL><<<bp.nbricks, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist,
-                                                                  bcnt, max_cnt);
+    k_nbh_brick_mask<false><<<bp.nbricks, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, 0, 0, blist, bcnt, max_cnt);
   }
   return hipGetLastError();
 }
 
 static constexpr int LJB_UNROLL = 8;
 
-template <bool MIXED, bool ACCUM, bool PUSH>
+template <bool MIXED, bool ACCUM, int EPI>
 static hipError_t ljb_launch(const Geom& g, const BrickParams& bp, uint32_t lds, const double4* r, const uint32_t* cell_off,
                              const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const LJConst& lj, const PState& A,
-                             double dth, hipStream_t st) {
-  hipError_t e = set_lds((const void*)k_lj_brick<MIXED, ACCUM, PUSH, LJB_UNROLL>, lds);
+                             double dth, const NextStep& nx, hipStream_t st) {
+  hipError_t e = set_lds((const void*)k_lj_brick<MIXED, ACCUM, EPI, LJB_UNROLL>, lds);
   if (e != hipSuccess) return e;
-  k_lj_brick<MIXED, ACCUM, PUSH, LJB_UNROLL><<<bp.nbricks, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, blist, bcnt, lj, A.fx, A.fy,
-                                                                          A.fz, A.ep, A.vx, A.vy, A.vz, dth);
+  k_lj_brick<MIXED, ACCUM, EPI, LJB_UNROLL><<<bp.nbricks, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, blist, bcnt, lj, A, dth, nx);
   return hipGetLastError();
 }
 
 uint32_t brick_list_granule() { return LJB_UNROLL; }
 
-hipError_t launch_lj_brick(int precision, bool accum, bool push, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
+hipError_t launch_lj_brick(int precision, bool accum, int epi, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
                            const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, double eps, double sigma, double rcut,
-                           double dth, hipStream_t st) {
+                           double dth, const NextStep* next, hipStream_t st) {
   const LJConst lj = make_lj_const(eps, sigma, rcut);
-  const uint32_t lds = brick_layout(bp, false);
-  const int key = (precision ? 4 : 0) | (accum ? 2 : 0) | (push ? 1 : 0);
-#define XNB_LJB_ARGS g, bp, lds, A.r, cell_off, cell_n, blist, bcnt, lj, A, dth, st
-  switch (key) {
-    case 0: return ljb_launch<false, false, false>(XNB_LJB_ARGS);
-    case 1: return ljb_launch<false, false, true>(XNB_LJB_ARGS);
-    case 2: return ljb_launch<false, true, false>(XNB_LJB_ARGS);
-    case 3: return ljb_launch<false, true, true>(XNB_LJB_ARGS);
-    case 4: return ljb_launch<true, false, false>(XNB_LJB_ARGS);
-    case 5: return ljb_launch<true, false, true>(XNB_LJB_ARGS);
-    case 6: return ljb_launch<true, true, false>(XNB_LJB_ARGS);
-    default: return ljb_launch<true, true, true>(XNB_LJB_ARGS);
+  const uint32_t lds = brick_layout(bp);
+  const NextStep nx = next ? *next : NextStep{};
+  if (epi == EPI_NEXT && (accum || !next)) return hipErrorInvalidValue;
+#define XNB_LJB_ARGS g, bp, lds, A.r, cell_off, cell_n, blist, bcnt, lj, A, dth, nx, st
+  if (precision) {
+    if (epi == EPI_NEXT) return ljb_launch<true, false, EPI_NEXT>(XNB_LJB_ARGS);
+    if (accum) return epi ? ljb_launch<true, true, EPI_KICK>(XNB_LJB_ARGS) : ljb_launch<true, true, EPI_NONE>(XNB_LJB_ARGS);
+    return epi ? ljb_launch<true, false, EPI_KICK>(XNB_LJB_ARGS) : ljb_launch<true, false, EPI_NONE>(XNB_LJB_ARGS);
   }
+  if (epi == EPI_NEXT) return ljb_launch<false, false, EPI_NEXT>(XNB_LJB_ARGS);
+  if (accum) return epi ? ljb_launch<false, true, EPI_KICK>(XNB_LJB_ARGS) : ljb_launch<false, true, EPI_NONE>(XNB_LJB_ARGS);
+  return epi ? ljb_launch<false, false, EPI_KICK>(XNB_LJB_ARGS) : ljb_launch<false, false, EPI_NONE>(XNB_LJB_ARGS);
 #undef XNB_LJB_ARGS
 }
 
 hipError_t launch_brick_to_slots(const Geom& g, BrickParams bp, const uint32_t* cell_off, const uint32_t* cell_n, const uint16_t* blist,
                                  const uint32_t* bcnt, uint32_t* nbr, uint32_t* nbr_cnt, uint32_t cap, hipStream_t st) {
-  const uint32_t lds = brick_layout(bp, false);
+  const uint32_t lds = brick_layout(bp);
   hipError_t e = set_lds((const void*)k_brick_to_slots, lds);
   if (e != hipSuccess) return e;
   k_brick_to_slots<<<bp.nbricks, bp.T, lds, st>>>(g, bp, cell_off, cell_n, blist, bcnt, nbr, nbr_cnt, cap);

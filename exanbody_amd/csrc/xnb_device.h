@@ -117,4 +117,23 @@ __host__ __device__ inline size_t nbr_index(uint32_t s, uint32_t k, uint32_t cap
   return (size_t)(s >> 6) * ((size_t)cap << 6) + ((size_t)k << 6) + (s & 63u);
 }
 
+#ifdef __HIPCC__
+// |r - backup|^2 >= thr2, backup decoded relative to the origin of cell c
+// (particle_displ_over.cu:62-65)
+__device__ inline bool displaced(const Geom& g, double4 p, uint32_t c, const uint32_t* backup, uint32_t s, double thr2) {
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  const double dx = p.x - restore_u32_double(backup[3 * (size_t)s + 0], cell_origin(g.ox, g.gl, i, g.cs), g.cs);
+  const double dy = p.y - restore_u32_double(backup[3 * (size_t)s + 1], cell_origin(g.oy, g.gl, j, g.cs), g.cs);
+  const double dz = p.z - restore_u32_double(backup[3 * (size_t)s + 2], cell_origin(g.oz, g.gl, k, g.cs), g.cs);
+  return ((dx * dx + dy * dy) + dz * dz) >= thr2;
+}
+
+// counter += number of lanes of the wave with pred set (every lane of the wave must call)
+__device__ inline void wave_count_add(bool pred, unsigned long long* counter) {
+  const unsigned long long m = __ballot(pred);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(counter, (unsigned long long)__popcll(m));
+}
+#endif
+
 }  // namespace xnb

@@ -19,10 +19,23 @@ struct BrickParams {
   uint32_t T;     // threads per workgroup (>= brick atoms, nrc, nbc; multiple of 64)
   uint32_t Rmax;  // largest region atom count
   uint32_t cap;   // list rows per brick slab
-  uint32_t off_rcnt, off_rpref, off_roff, off_bpref, off_px, off_py, off_pz, off_stage;  // LDS byte offsets
+  uint32_t off_rcnt, off_rpref, off_roff, off_bpref, off_pos;  // LDS byte offsets
 };
 
 // Particle arrays of one storage buffer (slots [0, N) inner, [N, N+G) ghosts).
+// Force-kernel epilogues (see k_lj_brick)
+enum { EPI_NONE = 0, EPI_KICK = 1, EPI_NEXT = 2 };
+
+// Next-step first half fused into the force kernel: positions go to r_next, the count of
+// atoms displaced by >= sqrt(thr2) from their backup is added to *count.
+struct NextStep {
+  double4* r_next;
+  const uint32_t* slot_cell;
+  const uint32_t* backup;
+  double dt, dt2, thr2;
+  unsigned long long* count;
+};
+
 struct PState {
   double4* r;  // x, y, z, type
   double *vx, *vy, *vz;
@@ -82,14 +95,14 @@ void launch_stream_fill(const Geom& g, uint32_t n_atoms, const uint32_t* slot_ce
 void launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t st);
 
 BrickParams brick_params(const Geom& g, int Bi, int Bj, int Bk);
-uint32_t brick_layout(BrickParams& bp, bool with_stage);
+uint32_t brick_layout(BrickParams& bp);
 uint32_t brick_list_granule();
 void launch_brick_stats(const Geom& g, const BrickParams& bp, const uint32_t* cell_n, uint32_t* out, hipStream_t st);
-hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, bool stage, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, double max2,
+hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, double max2,
                             int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt, uint32_t* max_cnt, hipStream_t st);
-hipError_t launch_lj_brick(int precision, bool accum, bool push, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
+hipError_t launch_lj_brick(int precision, bool accum, int epi, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
                            const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, double eps, double sigma, double rcut,
-                           double dth, hipStream_t st);
+                           double dth, const NextStep* next, hipStream_t st);
 hipError_t launch_brick_to_slots(const Geom& g, BrickParams bp, const uint32_t* cell_off, const uint32_t* cell_n, const uint16_t* blist,
                                  const uint32_t* bcnt, uint32_t* nbr, uint32_t* nbr_cnt, uint32_t cap, hipStream_t st);
 void launch_energies(uint32_t N, const PState& A, double* part, uint32_t nparts, double* out, hipStream_t st);

@@ -360,20 +360,6 @@ __global__ void k_backup_r(Geom g, uint32_t N, const double4* __restrict__ r, co
   backup[3 * (size_t)s + 2] = encode_double_u32(p.z, cell_origin(g.oz, g.gl, k, g.cs), g.cs);
 }
 
-__device__ inline bool displaced(const Geom& g, double4 p, uint32_t c, const uint32_t* backup, uint32_t s, double thr2) {
-  int i, j, k;
-  cell_ijk(g, c, i, j, k);
-  const double dx = p.x - restore_u32_double(backup[3 * (size_t)s + 0], cell_origin(g.ox, g.gl, i, g.cs), g.cs);
-  const double dy = p.y - restore_u32_double(backup[3 * (size_t)s + 1], cell_origin(g.oy, g.gl, j, g.cs), g.cs);
-  const double dz = p.z - restore_u32_double(backup[3 * (size_t)s + 2], cell_origin(g.oz, g.gl, k, g.cs), g.cs);
-  return ((dx * dx + dy * dy) + dz * dz) >= thr2;
-}
-
-__device__ inline void wave_count_add(bool pred, unsigned long long* counter) {
-  const unsigned long long m = __ballot(pred);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(counter, (unsigned long long)__popcll(m));
-}
-
 // particle_displ_over (particle_displ_over.cu:38-77)
 __global__ void k_displ_over(Geom g, uint32_t N, const double4* __restrict__ r, const uint32_t* __restrict__ slot_cell,
                              const uint32_t* __restrict__ backup, double thr2, unsigned long long* __restrict__ count) {

@@ -34,11 +34,12 @@ __device__ inline void store_force(uint32_t s, double tx, double ty, double tz, 
   }
 }
 
-// Wave-wide maximum (every lane of the wave must be active).
+// Wave-wide maximum (every lane of the wave must be active), returned as a wave-uniform (scalar) value so loops bounded by it stay
+// scalar branches
 __device__ inline uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
-  return v;
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
 // One pair of the plain FP64 form.  Pairs outside 0 < d2 <= rc^2 contribute through a

@@ -82,11 +82,14 @@ def _with_bricks(monkeypatch, value):
         monkeypatch.setenv("XNB_BRICKS", value)
 
 
-@pytest.mark.parametrize("bricks", ["0", "1,1,1", "2,2,1", "4,2,2", "3,2,2"])
-def test_list_paths_bitwise(monkeypatch, bricks):
+@pytest.mark.parametrize("bricks,fuse", [("0", "1"), ("1,1,1", "1"), ("2,2,1", "1"), ("4,2,2", "1"), ("3,2,2", "1"),
+                                         ("4,4,2", "1"), ("4,4,2", "0")])
+def test_list_paths_bitwise(monkeypatch, bricks, fuse):
     """Generic (per-slot) and brick-tiled lists of several shapes give the oracle's streams
-    and the oracle's FP64 forces bit for bit."""
+    and the oracle's FP64 forces bit for bit; the run matches with and without the next
+    step's first half fused into the force kernel."""
     _with_bricks(monkeypatch, bricks)
+    monkeypatch.setenv("XNB_FUSE_PUSH", fuse)
     r, v, ids, L = fcc_inputs(7, temperature=1.2, jitter=0.1, seed=21)
     o, b, dom = make_pair(r, v, ids, L)
     o.init(1.0, 1.0, RCUT, SKIN)
