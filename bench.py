@@ -162,7 +162,8 @@ def main():
     achieved_tflops = flops_per_atom * n / (fk["avg_ms"] * 1e-3) / 1e12 if fk["avg_ms"] else None
     roofline = dict(bound="mfma", achieved=achieved_tflops, peak=VALU_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=(achieved_tflops / VALU_PEAK_TFLOPS) if achieved_tflops else None, traffic=None,
-                    kernel=f"k_lj_{'mixed' if head_prec == 'mixed' else 'fp64'}<false,true> (LJ force + push_f_v)",
+                    kernel=f"k_lj_brick<{'true' if head_prec == 'mixed' else 'false'}, false, 2, 8> (LJ force + push_f_v + "
+                           "next step's push_f_v_r/push_f_v and displacement count)",
                     algorithmic_flops_per_atom=flops_per_atom, avg_launch_ms=fk["avg_ms"],
                     note="VALU-bound pair kernel (no MFMA): flops = 8/list entry + 24/in-cutoff pair; peak = FP64 "
                          "vector rate, which scalar FP32 shares with packed FP32 disabled")
