@@ -83,6 +83,7 @@ _SIGS = {
     "xnb_pair_counts": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "xnb_synchronize": (C.c_int, [P]),
     "xnb_copy_probe": (C.c_int, [P, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    "xnb_fp64_core": (C.c_int, [P, P, C.c_size_t, P, P]),
     "xnb_deck_create": (C.c_int, [C.POINTER(P)]),
     "xnb_deck_destroy": (C.c_int, [P]),
     "xnb_deck_load": (C.c_int, [P, C.c_char_p]),

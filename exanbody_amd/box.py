@@ -212,5 +212,12 @@ class Box:
         self._check(self.L.xnb_copy_probe(self._h, int(nbytes), int(iters), C.byref(g)), "copy_probe")
         return g.value
 
+    def fp64_core(self, x):
+        """Device sqrt / reciprocal sequences of the FP64 force kernels applied to x."""
+        x = np.ascontiguousarray(x, np.float64)
+        sq, rc = np.empty_like(x), np.empty_like(x)
+        self._check(self.L.xnb_fp64_core(self._h, _ptr(x), x.size, _ptr(sq), _ptr(rc)), "fp64_core")
+        return sq, rc
+
     def synchronize(self):
         self._check(self.L.xnb_synchronize(self._h), "synchronize")

@@ -493,9 +493,9 @@ int ensure_slot_lists(xnb_ctx* c) {
 // With next (brick lists only, no accum), the kernel also advances the next step's first
 // half into buffer B's position array, which the caller then swaps in.
 int launch_force(xnb_ctx* c, int precision, bool accum, bool push, bool ghost, double eps, double sigma, double rcut, double dth,
-                 const NextStep* next = nullptr) {
+                 const NextStep* next = nullptr, bool lean = false) {
   if (c->list_kind == 1 && !ghost) {
-    const int epi = next ? EPI_NEXT : (push ? EPI_KICK : EPI_NONE);
+    const int epi = next ? (lean ? EPI_NEXT_LEAN : EPI_NEXT) : (push ? EPI_KICK : EPI_NONE);
     hipError_t he = launch_lj_brick(precision, accum, epi, c->g, c->bp, c->A, c->cell_off.p, c->cell_n.p, c->blist.p, c->bcnt.p, eps,
                                     sigma, rcut, dth, next, c->st);
     if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick LJ kernel: ") + hipGetErrorString(he));
@@ -1002,7 +1002,9 @@ int xnb_init(xnb_ctx* c, const xnb_lj_params* p, double rcut, double rcut_inc, i
 // compute_force, push_f_v).  When the lists are brick lists, the force kernel of step s < steps
 // also performs step s+1's first half and displacement count, writing positions to buffer B
 // whose position array is then swapped with A's; the arithmetic is the same as the separate
-// push kernel's.  Displacement counts go to step_counts[step - 1].
+// push kernel's.  Forces and per-atom energies are stored on the last step and on steps
+// followed by a rebuild; the state at return equals the unfused loop's bit for bit.
+// Displacement counts go to step_counts[step - 1].
 int xnb_run(xnb_ctx* c, xnb_run_params* p) {
   int e;
   if ((e = check_ctx(c))) return e;
@@ -1045,9 +1047,12 @@ int xnb_run(xnb_ctx* c, xnb_run_params* p) {
       tstop(c);
     }
     const bool fuse = c->fuse_push && step < p->steps && c->list_kind == 1;
-    NextStep nx{c->rB.p, c->slot_cell_A.p, c->backup.p, dt, dt2, thr2, cnt + 1};
+    // forces and energies of this step are read before being recomputed only when the next
+    // step rebuilds (move_particles and the ghost refresh copy them): known in interval mode
+    const bool lean = fuse && interval && ((step + 1) % p->rebuild_every) != 0;
+    NextStep nx{c->rB.p, c->backup.p, dt, dt2, thr2, cnt + 1};
     tstart(c, XNB_TIMER_FORCE);
-    e = launch_force(c, p->precision, false, true, false, p->lj.epsilon, p->lj.sigma, p->rcut, dth, fuse ? &nx : nullptr);
+    e = launch_force(c, p->precision, false, true, false, p->lj.epsilon, p->lj.sigma, p->rcut, dth, fuse ? &nx : nullptr, lean);
     tstop(c);
     if (e) return e;
     if (fuse) {
@@ -1135,6 +1140,26 @@ int xnb_copy_probe(xnb_ctx* c, size_t bytes, int iters, double* gbps) {
   hipFree(b);
   if (he != hipSuccess) return fail(c, XNB_E_HIP, "copy probe failed");
   *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+  return XNB_OK;
+}
+
+int xnb_fp64_core(xnb_ctx* c, const double* x, size_t n, double* sqrt_out, double* recip_out) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (n == 0) return XNB_OK;
+  if (!x || !sqrt_out || !recip_out) return fail(c, XNB_E_ARG, "null argument");
+  double* d = nullptr;
+  HIPCHK(c, hipMalloc((void**)&d, 3 * n * sizeof(double)));
+  hipError_t he = hipMemcpyAsync(d, x, n * sizeof(double), hipMemcpyHostToDevice, c->st);
+  if (he == hipSuccess) {
+    launch_fp64_core(d, n, d + n, d + 2 * n, c->st);
+    he = hipGetLastError();
+  }
+  if (he == hipSuccess) he = hipMemcpyAsync(sqrt_out, d + n, n * sizeof(double), hipMemcpyDeviceToHost, c->st);
+  if (he == hipSuccess) he = hipMemcpyAsync(recip_out, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost, c->st);
+  if (he == hipSuccess) he = hipStreamSynchronize(c->st);
+  hipFree(d);
+  if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("fp64 core: ") + hipGetErrorString(he));
   return XNB_OK;
 }
 

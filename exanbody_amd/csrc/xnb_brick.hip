@@ -74,6 +74,11 @@ __device__ inline BrickLds brick_lds(const BrickParams& bp, char* lds) {
   return L;
 }
 
+// brick of this workgroup (XCD-contiguous order); >= nbricks for the few padding workgroups
+__device__ inline uint32_t brick_of_block(const BrickParams& bp) {
+  return (blockIdx.x % kXcds) * bp.per_xcd + blockIdx.x / kXcds;
+}
+
 struct BrickPos {
   int gi0, gj0, gk0;  // grid coordinates of the brick's first cell
   int ni, nj, nk;     // brick extent clipped to the inner domain
@@ -138,6 +143,11 @@ __device__ inline void brick_setup(const Geom& g, const BrickParams& bp, const B
       L.px[3 * idx] = p.x;
       L.py[3 * idx] = p.y;
       L.pz[3 * idx] = p.z;
+    }
+    if (t == 0) {  // sentinel atom R: every distance to it exceeds any cutoff
+      L.px[3 * R] = 1e30;
+      L.py[3 * R] = 1e30;
+      L.pz[3 * R] = 1e30;
     }
     __syncthreads();
   }
@@ -206,7 +216,8 @@ __global__ void __launch_bounds__(1024) k_nbh_brick_mask(Geom g, BrickParams bp,
                                                          double max2, int half_symmetric, int skip_ghosts, uint16_t* __restrict__ blist,
                                                          uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt) {
   extern __shared__ __align__(16) char lds[];
-  const uint32_t b = blockIdx.x;
+  const uint32_t b = brick_of_block(bp);
+  if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
   brick_setup(g, bp, P, L, r, cell_off, cell_n, true);
@@ -274,15 +285,24 @@ __global__ void __launch_bounds__(1024) k_nbh_brick_mask(Geom g, BrickParams bp,
   bcnt[(size_t)b * T + t] = cnt;
   const uint32_t wmax = wave_max_b(cnt);
   if ((t & 63) == 0 && wmax > 0) atomicMax(max_cnt, wmax);
+  // pad to the wave's batch boundary with the sentinel, so force kernels need no per-lane
+  // bound inside a batch
+  const uint32_t pad_end = min((wmax + kListBatch - 1) / kListBatch * kListBatch, bp.cap);
+  const uint16_t sentinel = (uint16_t)L.rpref[bp.nrc];
+  for (uint32_t q = min(cnt, bp.cap); q < pad_end; q++) out[lpos(q, t, T)] = sentinel;
 }
 
 // ------------------------------------------------------------------ force
 // EPI selects the epilogue after the force store (f = 0 + sum, or f += sum with ACCUM):
-//   EPI_NONE   nothing more;
-//   EPI_KICK   v += f dth, the closing push_f_v of the step (numerical-scheme.msp);
-//   EPI_NEXT   EPI_KICK, then the next step's push_f_v_r / push_f_v
-//              (x += v dt + f dt2; v += f dth) into NextStep::r_next, and the
-//              particle_displ_over count of the new positions against the backup.
+//   EPI_NONE       nothing more;
+//   EPI_KICK       v += f dth, the closing push_f_v of the step (numerical-scheme.msp);
+//   EPI_NEXT       EPI_KICK, then the next step's push_f_v_r / push_f_v
+//                  (x += v dt + f dt2; v += f dth) into NextStep::r_next, and the
+//                  particle_displ_over count of the new positions against the backup;
+//   EPI_NEXT_LEAN  EPI_NEXT without computing or storing f and ep: for a step whose
+//                  forces and energies nothing reads before the next force kernel
+//                  overwrites them (the run loop uses it when the next step does not
+//                  rebuild; the last step of a run always stores them).
 // Positions are read from r (staged in LDS) and, with EPI_NEXT, written to a second buffer,
 // so no workgroup reads a position another one has already advanced.
 template <bool MIXED, bool ACCUM, int EPI, int U>
@@ -291,7 +311,8 @@ __global__ void __launch_bounds__(1024) k_lj_brick(Geom g, BrickParams bp, const
                                                    const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt,
                                                    LJConst lj, PState A, double dth, NextStep nx) {
   extern __shared__ __align__(16) char lds[];
-  const uint32_t b = blockIdx.x;
+  const uint32_t b = brick_of_block(bp);
+  if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
   brick_setup(g, bp, P, L, r, cell_off, cell_n, true);
@@ -305,8 +326,9 @@ __global__ void __launch_bounds__(1024) k_lj_brick(Geom g, BrickParams bp, const
   ra.z = L.pz[3 * At.ia];
   ra.w = 0.;
   // U entries per batch = U / 4 eight-byte loads per lane; the next batch's loads are issued
-  // before the current batch is computed.  Batches never pass cap (a multiple of U).
-  static_assert(U % 4 == 0, "batches are whole groups of four entries");
+  // before the current batch is computed.  Batches never pass cap (a multiple of U), and the
+  // build padded every list of the wave with the sentinel up to a multiple of U.
+  static_assert(U % 4 == 0 && U == kListBatch, "batches are whole groups of four entries, as padded");
   constexpr int W = U / 4;
   const uint2* lst = reinterpret_cast<const uint2*>(blist + (size_t)b * bp.cap * T) + t;
   double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
@@ -332,9 +354,7 @@ __global__ void __launch_bounds__(1024) k_lj_brick(Geom g, BrickParams bp, const
       j[4 * w + 2] = wc[w].y & 0xffffu;
       j[4 * w + 3] = wc[w].y >> 16;
     }
-#pragma unroll
-    for (int u = 0; u < U; u++) j[u] = (q0 + u < n) ? j[u] : At.ia;  // own index: rejected by d2 > 0
-    double4 rb[U];
+    double4 rb[U];  // entries past a lane's count are the sentinel (rejected by the cutoff)
 #pragma unroll
     for (int u = 0; u < U; u++) {
       rb[u].x = L.px[3 * j[u]];
@@ -344,21 +364,26 @@ __global__ void __launch_bounds__(1024) k_lj_brick(Geom g, BrickParams bp, const
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      if (MIXED) lj_pair_mixed(lj, ra, rb[u], tx, ty, tz, te);
-      else lj_pair_fp64(lj, ra, rb[u], tx, ty, tz, te);
+      if (MIXED) lj_pair_mixed<EPI != EPI_NEXT_LEAN>(lj, ra, rb[u], tx, ty, tz, te);
+      else lj_pair_fp64<EPI != EPI_NEXT_LEAN>(lj, ra, rb[u], tx, ty, tz, te);
     }
   }
+  constexpr bool NEXT = EPI == EPI_NEXT || EPI == EPI_NEXT_LEAN;
   bool over = false;
   if (At.live) {
     const uint32_t s = At.slot;
-    store_force<ACCUM>(s, tx, ty, tz, te, A.fx, A.fy, A.fz, A.ep);
+    if (EPI != EPI_NEXT_LEAN) store_force<ACCUM>(s, tx, ty, tz, te, A.fx, A.fy, A.fz, A.ep);
     if (EPI != EPI_NONE) {
-      const double ax = A.fx[s], ay = A.fy[s], az = A.fz[s];
+      // the stored force (0 + sum with ACCUM false, which EPI_NEXT* requires)
+      const double ax = EPI == EPI_NEXT_LEAN ? 0.0 + tx : A.fx[s];
+      const double ay = EPI == EPI_NEXT_LEAN ? 0.0 + ty : A.fy[s];
+      const double az = EPI == EPI_NEXT_LEAN ? 0.0 + tz : A.fz[s];
       double ux = A.vx[s] + ax * dth;
       double uy = A.vy[s] + ay * dth;
       double uz = A.vz[s] + az * dth;
-      if (EPI == EPI_NEXT) {
-        double4 p = r[s];
+      if (NEXT) {
+        double4 p = ra;  // the LDS copy of r[s].xyz
+        p.w = reinterpret_cast<const double*>(r)[4 * (size_t)s + 3];
         p.x += ux * nx.dt + ax * nx.dt2;
         p.y += uy * nx.dt + ay * nx.dt2;
         p.z += uz * nx.dt + az * nx.dt2;
@@ -366,14 +391,16 @@ __global__ void __launch_bounds__(1024) k_lj_brick(Geom g, BrickParams bp, const
         ux += ax * dth;
         uy += ay * dth;
         uz += az * dth;
-        over = displaced(g, p, nx.slot_cell[s], nx.backup, s, nx.thr2);
+        const int ri = (int)(At.rc % (uint32_t)bp.RX), rj = (int)((At.rc / (uint32_t)bp.RX) % (uint32_t)bp.RY),
+                  rk = (int)(At.rc / ((uint32_t)bp.RX * (uint32_t)bp.RY));
+        over = displaced_ijk(g, p, P.gi0 - 1 + ri, P.gj0 - 1 + rj, P.gk0 - 1 + rk, nx.backup, s, nx.thr2);
       }
       A.vx[s] = ux;
       A.vy[s] = uy;
       A.vz[s] = uz;
     }
   }
-  if (EPI == EPI_NEXT) wave_count_add(over, nx.count);
+  if (NEXT) wave_count_add(over, nx.count);
 }
 
 // ------------------------------------------------------------------ conversion
@@ -384,7 +411,8 @@ __global__ void __launch_bounds__(1024) k_brick_to_slots(Geom g, BrickParams bp,
                                                          const uint32_t* __restrict__ bcnt, uint32_t* __restrict__ nbr,
                                                          uint32_t* __restrict__ nbr_cnt, uint32_t cap) {
   extern __shared__ __align__(16) char lds[];
-  const uint32_t b = blockIdx.x;
+  const uint32_t b = brick_of_block(bp);
+  if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
   brick_setup(g, bp, P, L, nullptr, cell_off, cell_n, false);
@@ -412,6 +440,7 @@ BrickParams brick_params(const Geom& g, int Bi, int Bj, int Bk) {
   bp.nby = (g.Dj + Bj - 1) / Bj;
   bp.nbz = (g.Dk + Bk - 1) / Bk;
   bp.nbricks = (uint32_t)bp.nbx * bp.nby * bp.nbz;
+  bp.per_xcd = (bp.nbricks + kXcds - 1) / kXcds;
   bp.RX = Bi + 2;
   bp.RY = Bj + 2;
   bp.RZ = Bk + 2;
@@ -434,7 +463,7 @@ uint32_t brick_layout(BrickParams& bp) {
   bp.off_bpref = o;
   o = align8(o + 4u * (bp.nbc + 1));
   bp.off_pos = o;
-  o += 24u * bp.Rmax;
+  o += 24u * (bp.Rmax + 1);  // region atoms + the sentinel
   return align8(o);
 }
 
@@ -454,12 +483,12 @@ hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, con
   if (half_symmetric || skip_ghosts) {
     e = set_lds((const void*)k_nbh_brick_mask<true>, lds);
     if (e != hipSuccess) return e;
-    k_nbh_brick_mask<true><<<bp.nbricks, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist, bcnt,
+    k_nbh_brick_mask<true><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist, bcnt,
                                                           max_cnt);
   } else {
     e = set_lds((const void*)k_nbh_brick_mask<false>, lds);
     if (e != hipSuccess) return e;
-    k_nbh_brick_mask<false><<<bp.nbricks, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, 0, 0, blist, bcnt, max_cnt);
+    k_nbh_brick_mask<false><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, 0, 0, blist, bcnt, max_cnt);
   }
   return hipGetLastError();
 }
@@ -472,7 +501,7 @@ static hipError_t ljb_launch(const Geom& g, const BrickParams& bp, uint32_t lds,
                              double dth, const NextStep& nx, hipStream_t st) {
   hipError_t e = set_lds((const void*)k_lj_brick<MIXED, ACCUM, EPI, LJB_UNROLL>, lds);
   if (e != hipSuccess) return e;
-  k_lj_brick<MIXED, ACCUM, EPI, LJB_UNROLL><<<bp.nbricks, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, blist, bcnt, lj, A, dth, nx);
+  k_lj_brick<MIXED, ACCUM, EPI, LJB_UNROLL><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, blist, bcnt, lj, A, dth, nx);
   return hipGetLastError();
 }
 
@@ -484,14 +513,17 @@ hipError_t launch_lj_brick(int precision, bool accum, int epi, const Geom& g, Br
   const LJConst lj = make_lj_const(eps, sigma, rcut);
   const uint32_t lds = brick_layout(bp);
   const NextStep nx = next ? *next : NextStep{};
-  if (epi == EPI_NEXT && (accum || !next)) return hipErrorInvalidValue;
+  const bool next_epi = epi == EPI_NEXT || epi == EPI_NEXT_LEAN;
+  if (next_epi && (accum || !next)) return hipErrorInvalidValue;
 #define XNB_LJB_ARGS g, bp, lds, A.r, cell_off, cell_n, blist, bcnt, lj, A, dth, nx, st
   if (precision) {
     if (epi == EPI_NEXT) return ljb_launch<true, false, EPI_NEXT>(XNB_LJB_ARGS);
+    if (epi == EPI_NEXT_LEAN) return ljb_launch<true, false, EPI_NEXT_LEAN>(XNB_LJB_ARGS);
     if (accum) return epi ? ljb_launch<true, true, EPI_KICK>(XNB_LJB_ARGS) : ljb_launch<true, true, EPI_NONE>(XNB_LJB_ARGS);
     return epi ? ljb_launch<true, false, EPI_KICK>(XNB_LJB_ARGS) : ljb_launch<true, false, EPI_NONE>(XNB_LJB_ARGS);
   }
   if (epi == EPI_NEXT) return ljb_launch<false, false, EPI_NEXT>(XNB_LJB_ARGS);
+  if (epi == EPI_NEXT_LEAN) return ljb_launch<false, false, EPI_NEXT_LEAN>(XNB_LJB_ARGS);
   if (accum) return epi ? ljb_launch<false, true, EPI_KICK>(XNB_LJB_ARGS) : ljb_launch<false, true, EPI_NONE>(XNB_LJB_ARGS);
   return epi ? ljb_launch<false, false, EPI_KICK>(XNB_LJB_ARGS) : ljb_launch<false, false, EPI_NONE>(XNB_LJB_ARGS);
 #undef XNB_LJB_ARGS
@@ -502,7 +534,7 @@ hipError_t launch_brick_to_slots(const Geom& g, BrickParams bp, const uint32_t* 
   const uint32_t lds = brick_layout(bp);
   hipError_t e = set_lds((const void*)k_brick_to_slots, lds);
   if (e != hipSuccess) return e;
-  k_brick_to_slots<<<bp.nbricks, bp.T, lds, st>>>(g, bp, cell_off, cell_n, blist, bcnt, nbr, nbr_cnt, cap);
+  k_brick_to_slots<<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, cell_off, cell_n, blist, bcnt, nbr, nbr_cnt, cap);
   return hipGetLastError();
 }
 

@@ -118,15 +118,19 @@ __host__ __device__ inline size_t nbr_index(uint32_t s, uint32_t k, uint32_t cap
 }
 
 #ifdef __HIPCC__
-// |r - backup|^2 >= thr2, backup decoded relative to the origin of cell c
-// (particle_displ_over.cu:62-65)
-__device__ inline bool displaced(const Geom& g, double4 p, uint32_t c, const uint32_t* backup, uint32_t s, double thr2) {
-  int i, j, k;
-  cell_ijk(g, c, i, j, k);
+// |r - backup|^2 >= thr2, backup decoded relative to the origin of the cell at grid
+// coordinates (i, j, k) (particle_displ_over.cu:62-65)
+__device__ inline bool displaced_ijk(const Geom& g, double4 p, int i, int j, int k, const uint32_t* backup, uint32_t s, double thr2) {
   const double dx = p.x - restore_u32_double(backup[3 * (size_t)s + 0], cell_origin(g.ox, g.gl, i, g.cs), g.cs);
   const double dy = p.y - restore_u32_double(backup[3 * (size_t)s + 1], cell_origin(g.oy, g.gl, j, g.cs), g.cs);
   const double dz = p.z - restore_u32_double(backup[3 * (size_t)s + 2], cell_origin(g.oz, g.gl, k, g.cs), g.cs);
   return ((dx * dx + dy * dy) + dz * dz) >= thr2;
+}
+
+__device__ inline bool displaced(const Geom& g, double4 p, uint32_t c, const uint32_t* backup, uint32_t s, double thr2) {
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  return displaced_ijk(g, p, i, j, k, backup, s, thr2);
 }
 
 // counter += number of lanes of the wave with pred set (every lane of the wave must call)

@@ -10,12 +10,21 @@
 namespace xnb {
 
 // Brick tiling of the inner domain (see xnb_brick.hip) and the LDS layout of one workgroup.
+// Workgroups are dispatched to the 8 XCDs of the chip round-robin; brick kernels give each
+// XCD a contiguous run of bricks so that neighbouring bricks, which share halo cells, run
+// under the same L2.
+constexpr uint32_t kXcds = 8;
+// Brick lists are padded per wave to a multiple of this many entries with the region's
+// sentinel atom (far away, so every pair with it fails the cutoff test).
+constexpr uint32_t kListBatch = 8;
+
 struct BrickParams {
   int Bi, Bj, Bk;     // brick size in cells
   int nbx, nby, nbz;  // bricks per axis
   int RX, RY, RZ;     // region (brick + 1 cell each side)
   int nrc, nbc;       // region cells, brick cells
   uint32_t nbricks;
+  uint32_t per_xcd;  // bricks per XCD: workgroup i runs brick (i % 8) * per_xcd + i / 8
   uint32_t T;     // threads per workgroup (>= brick atoms, nrc, nbc; multiple of 64)
   uint32_t Rmax;  // largest region atom count
   uint32_t cap;   // list rows per brick slab
@@ -24,13 +33,12 @@ struct BrickParams {
 
 // Particle arrays of one storage buffer (slots [0, N) inner, [N, N+G) ghosts).
 // Force-kernel epilogues (see k_lj_brick)
-enum { EPI_NONE = 0, EPI_KICK = 1, EPI_NEXT = 2 };
+enum { EPI_NONE = 0, EPI_KICK = 1, EPI_NEXT = 2, EPI_NEXT_LEAN = 3 };
 
 // Next-step first half fused into the force kernel: positions go to r_next, the count of
 // atoms displaced by >= sqrt(thr2) from their backup is added to *count.
 struct NextStep {
   double4* r_next;
-  const uint32_t* slot_cell;
   const uint32_t* backup;
   double dt, dt2, thr2;
   unsigned long long* count;
@@ -93,6 +101,7 @@ void launch_stream_fill(const Geom& g, uint32_t n_atoms, const uint32_t* slot_ce
                         const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap, const uint32_t* pstart, const uint32_t* plen,
                         const uint32_t* coff, int ntables, int random_access, uint16_t* out, hipStream_t st);
 void launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t st);
+void launch_fp64_core(const double* x, size_t n, double* sq, double* rc, hipStream_t st);
 
 BrickParams brick_params(const Geom& g, int Bi, int Bj, int Bk);
 uint32_t brick_layout(BrickParams& bp);

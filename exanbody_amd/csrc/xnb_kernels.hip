@@ -805,6 +805,19 @@ __global__ void __launch_bounds__(256) k_copy_probe(const uint4* __restrict__ sr
   for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) dst[q] = src[q];
 }
 
+// device evaluation of the FP64 sqrt / reciprocal sequences of the LJ kernels
+__global__ void k_fp64_core(const double* __restrict__ x, size_t n, double* __restrict__ sq, double* __restrict__ rc) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  sq[i] = sqrt_normal(x[i]);
+  rc[i] = recip_normal(x[i]);
+}
+
+void launch_fp64_core(const double* x, size_t n, double* sq, double* rc, hipStream_t st) {
+  if (n == 0) return;
+  k_fp64_core<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(x, n, sq, rc);
+}
+
 void launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t st) {
   const size_t n = bytes / 16;
   k_copy_probe<<<4096, 256, 0, st>>>((const uint4*)src, (uint4*)dst, n);

@@ -42,10 +42,52 @@ __device__ inline uint32_t wave_max_u32(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
+// Correctly rounded FP64 square root and reciprocal for normal-range arguments.
+//
+// sqrt_normal: Goldschmidt iteration from the hardware reciprocal square root estimate
+// (v_rsq_f64), then two fma-corrected Newton steps on the root (P. Markstein, "IA-64 and
+// Elementary Functions", 2000, ch. 9).  recip_normal: two Newton-Raphson steps on the
+// hardware reciprocal estimate (v_rcp_f64) and one fma-corrected quotient step.  For
+// sqrt_normal(x), x in [2^-767, 2^1000], and recip_normal(y), y in [2^-383, 2^383], these
+// are the sequences hipcc emits for sqrt(x) and 1.0 / y without the range scaling and the
+// special-value fix-ups, which are identities there; the results are the IEEE correctly
+// rounded ones (checked against the host's IEEE sqrt and division by the GPU tests).
+__device__ inline double sqrt_normal(double x) {
+  const double g = __builtin_amdgcn_rsq(x);
+  double s = x * g;
+  double h = g * 0.5;
+  const double r = fma(-h, s, 0.5);
+  s = fma(s, r, s);
+  h = fma(h, r, h);
+  double d = fma(-s, s, x);
+  s = fma(d, h, s);
+  d = fma(-s, s, x);
+  return fma(d, h, s);
+}
+
+__device__ inline double recip_normal(double y) {
+  double a = __builtin_amdgcn_rcp(y);
+  double e = fma(-y, a, 1.0);
+  a = fma(a, e, a);
+  e = fma(-y, a, 1.0);
+  a = fma(a, e, a);
+  const double rem = fma(-y, a, 1.0);
+  return fma(rem, a, a);
+}
+
+// 1 / sqrt(x) as two correctly rounded steps, r = sqrt(x) then 1.0 / r; the normal-range
+// sequences unless some lane of the wave has x below their range (never for physical pair
+// distances), in which case the whole wave takes the library operations.
+__device__ inline double inv_sqrt_rounded(double x) {
+  if (__builtin_expect(__any(x < 0x1p-767), 0)) return 1.0 / sqrt(x);
+  return recip_normal(sqrt_normal(x));
+}
+
 // One pair of the plain FP64 form.  Pairs outside 0 < d2 <= rc^2 contribute through a
 // select of 0.0: adding +-0 leaves the running sums bitwise unchanged (they start at +0.0 and
 // round-to-nearest never produces -0 from a sum with a +0 operand), so the result equals the
 // reference's branch.
+template <bool ENERGY = true>
 __device__ inline void lj_pair_fp64(const LJConst& lj, const double4& ra, const double4& rb, double& tx, double& ty, double& tz,
                                     double& te) {
   const double dx = rb.x - ra.x;
@@ -53,8 +95,7 @@ __device__ inline void lj_pair_fp64(const LJConst& lj, const double4& ra, const 
   const double dz = rb.z - ra.z;
   const double d2 = (dx * dx + dy * dy) + dz * dz;
   const bool in = d2 > 0.0 && d2 <= lj.rc2;
-  const double rr = sqrt(in ? d2 : 1.0);
-  const double inv_r = 1.0 / rr;
+  const double inv_r = inv_sqrt_rounded(in ? d2 : 1.0);  // 1.0 / sqrt(d2), each correctly rounded
   const double ratio = lj.sigma * inv_r;
   const double ratio2 = ratio * ratio;
   const double ratio6 = ratio2 * ratio2 * ratio2;
@@ -66,12 +107,13 @@ __device__ inline void lj_pair_fp64(const LJConst& lj, const double4& ra, const 
   tx += de * dx;
   ty += de * dy;
   tz += de * dz;
-  te += in ? e : 0.0;
+  if (ENERGY) te += in ? e : 0.0;
 }
 
 // One pair of the mixed form: FP64 displacement and cutoff test, FP32 coefficient
 // de/r = -24 eps (2 s^12 - s^6) / r^2 from (float)d2 with a hardware reciprocal, FP64
 // accumulation f = fma(c, d, f).
+template <bool ENERGY = true>
 __device__ inline void lj_pair_mixed(const LJConst& lj, const double4& ra, const double4& rb, double& tx, double& ty, double& tz,
                                      double& te) {
   const double dx = rb.x - ra.x;
@@ -79,16 +121,16 @@ __device__ inline void lj_pair_mixed(const LJConst& lj, const double4& ra, const
   const double dz = rb.z - ra.z;
   const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
   const bool in = d2 > 0.0 && d2 <= lj.rc2;
-  const float inv_r2 = __builtin_amdgcn_rcpf(in ? (float)d2 : 1.0f);
+  // outside the cutoff 1/r^2 = 0 makes the coefficient and the energy +-0
+  const float inv_r2 = in ? __builtin_amdgcn_rcpf((float)d2) : 0.0f;
   const float s2 = lj.sigma2f * inv_r2;
   const float s6 = s2 * s2 * s2;
   const float s12 = s6 * s6;
-  const float c = in ? lj.m24epsf * (2.0f * s12 - s6) * inv_r2 : 0.0f;
-  const double cd = (double)c;
+  const double cd = (double)(lj.m24epsf * (2.0f * s12 - s6) * inv_r2);
   tx = fma(cd, dx, tx);
   ty = fma(cd, dy, ty);
   tz = fma(cd, dz, tz);
-  te += (double)(in ? lj.eps4f * (s12 - s6) : 0.0f);
+  if (ENERGY) te += (double)(lj.eps4f * (s12 - s6));
 }
 
 inline LJConst make_lj_const(double eps, double sigma, double rcut) {

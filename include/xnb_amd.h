@@ -179,6 +179,11 @@ int xnb_synchronize(xnb_ctx* ctx);
 /* Device copy of `bytes` (read + write), iters times, timed with HIP events: achieved GB/s
  * counting read and write bytes.  Also the known-size kernel that calibrates HBM counters. */
 int xnb_copy_probe(xnb_ctx* ctx, size_t bytes, int iters, double* gbps);
+/* Diagnostic with no counterpart in the reference: evaluates on the device the FP64 square
+ * root and reciprocal sequences the plain-form LJ kernels use for r = sqrt(d2) and 1.0 / r,
+ * so tests can check them against IEEE results.  sqrt_out[i] is defined for x[i] in
+ * [2^-767, 2^1000], recip_out[i] for x[i] in [2^-383, 2^383]. */
+int xnb_fp64_core(xnb_ctx* ctx, const double* x, size_t n, double* sqrt_out, double* recip_out);
 
 /* ---- operator decks ---------------------------------------------------------- */
 /* YAML-subset decks with the reference's operator names and slots (the .msp structure of

@@ -167,3 +167,32 @@ def test_stream_fixture_gpu():
         for c, want in case["streams"].items():
             assert gs[int(c)].tolist() == want, (name, c)
         b.close()
+
+
+def test_fp64_sqrt_recip_sequences_are_ieee():
+    """The FP64 force kernels compute r = sqrt(d2) and 1.0 / r with their own normal-range
+    sequences; they must equal the IEEE correctly rounded results (numpy's sqrt and division)
+    bit for bit over the ranges they are used on, including near-midpoint cases."""
+    from exanbody_amd import Box, cubic_domain
+    dom = cubic_domain(10.0, 2.8)
+    b = Box(dom["bounds_min"], dom["bounds_max"], dom["grid_dims"], dom["cell_size"])
+    rng = np.random.default_rng(7)
+    parts = [
+        rng.uniform(1e-6, 6.25, 2_000_000),                              # pair d2 range (rc = 2.5)
+        np.exp2(rng.uniform(-767.0, 1000.0, 500_000)),                   # whole sqrt range
+        np.array([1.0, 2.0, 4.0, 6.25, 0.25, 2.0 ** -767, 2.0 ** 1000]),
+    ]
+    # near-midpoint squares: y^2 rounded, and its neighbours
+    y = rng.uniform(0.5, 2.5, 200_000)
+    sq = y * y
+    parts += [sq, np.nextafter(sq, 0), np.nextafter(sq, 10)]
+    x = np.concatenate(parts)
+    s, _ = b.fp64_core(x)
+    np.testing.assert_array_equal(s.view(np.uint64), np.sqrt(x).view(np.uint64))
+    # reciprocal over its range and on the roots the kernel feeds it
+    y = np.concatenate([rng.uniform(1e-3, 2.5, 2_000_000), np.exp2(rng.uniform(-383.0, 383.0, 500_000)),
+                        np.sqrt(rng.uniform(1e-6, 6.25, 1_000_000)), 1.0 + np.arange(1, 100_000) * 2.0 ** -52,
+                        2.0 - np.arange(1, 100_000) * 2.0 ** -52])
+    _, rc = b.fp64_core(y)
+    np.testing.assert_array_equal(rc.view(np.uint64), (1.0 / y).view(np.uint64))
+    b.close()
