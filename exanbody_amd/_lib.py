@@ -84,6 +84,13 @@ _SIGS = {
     "xnb_synchronize": (C.c_int, [P]),
     "xnb_copy_probe": (C.c_int, [P, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "xnb_fp64_core": (C.c_int, [P, P, C.c_size_t, P, P]),
+    "xnb_batch_create": (C.c_int, [C.POINTER(P), C.c_int]),
+    "xnb_batch_destroy": (C.c_int, [P]),
+    "xnb_batch_error": (C.c_char_p, [P]),
+    "xnb_batch_attach": (C.c_int, [P, P, C.POINTER(RunParams), C.POINTER(C.c_int)]),
+    "xnb_batch_step": (C.c_int, [P, C.c_int, C.POINTER(C.c_int)]),
+    "xnb_batch_box_status": (C.c_int, [P, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "xnb_batch_detach": (C.c_int, [P, C.c_int, C.POINTER(RunParams)]),
     "xnb_deck_create": (C.c_int, [C.POINTER(P)]),
     "xnb_deck_destroy": (C.c_int, [P]),
     "xnb_deck_load": (C.c_int, [P, C.c_char_p]),

@@ -12,88 +12,11 @@
 #include <vector>
 
 #include "../../include/xnb_amd.h"
-#include "xnb_kernels.h"
+#include "xnb_ctx.h"
 
 using namespace xnb;
 
-namespace {
-
-template <class T>
-struct DArr {
-  T* p = nullptr;
-  size_t n = 0;
-};
-
-struct TimerRec {
-  int id;
-  hipEvent_t a, b;
-};
-
-}  // namespace
-
-struct xnb_ctx {
-  int device = 0;
-  hipStream_t st = nullptr;
-  Geom g{};
-  xnb_domain dom{};
-  bool nbh_set = false;
-  double nbh_dist_lab = 0., max_displ = 0., ghost_dist = 0.;
-
-  uint32_t N = 0, G = 0;
-  size_t cap = 0;  // particle slots allocated in each buffer
-  PState A{}, B{};
-  DArr<uint32_t> slot_cell_A, slot_cell_B;
-  DArr<double4> rA, rB;
-  DArr<double> vA[3], vB[3], fA[3], fB[3], eA, eB;
-  DArr<uint64_t> idA, idB;
-
-  // per cell
-  DArr<uint32_t> cell_off, cell_n, cnt_out, cnt_in, new_n, new_off, in_cur, gcnt, goff, clen, coff;
-  // per slot scratch
-  DArr<uint32_t> mv_dst, pk, newloc, in_list, plen, pstart;
-  DArr<uint32_t> ghost_src, ghost_shift, backup;
-  // neighbour lists
-  DArr<uint32_t> nbr, nbr_cnt;
-  uint32_t nbr_cap = 96;
-  uint32_t list_atoms = 0;
-  uint32_t max_neighbors = 0;
-  xnb_chunk_neighbors_config list_cfg{1, 0, 0, 0, 0};
-  bool lists_valid = false, ghosts_valid = false, backup_valid = false;
-  DArr<uint16_t> stream_buf;
-
-  // scan / scalar scratch
-  DArr<uint32_t> bsum, u32s;            // u32s: [0] err, [1] scan total, [2] max count
-  DArr<unsigned long long> u64s;        // [0] displacement count, [1..2] pair counts
-  DArr<unsigned long long> step_counts; // per-step displacement counts of xnb_run
-  DArr<double> red_part, red_out;
-  uint32_t* h_u32 = nullptr;            // pinned
-  unsigned long long* h_u64 = nullptr;  // pinned
-
-  // timers
-  bool timing = false;
-  std::vector<TimerRec> pending;
-  std::vector<hipEvent_t> pool;
-  double t_ms[XNB_TIMER_COUNT] = {};
-  int64_t t_n[XNB_TIMER_COUNT] = {};
-  int t_cur = -1;
-  hipEvent_t t_a = nullptr;
-
-  // brick-tiled production lists (xnb_brick.hip)
-  int list_kind = 0;  // 0: generic per-slot u32 lists in nbr; 1: brick u16 lists in blist
-  bool use_bricks = true;
-  BrickParams bp{};
-  int brick_dims[3] = {0, 0, 0};  // 0: choose automatically
-  DArr<uint16_t> blist;
-  DArr<uint32_t> bcnt;
-  bool slot_lists_current = false;  // nbr/nbr_cnt mirror the brick lists
-  bool fuse_push = true;            // force kernel performs the next step's first half (brick lists)
-  uint32_t lds_limit = 65536;
-
-  double last_rcut = 0.;
-  std::string err;
-};
-
-namespace {
+namespace xnb_impl {
 
 thread_local std::string g_err;
 
@@ -101,36 +24,6 @@ int fail(xnb_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   g_err = msg;
   return code;
-}
-
-#define HIPCHK(ctx, expr)                                                                            \
-  do {                                                                                               \
-    hipError_t _e = (expr);                                                                          \
-    if (_e != hipSuccess)                                                                            \
-      return fail((ctx), XNB_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));              \
-  } while (0)
-
-template <class T>
-int dalloc(xnb_ctx* c, DArr<T>& a, size_t n) {
-  if (a.n >= n && a.p) return XNB_OK;
-  if (a.p) hipFree(a.p);
-  a.p = nullptr;
-  a.n = 0;
-  size_t nn = n ? n : 1;
-  hipError_t e = hipMalloc((void**)&a.p, nn * sizeof(T));
-  if (e != hipSuccess) {
-    a.p = nullptr;
-    return fail(c, XNB_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
-  }
-  a.n = nn;
-  return XNB_OK;
-}
-
-template <class T>
-void dfree(DArr<T>& a) {
-  if (a.p) hipFree(a.p);
-  a.p = nullptr;
-  a.n = 0;
 }
 
 inline uint32_t nb256(size_t n) { return (uint32_t)((n + 255) / 256); }
@@ -342,7 +235,8 @@ int op_ghost_all(xnb_ctx* c) {
   if ((e = read_u32(c, c->u32s.p + 1, &G))) return e;
   if ((e = ensure_capacity(c, (size_t)c->N + G, c->N))) return e;
   tstart(c, XNB_TIMER_GHOST);
-  launch_ghost_fill(g, c->N, c->A, c->goff.p, c->cell_off.p, c->cell_n.p, c->slot_cell_A.p, c->ghost_src.p, c->ghost_shift.p, c->st);
+  launch_ghost_fill(g, c->N, c->A, c->goff.p, c->cell_off.p, c->cell_n.p, c->slot_cell_A.p, c->ghost_src.p, c->ghost_shift.p,
+                    (uint32_t)std::min<size_t>(c->cap, 0xffffffffu), c->u32s.p + 7, c->st);
   tstop(c);
   HIPCHK(c, hipGetLastError());
   c->G = G;
@@ -389,13 +283,16 @@ int choose_bricks(xnb_ctx* c, uint32_t cap, BrickParams& out) {
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(c->h_u32, c->u32s.p + 3, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    const uint32_t nb_max = c->h_u32[0], r_max = c->h_u32[1];
+    // head room for later rebuilds of a batched box (brick_margin > 1): atoms may regroup
+    const uint32_t nb_max = (uint32_t)std::ceil(c->h_u32[0] * c->brick_margin);
+    const uint32_t r_max = (uint32_t)std::ceil(c->h_u32[1] * c->brick_margin);
     uint32_t T = std::max<uint32_t>({nb_max, (uint32_t)bp.nrc, (uint32_t)bp.nbc, 64u});
     T = (T + 63) / 64 * 64;
     if (T > 1024 || r_max >= 65535u) continue;
     bp.T = T;
     bp.Rmax = std::max<uint32_t>(r_max, 1u);
     bp.cap = cap;
+    bp.status = c->u32s.p + 6;
     if (brick_layout(bp) > c->lds_limit) continue;
     out = bp;
     return XNB_OK;
@@ -414,13 +311,16 @@ int op_nbh_brick(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& 
     if ((e = dalloc(c, c->blist, (size_t)bp.nbricks * bp.cap * bp.T))) return e;
     if ((e = dalloc(c, c->bcnt, (size_t)bp.nbricks * bp.T))) return e;
     HIPCHK(c, hipMemsetAsync(c->u32s.p + 2, 0, sizeof(uint32_t), c->st));
+    HIPCHK(c, hipMemsetAsync(c->u32s.p + 6, 0, sizeof(uint32_t), c->st));
     tstart(c, XNB_TIMER_NBH);
     hipError_t he = launch_nbh_brick(c->g, bp, c->A.r, c->cell_off.p, c->cell_n.p, max2, cfg.half_symmetric, cfg.skip_ghosts,
                                      c->blist.p, c->bcnt.p, c->u32s.p + 2, c->st);
     tstop(c);
     if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick neighbour build: ") + hipGetErrorString(he));
-    uint32_t mx = 0;
+    uint32_t mx = 0, bad = 0;
     if ((e = read_u32(c, c->u32s.p + 2, &mx))) return e;
+    if ((e = read_u32(c, c->u32s.p + 6, &bad))) return e;
+    if (bad) return fail(c, XNB_E_STATE, "brick sizes changed between statistics and build");
     c->max_neighbors = mx;
     if (mx <= bp.cap) {
       c->bp = bp;
@@ -525,7 +425,9 @@ int check_ctx(xnb_ctx* c) {
   return XNB_OK;
 }
 
-}  // namespace
+}  // namespace xnb_impl
+
+using namespace xnb_impl;
 
 // =========================================================================== C ABI
 

@@ -1,0 +1,518 @@
+// xnb_batch.hip -- many independent boxes advanced together: every kernel of a step (first
+// half, rebuild, ghost refresh, fused force) is one launch over all boxes of the batch.
+//
+// A box is an initialised xnb_ctx (xnb_create, xnb_set_particles, xnb_nbh_dist, xnb_init)
+// attached with its run parameters; it keeps its own device arrays, which the batch addresses
+// through a device table of BoxFrame entries.  Each box follows exactly the operator sequence
+// of xnb_run (same kernels' per-atom bodies, same order of operations per atom), so a box
+// advanced in a batch ends bitwise equal to the same box advanced alone.  Boxes may be
+// attached and detached between steps (continuous batching); their step counters are their
+// own.
+//
+// Host synchronisation: none on a step without rebuilds; one read-back of four status words
+// per box on a step where some box rebuilds (ghost counts for the next work tables, overflow
+// flags).  A box whose rebuild overflowed the sizes its frame was laid out for (ghost slots,
+// brick workgroup or region, list capacity) is redone with the one-box operators, which
+// re-size its arrays.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "xnb_ctx.h"
+
+using namespace xnb;
+using namespace xnb_impl;
+
+namespace {
+
+struct BatchBox {
+  xnb_ctx* c = nullptr;
+  bool used = false, finished = false, failed = false;
+  BoxFrame f{};  // host mirror of the device frame
+  int64_t steps = 0, done = 0;
+  int every = 10;
+  bool fresh = true;      // the next step's first half still has to be launched
+  uint32_t last_par = 0;  // parity of the box's last step
+  int64_t n_rebuilds = 0, n_violations = 0;
+  std::vector<char> rebuilt;
+  DArr<unsigned long long> counts;
+  xnb_run_params p{};
+  std::string err;
+};
+
+constexpr double kBrickMargin = 1.25;  // brick T / Rmax head room of attached boxes
+constexpr double kSlotMargin = 1.3;    // slot capacity head room for ghosts
+
+}  // namespace
+
+struct xnb_batch {
+  int device = 0;
+  hipStream_t st = nullptr;
+  std::vector<BatchBox> box;
+  DArr<BoxFrame> dF;
+  BoxFrame* hF = nullptr;  // pinned staging of frame uploads, one entry per slot
+  DArr<uint32_t> dwords;   // 4 words per slot
+  uint32_t* hwords = nullptr;
+  DArr<uint32_t> dwork;
+  uint32_t* hwork[2] = {nullptr, nullptr};
+  size_t hwork_cap = 0;
+  hipEvent_t hev[2] = {nullptr, nullptr};
+  int hcur = 0;
+  uint32_t par = 0;
+  uint32_t gstep = 1;  // global step number of the next step
+  uint32_t lds_limit = 65536;
+  std::string err;
+};
+
+namespace {
+
+int bfail(xnb_batch* B, int code, const std::string& msg) {
+  if (B) B->err = msg;
+  return code;
+}
+
+#define BHIP(B, expr)                                                                      \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess) return bfail((B), XNB_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+// ---- frames <-> contexts
+
+void frame_from_ctx(xnb_batch* B, int i, const xnb_run_params& p) {
+  BatchBox& bb = B->box[i];
+  xnb_ctx* c = bb.c;
+  BoxFrame f{};
+  f.g = c->g;
+  f.bp = c->bp;
+  f.bp.status = B->dwords.p + 4 * (size_t)i;
+  f.N = c->N;
+  f.G = c->G;
+  f.cap = (uint32_t)std::min<size_t>(c->cap, 0xffffffffu);
+  f.ab = 0;
+  f.rcur = B->par;  // current positions: rbuf[rcur ^ par] = rbuf[0] = the context's rA
+  f.mixed = p.precision == XNB_PRECISION_MIXED ? 1u : 0u;
+  f.A = c->A;
+  f.B = c->B;
+  f.rbuf[0] = c->rA.p;
+  f.rbuf[1] = c->rB.p;
+  f.slot_cell[0] = c->slot_cell_A.p;
+  f.slot_cell[1] = c->slot_cell_B.p;
+  f.cell_off = c->cell_off.p;
+  f.cell_n = c->cell_n.p;
+  f.cnt_out = c->cnt_out.p;
+  f.cnt_in = c->cnt_in.p;
+  f.new_n = c->new_n.p;
+  f.new_off = c->new_off.p;
+  f.in_cur = c->in_cur.p;
+  f.gcnt = c->gcnt.p;
+  f.goff = c->goff.p;
+  f.mv_dst = c->mv_dst.p;
+  f.pk = c->pk.p;
+  f.newloc = c->newloc.p;
+  f.in_list = c->in_list.p;
+  f.ghost_src = c->ghost_src.p;
+  f.ghost_shift = c->ghost_shift.p;
+  f.backup = c->backup.p;
+  f.blist = c->blist.p;
+  f.bcnt = c->bcnt.p;
+  f.lj = make_lj_const(p.lj.epsilon, p.lj.sigma, p.rcut);
+  f.dt = p.dt * 1.0;  // the same expressions as xnb_run
+  f.dt2 = f.dt * f.dt * 0.5;
+  f.dth = p.dt * 0.5;
+  f.thr2 = c->max_displ * c->max_displ;
+  f.max2 = c->nbh_dist_lab * c->nbh_dist_lab;
+  f.words = B->dwords.p + 4 * (size_t)i;
+  f.counts = bb.counts.p;
+  f.gstep0 = B->gstep - (uint32_t)bb.done;  // local step k runs at global step gstep0 + k - 1
+  bb.f = f;
+}
+
+int upload_frame(xnb_batch* B, int i) {
+  B->hF[i] = B->box[i].f;
+  BHIP(B, hipMemcpyAsync(B->dF.p + i, B->hF + i, sizeof(BoxFrame), hipMemcpyHostToDevice, B->st));
+  return XNB_OK;
+}
+
+// Hand the box's current arrays back to its context (positions in rbuf[rcur ^ par]).
+void ctx_from_frame(xnb_batch* B, int i, uint32_t par) {
+  BatchBox& bb = B->box[i];
+  xnb_ctx* c = bb.c;
+  const BoxFrame& f = bb.f;
+  const uint32_t cur = (f.rcur ^ par) & 1u;
+  if (cur) std::swap(c->rA, c->rB);  // rbuf[0] was rA when the frame was built
+  if (f.ab) {
+    for (int a = 0; a < 3; a++) {
+      std::swap(c->vA[a], c->vB[a]);
+      std::swap(c->fA[a], c->fB[a]);
+    }
+    std::swap(c->eA, c->eB);
+    std::swap(c->idA, c->idB);
+    std::swap(c->slot_cell_A, c->slot_cell_B);
+  }
+  bind_states(c);
+  c->G = f.G;
+}
+
+// ---- work tables
+
+struct Tables {
+  std::vector<uint32_t> v;
+  size_t add(const std::vector<uint32_t>& t) {
+    const size_t o = v.size();
+    v.insert(v.end(), t.begin(), t.end());
+    return o;
+  }
+};
+
+std::vector<uint32_t> blocks_table(const std::vector<int>& boxes, const std::vector<uint32_t>& items) {
+  std::vector<uint32_t> t;
+  for (size_t q = 0; q < boxes.size(); q++) {
+    const uint32_t nb = (items[q] + kBatchSlotBlock - 1) / kBatchSlotBlock;
+    for (uint32_t b = 0; b < nb; b++) t.push_back(((uint32_t)boxes[q] << kWorkBoxShift) | b);
+  }
+  return t;
+}
+
+// bricks of all listed boxes, concatenated, dealt to the 8 XCDs in contiguous runs
+std::vector<uint32_t> bricks_table(const std::vector<int>& boxes, const std::vector<uint32_t>& nbricks, const std::vector<uint32_t>& tag) {
+  std::vector<uint32_t> all;
+  for (size_t q = 0; q < boxes.size(); q++)
+    for (uint32_t b = 0; b < nbricks[q]; b++) all.push_back(((uint32_t)boxes[q] << kWorkBoxShift) | (tag[q] << kWorkEpiShift) | b);
+  const size_t n = all.size(), per = (n + kXcds - 1) / kXcds;
+  std::vector<uint32_t> t(per * kXcds, NONE);
+  for (size_t j = 0; j < t.size(); j++) {
+    const size_t g = (j % kXcds) * per + j / kXcds;
+    if (g < n) t[j] = all[g];
+  }
+  return t;
+}
+
+int upload_tables(xnb_batch* B, const Tables& T) {
+  const size_t n = T.v.size();
+  if (n == 0) return XNB_OK;
+  if (n > B->hwork_cap) {
+    BHIP(B, hipStreamSynchronize(B->st));
+    for (int k = 0; k < 2; k++) {
+      if (B->hwork[k]) hipHostFree(B->hwork[k]);
+      B->hwork[k] = nullptr;
+    }
+    const size_t cap = n + n / 2 + 1024;
+    for (int k = 0; k < 2; k++) BHIP(B, hipHostMalloc((void**)&B->hwork[k], cap * sizeof(uint32_t), hipHostMallocDefault));
+    if (B->dwork.p) hipFree(B->dwork.p);
+    B->dwork.p = nullptr;
+    BHIP(B, hipMalloc((void**)&B->dwork.p, cap * sizeof(uint32_t)));
+    B->dwork.n = cap;
+    B->hwork_cap = cap;
+  }
+  const int k = B->hcur;
+  B->hcur ^= 1;
+  BHIP(B, hipEventSynchronize(B->hev[k]));  // the copy that last read this staging buffer is done
+  std::memcpy(B->hwork[k], T.v.data(), n * sizeof(uint32_t));
+  BHIP(B, hipMemcpyAsync(B->dwork.p, B->hwork[k], n * sizeof(uint32_t), hipMemcpyHostToDevice, B->st));
+  BHIP(B, hipEventRecord(B->hev[k], B->st));
+  return XNB_OK;
+}
+
+uint32_t brick_lds_bytes(BrickParams bp) { return brick_layout(bp); }
+
+bool active(const BatchBox& bb) { return bb.used && !bb.finished && !bb.failed; }
+
+// One-box recovery after a batched rebuild overflowed the box's frame sizes: ghosts and
+// lists again with the one-box operators (which re-size), then a new frame.
+int recover_box(xnb_batch* B, int i, const xnb_run_params& p) {
+  BatchBox& bb = B->box[i];
+  xnb_ctx* c = bb.c;
+  ctx_from_frame(B, i, B->par);
+  int e;
+  c->brick_margin = kBrickMargin;
+  if ((e = ensure_capacity(c, (size_t)std::ceil((c->N + c->G) * kSlotMargin) + 256, c->N))) return e;
+  if ((e = op_ghost_all(c))) return e;
+  xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
+  if ((e = op_nbh(c, c->nbh_dist_lab, cfg, 0))) return e;
+  if (c->list_kind != 1) return fail(c, XNB_E_STATE, "box no longer fits the brick path");
+  BHIP(B, hipStreamSynchronize(c->st));
+  frame_from_ctx(B, i, p);
+  return upload_frame(B, i);
+}
+
+}  // namespace
+
+extern "C" {
+
+int xnb_batch_create(xnb_batch** out, int device) {
+  if (!out) return XNB_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return XNB_E_HIP;
+  if (hipSetDevice(device) != hipSuccess) return XNB_E_HIP;
+  xnb_batch* B = new xnb_batch;
+  B->device = device;
+  B->box.resize(kMaxBatchBoxes);
+  int lds = 0;
+  hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+  if (lds > 0) B->lds_limit = (uint32_t)lds;
+  bool ok = hipStreamCreate(&B->st) == hipSuccess;
+  ok = ok && hipMalloc((void**)&B->dF.p, kMaxBatchBoxes * sizeof(BoxFrame)) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&B->hF, kMaxBatchBoxes * sizeof(BoxFrame), hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipMalloc((void**)&B->dwords.p, 4 * kMaxBatchBoxes * sizeof(uint32_t)) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&B->hwords, 4 * kMaxBatchBoxes * sizeof(uint32_t), hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipEventCreate(&B->hev[0]) == hipSuccess && hipEventCreate(&B->hev[1]) == hipSuccess;
+  if (ok) ok = hipMemset(B->dwords.p, 0, 4 * kMaxBatchBoxes * sizeof(uint32_t)) == hipSuccess;
+  if (!ok) {
+    delete B;
+    return XNB_E_HIP;
+  }
+  *out = B;
+  return XNB_OK;
+}
+
+int xnb_batch_destroy(xnb_batch* B) {
+  if (!B) return XNB_E_ARG;
+  hipSetDevice(B->device);
+  hipStreamSynchronize(B->st);
+  for (auto& bb : B->box)
+    if (bb.counts.p) hipFree(bb.counts.p);
+  if (B->dF.p) hipFree(B->dF.p);
+  if (B->hF) hipHostFree(B->hF);
+  if (B->dwords.p) hipFree(B->dwords.p);
+  if (B->hwords) hipHostFree(B->hwords);
+  if (B->dwork.p) hipFree(B->dwork.p);
+  for (int k = 0; k < 2; k++) {
+    if (B->hwork[k]) hipHostFree(B->hwork[k]);
+    if (B->hev[k]) hipEventDestroy(B->hev[k]);
+  }
+  if (B->st) hipStreamDestroy(B->st);
+  delete B;
+  return XNB_OK;
+}
+
+const char* xnb_batch_error(const xnb_batch* B) { return B ? B->err.c_str() : "null batch"; }
+
+int xnb_batch_attach(xnb_batch* B, xnb_ctx* c, const xnb_run_params* p, int* slot) {
+  if (!B || !c || !p || !slot || p->steps < 1 || p->rebuild_every < 1) return bfail(B, XNB_E_ARG, "bad attach arguments");
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (c->device != B->device) return bfail(B, XNB_E_ARG, "box on another device");
+  if (!c->lists_valid || !c->ghosts_valid || !c->backup_valid) return bfail(B, XNB_E_STATE, "xnb_init must run first");
+  int i = -1;
+  for (int q = 0; q < (int)kMaxBatchBoxes; q++)
+    if (!B->box[q].used) {
+      i = q;
+      break;
+    }
+  if (i < 0) return bfail(B, XNB_E_CAPACITY, "batch is full");
+  // head room: ghost slots, and bricks / lists rebuilt with margins
+  c->brick_margin = kBrickMargin;
+  if ((e = ensure_capacity(c, (size_t)std::ceil((c->N + c->G) * kSlotMargin) + 256, (size_t)c->N + c->G))) return e;
+  xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
+  if ((e = op_nbh(c, c->nbh_dist_lab, cfg, 0))) return e;
+  if (c->list_kind != 1) return bfail(B, XNB_E_STATE, "box geometry is not on the brick path (cell edge < nbh_dist)");
+  if (brick_lds_bytes(c->bp) > B->lds_limit) return bfail(B, XNB_E_CAPACITY, "box bricks exceed LDS");
+  if (hipStreamSynchronize(c->st) != hipSuccess) return bfail(B, XNB_E_HIP, "box stream");
+  BatchBox& bb = B->box[i];
+  bb = BatchBox{};
+  bb.c = c;
+  bb.used = true;
+  bb.steps = p->steps;
+  bb.every = p->rebuild_every;
+  bb.rebuilt.assign((size_t)p->steps, 0);
+  if (hipMalloc((void**)&bb.counts.p, (size_t)p->steps * sizeof(unsigned long long)) != hipSuccess) {
+    bb.used = false;
+    return bfail(B, XNB_E_NOMEM, "step counts");
+  }
+  bb.counts.n = (size_t)p->steps;
+  BHIP(B, hipMemsetAsync(bb.counts.p, 0, (size_t)p->steps * sizeof(unsigned long long), B->st));
+  bb.p = *p;
+  c->last_rcut = p->rcut;
+  frame_from_ctx(B, i, *p);
+  if ((e = upload_frame(B, i))) return e;
+  *slot = i;
+  return XNB_OK;
+}
+
+// Advance every unfinished box by up to nsteps steps (fewer for boxes that reach their own
+// step count).  *n_active receives the number of boxes still unfinished afterwards.
+int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
+  if (!B || nsteps < 0) return XNB_E_ARG;
+  BHIP(B, hipSetDevice(B->device));
+  int e;
+  for (int s = 0; s < nsteps; s++) {
+    std::vector<int> act;
+    for (int i = 0; i < (int)kMaxBatchBoxes; i++)
+      if (active(B->box[i])) act.push_back(i);
+    if (act.empty()) break;
+    const uint32_t par = B->par, gstep = B->gstep;
+    // 1. first half of boxes whose previous force kernel did not do it
+    std::vector<int> fresh, reb, quiet;
+    for (int i : act) {
+      BatchBox& bb = B->box[i];
+      if (bb.fresh) fresh.push_back(i);
+      const int64_t k = bb.done + 1;
+      if (k % bb.every == 0) reb.push_back(i);
+      else quiet.push_back(i);
+    }
+    Tables T1;
+    std::vector<uint32_t> nN, nC, nBr;
+    for (int i : fresh) nN.push_back(B->box[i].f.N);
+    const size_t o_fresh = T1.add(blocks_table(fresh, nN));
+    const size_t n_fresh = T1.v.size() - o_fresh;
+    size_t o_rs = 0, n_rs = 0, o_rc = 0, n_rc = 0, o_rb = 0, n_rb = 0, o_list = 0;
+    uint32_t Treb = 64, ldsreb = 0;
+    if (!reb.empty()) {
+      nN.clear();
+      for (int i : reb) {
+        nN.push_back(B->box[i].f.N);
+        nC.push_back(B->box[i].f.g.ncells);
+        nBr.push_back(B->box[i].f.bp.nbricks);
+        Treb = std::max(Treb, B->box[i].f.bp.T);
+        ldsreb = std::max(ldsreb, brick_lds_bytes(B->box[i].f.bp));
+      }
+      o_rs = T1.add(blocks_table(reb, nN));
+      n_rs = T1.v.size() - o_rs;
+      o_rc = T1.add(blocks_table(reb, nC));
+      n_rc = T1.v.size() - o_rc;
+      std::vector<uint32_t> zero(reb.size(), 0u);
+      o_rb = T1.add(bricks_table(reb, nBr, zero));
+      n_rb = T1.v.size() - o_rb;
+      std::vector<uint32_t> lst(reb.begin(), reb.end());
+      o_list = T1.add(lst);
+    }
+    if ((e = upload_tables(B, T1))) return e;
+    const uint32_t* W = B->dwork.p;
+    if (n_fresh) launch_batch_first_half(B->dF.p, W + o_fresh, (uint32_t)n_fresh, par, gstep, B->st);
+    for (int i : fresh) B->box[i].fresh = false;
+    // 2. rebuilds (move_particles, backup_r, ghosts, chunk_neighbors), one launch each
+    if (!reb.empty()) {
+      launch_batch_zero_words(B->dwords.p, W + o_list, (uint32_t)reb.size(), B->st);
+      launch_batch_move(B->dF.p, W + o_rs, (uint32_t)n_rs, W + o_rc, (uint32_t)n_rc, W + o_list, (uint32_t)reb.size(), par, B->st);
+      for (int i : reb) {
+        B->box[i].f.ab ^= 1u;
+        B->box[i].f.rcur ^= 1u;
+        B->box[i].f.G = 0;
+      }
+      launch_batch_ghosts(B->dF.p, W + o_rs, (uint32_t)n_rs, W + o_rc, (uint32_t)n_rc, W + o_list, (uint32_t)reb.size(), par, B->st);
+      hipError_t he = launch_batch_nbh(B->dF.p, W + o_rb, (uint32_t)n_rb, Treb, ldsreb, par, B->st);
+      if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched neighbour build: ") + hipGetErrorString(he));
+      BHIP(B, hipMemcpyAsync(B->hwords, B->dwords.p, 4 * kMaxBatchBoxes * sizeof(uint32_t), hipMemcpyDeviceToHost, B->st));
+      BHIP(B, hipStreamSynchronize(B->st));
+      for (int i : reb) {
+        BatchBox& bb = B->box[i];
+        const uint32_t* w = B->hwords + 4 * (size_t)i;
+        bb.f.G = w[3];
+        bb.c->max_neighbors = w[1];
+        bb.c->slot_lists_current = false;
+        bb.n_rebuilds++;
+        bb.rebuilt[(size_t)bb.done] = 1;
+        if (w[2]) {
+          bb.failed = true;
+          bb.err = "a particle left the domain along a non-periodic axis";
+          continue;
+        }
+        if (w[0] || w[1] > bb.f.bp.cap) {
+          if ((e = recover_box(B, i, bb.p))) {
+            bb.failed = true;
+            bb.err = std::string("rebuild recovery failed: ") + bb.c->err;
+          }
+        }
+      }
+    }
+    // 3. ghost refresh of boxes that did not rebuild, then the force of every box
+    Tables T2;
+    std::vector<int> q2;
+    std::vector<uint32_t> nG;
+    for (int i : quiet)
+      if (B->box[i].f.G) {
+        q2.push_back(i);
+        nG.push_back(B->box[i].f.G);
+      }
+    const size_t o_g = T2.add(blocks_table(q2, nG));
+    const size_t n_g = T2.v.size() - o_g;
+    std::vector<int> frc;
+    std::vector<uint32_t> nbr, tag;
+    uint32_t Tf = 64, ldsf = 0;
+    for (int i : act) {
+      BatchBox& bb = B->box[i];
+      if (bb.failed) continue;
+      const int64_t k = bb.done + 1;
+      const bool fuse = k < bb.steps;
+      const bool lean = fuse && ((k + 1) % bb.every) != 0;
+      frc.push_back(i);
+      nbr.push_back(bb.f.bp.nbricks);
+      tag.push_back(fuse ? (lean ? (uint32_t)EPI_NEXT_LEAN : (uint32_t)EPI_NEXT) : (uint32_t)EPI_KICK);
+      Tf = std::max(Tf, bb.f.bp.T);
+      ldsf = std::max(ldsf, brick_lds_bytes(bb.f.bp));
+    }
+    const size_t o_f = T2.add(bricks_table(frc, nbr, tag));
+    const size_t n_f = T2.v.size() - o_f;
+    if ((e = upload_tables(B, T2))) return e;
+    W = B->dwork.p;
+    launch_batch_ghost_update_r(B->dF.p, W + o_g, (uint32_t)n_g, par, B->st);
+    hipError_t he = launch_batch_lj(B->dF.p, W + o_f, (uint32_t)n_f, Tf, ldsf, par, gstep, B->st);
+    if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched force: ") + hipGetErrorString(he));
+    for (size_t q = 0; q < frc.size(); q++) {
+      BatchBox& bb = B->box[frc[q]];
+      bb.done++;
+      if (tag[q] == (uint32_t)EPI_KICK) {
+        bb.finished = true;
+        bb.last_par = par;
+      }
+    }
+    B->par ^= 1u;
+    B->gstep++;
+  }
+  if (n_active) {
+    int n = 0;
+    for (auto& bb : B->box)
+      if (active(bb)) n++;
+    *n_active = n;
+  }
+  BHIP(B, hipGetLastError());
+  return XNB_OK;
+}
+
+// Progress of the box in a slot: steps done, finished (1), failed (-1, message in
+// xnb_batch_error after the call).
+int xnb_batch_box_status(xnb_batch* B, int slot, int64_t* done, int* state) {
+  if (!B || slot < 0 || slot >= (int)kMaxBatchBoxes || !B->box[slot].used) return XNB_E_ARG;
+  const BatchBox& bb = B->box[slot];
+  if (done) *done = bb.done;
+  if (state) *state = bb.failed ? -1 : (bb.finished ? 1 : 0);
+  if (bb.failed) B->err = bb.err;
+  return XNB_OK;
+}
+
+// Detach a box (finished or not): its context holds the current state again, as after
+// xnb_run; the run's rebuild and guard-violation counts go to p->n_rebuilds/n_violations.
+int xnb_batch_detach(xnb_batch* B, int slot, xnb_run_params* p) {
+  if (!B || slot < 0 || slot >= (int)kMaxBatchBoxes || !B->box[slot].used) return XNB_E_ARG;
+  BHIP(B, hipSetDevice(B->device));
+  BHIP(B, hipStreamSynchronize(B->st));
+  BatchBox& bb = B->box[slot];
+  ctx_from_frame(B, slot, bb.finished ? bb.last_par : B->par);
+  // a box detached before its last step keeps the fused first half of its next step; the
+  // one-box loop restarts from a completed step, so only finished boxes are returned as
+  // such (unfinished boxes are for tear-down)
+  int64_t viol = 0;
+  if (bb.done > 0) {
+    std::vector<unsigned long long> h((size_t)bb.done);
+    BHIP(B, hipMemcpy(h.data(), bb.counts.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < h.size(); q++)
+      if (h[q] > 0 && !bb.rebuilt[q]) viol++;
+  }
+  if (p) {
+    p->n_rebuilds = bb.n_rebuilds;
+    p->n_violations = viol;
+  }
+  const int rc = bb.failed ? XNB_E_STATE : XNB_OK;
+  if (bb.failed) B->err = bb.err;
+  hipFree(bb.counts.p);
+  bb = BatchBox{};
+  return rc;
+}
+
+}  // extern "C"

@@ -54,6 +54,8 @@ __device__ inline uint32_t upper_index(const uint32_t* pref, uint32_t n, uint32_
   return lo;
 }
 
+static constexpr int LJB_UNROLL_B = (int)kListBatch;  // list entries per force batch
+
 // position of entry q of lane t in a brick's list slab (T lanes)
 __device__ inline size_t lpos(uint32_t q, uint32_t t, uint32_t T) { return ((size_t)(q >> 2) * T + t) * 4 + (q & 3u); }
 
@@ -99,8 +101,10 @@ __device__ inline BrickPos brick_pos(const Geom& g, const BrickParams& bp, uint3
 }
 
 // Region tables (+ positions when load_pos) into LDS.  Needs nrc <= blockDim.x and
-// nbc <= blockDim.x.  Ends with a workgroup barrier.
-__device__ inline void brick_setup(const Geom& g, const BrickParams& bp, const BrickPos& P, const BrickLds& L,
+// nbc <= blockDim.x.  Ends with a workgroup barrier.  Returns false (the same for every
+// thread, with bp.status flagged) when the region holds more than Rmax atoms or the brick
+// more than T, the sizes its LDS and lists were laid out for.
+__device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const BrickPos& P, const BrickLds& L,
                                    const double4* __restrict__ r, const uint32_t* __restrict__ cell_off,
                                    const uint32_t* __restrict__ cell_n, bool load_pos) {
   const uint32_t t = threadIdx.x;
@@ -135,6 +139,11 @@ __device__ inline void brick_setup(const Geom& g, const BrickParams& bp, const B
   if (t < (uint32_t)bp.nbc) L.bpref[t] = bex;
   if (t == 0) L.bpref[bp.nbc] = tot;
   __syncthreads();
+  const bool fits_r = L.rpref[bp.nrc] <= bp.Rmax, fits_t = L.bpref[bp.nbc] <= bp.T;
+  if (!fits_r || !fits_t) {
+    if (t == 0) atomicOr(bp.status, (fits_r ? 0u : 1u) | (fits_t ? 0u : 2u));
+    return false;
+  }
   if (load_pos) {
     const uint32_t R = L.rpref[bp.nrc];
     for (uint32_t idx = t; idx < R; idx += blockDim.x) {
@@ -151,6 +160,7 @@ __device__ inline void brick_setup(const Geom& g, const BrickParams& bp, const B
     }
     __syncthreads();
   }
+  return true;
 }
 
 struct LaneAtom {
@@ -210,19 +220,19 @@ __global__ void k_brick_stats(Geom g, BrickParams bp, const uint32_t* __restrict
 // entries in candidate order.  Stores go to row
 // min(cnt, cap - 1): an overflowing list is rebuilt with a larger cap, so its last row may be
 // overwritten.  FILTER enables the half_symmetric / skip_ghosts filters.
+// Brick b of one box; the workgroup may be wider than bp.T (batched launches), lanes from
+// bp.T on hold no atom and touch no list.
 template <bool FILTER>
-__global__ void __launch_bounds__(1024) k_nbh_brick_mask(Geom g, BrickParams bp, const double4* __restrict__ r,
-                                                         const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
-                                                         double max2, int half_symmetric, int skip_ghosts, uint16_t* __restrict__ blist,
-                                                         uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt) {
+__device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams& bp, uint32_t b, const double4* __restrict__ r,
+                                               const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
+                                               double max2, int half_symmetric, int skip_ghosts, uint16_t* __restrict__ blist,
+                                               uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt) {
   extern __shared__ __align__(16) char lds[];
-  const uint32_t b = brick_of_block(bp);
-  if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
-  brick_setup(g, bp, P, L, r, cell_off, cell_n, true);
+  if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, true)) return;
   const LaneAtom A = lane_atom(bp, L);
-  const uint32_t t = threadIdx.x, T = blockDim.x;
+  const uint32_t t = threadIdx.x, T = bp.T;
   uint16_t* out = blist + (size_t)b * bp.cap * T;
   // every lane, live or not, tests against its own LDS atom when past the candidate cell's
   // end (A.ia = 0 for lanes without an atom), so d2 == 0 rejects those candidates
@@ -282,7 +292,7 @@ __global__ void __launch_bounds__(1024) k_nbh_brick_mask(Geom g, BrickParams bp,
         }
       }
   if (!A.live) cnt = 0;
-  bcnt[(size_t)b * T + t] = cnt;
+  if (t < T) bcnt[(size_t)b * T + t] = cnt;
   const uint32_t wmax = wave_max_b(cnt);
   if ((t & 63) == 0 && wmax > 0) atomicMax(max_cnt, wmax);
   // pad to the wave's batch boundary with the sentinel, so force kernels need no per-lane
@@ -290,6 +300,16 @@ __global__ void __launch_bounds__(1024) k_nbh_brick_mask(Geom g, BrickParams bp,
   const uint32_t pad_end = min((wmax + kListBatch - 1) / kListBatch * kListBatch, bp.cap);
   const uint16_t sentinel = (uint16_t)L.rpref[bp.nrc];
   for (uint32_t q = min(cnt, bp.cap); q < pad_end; q++) out[lpos(q, t, T)] = sentinel;
+}
+
+template <bool FILTER>
+__global__ void __launch_bounds__(1024) k_nbh_brick_mask(Geom g, BrickParams bp, const double4* __restrict__ r,
+                                                         const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
+                                                         double max2, int half_symmetric, int skip_ghosts, uint16_t* __restrict__ blist,
+                                                         uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt) {
+  const uint32_t b = brick_of_block(bp);
+  if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
+  nbh_brick_body<FILTER>(g, bp, b, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist, bcnt, max_cnt);
 }
 
 // ------------------------------------------------------------------ force
@@ -306,18 +326,16 @@ __global__ void __launch_bounds__(1024) k_nbh_brick_mask(Geom g, BrickParams bp,
 // Positions are read from r (staged in LDS) and, with EPI_NEXT, written to a second buffer,
 // so no workgroup reads a position another one has already advanced.
 template <bool MIXED, bool ACCUM, int EPI, int U>
-__global__ void __launch_bounds__(1024) k_lj_brick(Geom g, BrickParams bp, const double4* __restrict__ r,
-                                                   const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
-                                                   const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt,
-                                                   LJConst lj, PState A, double dth, NextStep nx) {
+__device__ __forceinline__ void lj_brick_body(const Geom& g, const BrickParams& bp, uint32_t b, const double4* __restrict__ r,
+                                              const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
+                                              const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt,
+                                              const LJConst& lj, const PState& A, double dth, const NextStep& nx) {
   extern __shared__ __align__(16) char lds[];
-  const uint32_t b = brick_of_block(bp);
-  if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
-  brick_setup(g, bp, P, L, r, cell_off, cell_n, true);
+  if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, true)) return;
   const LaneAtom At = lane_atom(bp, L);
-  const uint32_t t = threadIdx.x, T = blockDim.x;
+  const uint32_t t = threadIdx.x, T = bp.T;
   const uint32_t n = At.live ? bcnt[(size_t)b * T + t] : 0u;
   const uint32_t nmax = wave_max_b(n);
   double4 ra;
@@ -403,6 +421,16 @@ __global__ void __launch_bounds__(1024) k_lj_brick(Geom g, BrickParams bp, const
   if (NEXT) wave_count_add(over, nx.count);
 }
 
+template <bool MIXED, bool ACCUM, int EPI, int U>
+__global__ void __launch_bounds__(1024) k_lj_brick(Geom g, BrickParams bp, const double4* __restrict__ r,
+                                                   const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
+                                                   const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt,
+                                                   LJConst lj, PState A, double dth, NextStep nx) {
+  const uint32_t b = brick_of_block(bp);
+  if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
+  lj_brick_body<MIXED, ACCUM, EPI, U>(g, bp, b, r, cell_off, cell_n, blist, bcnt, lj, A, dth, nx);
+}
+
 // ------------------------------------------------------------------ conversion
 // brick list -> generic per-slot u32 list (slot indices), for stream export and the generic
 // traversals.
@@ -415,10 +443,10 @@ __global__ void __launch_bounds__(1024) k_brick_to_slots(Geom g, BrickParams bp,
   if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
-  brick_setup(g, bp, P, L, nullptr, cell_off, cell_n, false);
+  if (!brick_setup(g, bp, P, L, nullptr, cell_off, cell_n, false)) return;
   const LaneAtom A = lane_atom(bp, L);
   if (!A.live) return;
-  const uint32_t t = threadIdx.x, T = blockDim.x;
+  const uint32_t t = threadIdx.x, T = bp.T;
   const uint32_t n = min(bcnt[(size_t)b * T + t], bp.cap);
   const uint16_t* lst = blist + (size_t)b * bp.cap * T;
   uint32_t* outp = nbr + nbr_index(A.slot, 0, cap);
@@ -493,7 +521,7 @@ hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, con
   return hipGetLastError();
 }
 
-static constexpr int LJB_UNROLL = 8;
+static constexpr int LJB_UNROLL = LJB_UNROLL_B;
 
 template <bool MIXED, bool ACCUM, int EPI>
 static hipError_t ljb_launch(const Geom& g, const BrickParams& bp, uint32_t lds, const double4* r, const uint32_t* cell_off,
@@ -535,6 +563,75 @@ hipError_t launch_brick_to_slots(const Geom& g, BrickParams bp, const uint32_t* 
   hipError_t e = set_lds((const void*)k_brick_to_slots, lds);
   if (e != hipSuccess) return e;
   k_brick_to_slots<<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, cell_off, cell_n, blist, bcnt, nbr, nbr_cnt, cap);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ batched boxes
+// work[blockIdx.x] = box << 20 | brick of the box (NONE: idle).  The workgroup has the batch's
+// widest T; a box with a smaller T leaves the extra waves without atoms.
+
+__device__ inline bool batch_brick(const uint32_t* __restrict__ work, uint32_t& box, uint32_t& b) {
+  const uint32_t w = work[blockIdx.x];
+  if (w == NONE) return false;
+  box = w >> kWorkBoxShift;
+  b = w & kWorkBlockMask;
+  return true;
+}
+
+__global__ void __launch_bounds__(1024) kb_nbh_brick(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work, uint32_t par) {
+  uint32_t box, b;
+  if (!batch_brick(work, box, b)) return;  // whole workgroup
+  const BoxFrame& f = F[box];
+  const double4* r = f.rbuf[(f.rcur ^ par) & 1u];
+  nbh_brick_body<false>(f.g, f.bp, b, r, f.cell_off, f.cell_n, f.max2, 0, 0, f.blist, f.bcnt, f.words + 1);
+}
+
+template <bool MIXED, int EPI>
+__device__ __forceinline__ void lj_batch_case(const BoxFrame& f, uint32_t b, uint32_t par, uint32_t gstep) {
+  PState A = f.ab ? f.B : f.A;
+  const uint32_t cur = (f.rcur ^ par) & 1u;
+  A.r = f.rbuf[cur];
+  // the next step's count: local step k = gstep - gstep0 + 1, count of step k + 1 at [k]
+  const NextStep nx{f.rbuf[cur ^ 1u], f.backup, f.dt, f.dt2, f.thr2, f.counts + (gstep - f.gstep0 + 1)};
+  lj_brick_body<MIXED, false, EPI, LJB_UNROLL_B>(f.g, f.bp, b, A.r, f.cell_off, f.cell_n, f.blist, f.bcnt, f.lj, A, f.dth, nx);
+}
+
+// Force of every brick of every box; precision and epilogue are per box (uniform per
+// workgroup).  EPI_KICK closes a box's run; EPI_NEXT / EPI_NEXT_LEAN fuse its next step.
+__global__ void __launch_bounds__(1024) kb_lj_brick(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work, uint32_t par,
+                                                    uint32_t gstep) {
+  uint32_t box, b;
+  if (!batch_brick(work, box, b)) return;  // whole workgroup
+  const uint32_t epi = b >> kWorkEpiShift;
+  b &= kWorkBrickMask;
+  const BoxFrame& f = F[box];
+  const uint32_t key = (f.mixed ? 4u : 0u) | epi;
+  switch (key) {
+    case EPI_KICK: lj_batch_case<false, EPI_KICK>(f, b, par, gstep); break;
+    case EPI_NEXT: lj_batch_case<false, EPI_NEXT>(f, b, par, gstep); break;
+    case EPI_NEXT_LEAN: lj_batch_case<false, EPI_NEXT_LEAN>(f, b, par, gstep); break;
+    case 4u | EPI_KICK: lj_batch_case<true, EPI_KICK>(f, b, par, gstep); break;
+    case 4u | EPI_NEXT: lj_batch_case<true, EPI_NEXT>(f, b, par, gstep); break;
+    case 4u | EPI_NEXT_LEAN: lj_batch_case<true, EPI_NEXT_LEAN>(f, b, par, gstep); break;
+    default: break;
+  }
+}
+
+hipError_t launch_batch_nbh(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds, uint32_t par,
+                            hipStream_t st) {
+  if (!nwork) return hipSuccess;
+  hipError_t e = set_lds((const void*)kb_nbh_brick, lds);
+  if (e != hipSuccess) return e;
+  kb_nbh_brick<<<nwork, T, lds, st>>>(F, work, par);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_lj(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds, uint32_t par,
+                           uint32_t gstep, hipStream_t st) {
+  if (!nwork) return hipSuccess;
+  hipError_t e = set_lds((const void*)kb_lj_brick, lds);
+  if (e != hipSuccess) return e;
+  kb_lj_brick<<<nwork, T, lds, st>>>(F, work, par, gstep);
   return hipGetLastError();
 }
 

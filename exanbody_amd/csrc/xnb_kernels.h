@@ -28,6 +28,7 @@ struct BrickParams {
   uint32_t T;     // threads per workgroup (>= brick atoms, nrc, nbc; multiple of 64)
   uint32_t Rmax;  // largest region atom count
   uint32_t cap;   // list rows per brick slab
+  uint32_t* status;  // device word: bit 0 set when a region exceeds Rmax, bit 1 when a brick exceeds T
   uint32_t off_rcnt, off_rpref, off_roff, off_bpref, off_pos;  // LDS byte offsets
 };
 
@@ -52,6 +53,52 @@ struct PState {
   uint64_t* id;
 };
 
+// ------------------------------------------------------------------ batched boxes
+// One box of a batch as the batched kernels see it (device memory, one entry per slot of the
+// batch).  Kernels may update ab, rcur and G; the host keeps a mirror.
+struct BoxFrame {
+  Geom g;
+  BrickParams bp;  // bp.status == words + 0
+  uint32_t N, G, cap;
+  uint32_t ab;     // 1: the roles of the A and B field arrays are exchanged
+  uint32_t rcur;   // current positions are rbuf[rcur ^ parity] (parity: argument of the launch)
+  uint32_t mixed;  // 1: mixed precision force
+  PState A, B;     // .r unused: positions live in rbuf
+  double4* rbuf[2];
+  uint32_t* slot_cell[2];  // [0] goes with A, [1] with B
+  uint32_t *cell_off, *cell_n, *cnt_out, *cnt_in, *new_n, *new_off, *in_cur, *gcnt, *goff;
+  uint32_t *mv_dst, *pk, *newloc, *in_list, *ghost_src, *ghost_shift, *backup;
+  uint16_t* blist;
+  uint32_t* bcnt;
+  LJConst lj;
+  double dt, dt2, dth, thr2, max2;
+  uint32_t* words;             // [0] status bits (1, 2 brick sizes; 4 ghost capacity), [1] max list count, [2] left the domain, [3] ghost count
+  unsigned long long* counts;  // displacement count of local step k at [k - 1]
+  uint32_t gstep0;             // global step number of the box's local step 1
+};
+
+// work tables: one u32 per workgroup, box << 20 | block within the box, or NONE; force tables
+// carry the box's epilogue in bits 18-19 (box << 20 | epi << 18 | brick)
+constexpr uint32_t kWorkBoxShift = 20;
+constexpr uint32_t kWorkBlockMask = (1u << kWorkBoxShift) - 1;
+constexpr uint32_t kWorkEpiShift = 18;
+constexpr uint32_t kWorkBrickMask = (1u << kWorkEpiShift) - 1;
+constexpr uint32_t kMaxBatchBoxes = 4096;  // box index < 2^12
+
+void launch_batch_zero_cells(const BoxFrame* F, const uint32_t* work, uint32_t nwork, hipStream_t st);
+void launch_batch_zero_words(uint32_t* words, const uint32_t* boxes, uint32_t n, hipStream_t st);
+void launch_batch_move(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wcells, uint32_t ncells,
+                       const uint32_t* boxes, uint32_t nboxes, uint32_t par, hipStream_t st);
+void launch_batch_ghosts(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wcells, uint32_t ncells,
+                         const uint32_t* boxes, uint32_t nboxes, uint32_t par, hipStream_t st);
+void launch_batch_ghost_update_r(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t par, hipStream_t st);
+void launch_batch_first_half(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t par, uint32_t gstep, hipStream_t st);
+hipError_t launch_batch_nbh(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds, uint32_t par,
+                            hipStream_t st);
+hipError_t launch_batch_lj(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds, uint32_t par,
+                           uint32_t gstep, hipStream_t st);
+constexpr uint32_t kBatchSlotBlock = 256;  // threads per workgroup of the per-slot / per-cell batched kernels
+
 void scan_exclusive(const uint32_t* in, uint32_t n, uint32_t add, uint32_t* out, uint32_t* bsum, uint32_t* total, hipStream_t st);
 uint32_t scan_max_items();
 
@@ -71,7 +118,7 @@ void launch_set_inner_cells(const Geom& g, const uint32_t* new_n, const uint32_t
 void launch_ghost_count(const Geom& g, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, uint32_t* gcnt,
                         hipStream_t st);
 void launch_ghost_fill(const Geom& g, uint32_t N, const PState& B, const uint32_t* goff, uint32_t* cell_off, uint32_t* cell_n,
-                       uint32_t* slot_cell, uint32_t* ghost_src, uint32_t* ghost_shift, hipStream_t st);
+                       uint32_t* slot_cell, uint32_t* ghost_src, uint32_t* ghost_shift, uint32_t cap, uint32_t* status, hipStream_t st);
 void launch_ghost_update_r(const Geom& g, uint32_t N, uint32_t G, double4* r, const uint32_t* ghost_src, const uint32_t* ghost_shift,
                            hipStream_t st);
 void launch_backup_r(const Geom& g, uint32_t N, const double4* r, const uint32_t* slot_cell, uint32_t* backup, hipStream_t st);

@@ -108,11 +108,9 @@ uint32_t scan_max_items() { return (uint32_t)SCAN_TILE * (uint32_t)SCAN_TILE; }
 // (source cells are visited in index order, particles in cell order, and inner slots are
 // laid out in exactly that order).
 
-__global__ void k_move_flag(Geom g, uint32_t N, const double4* __restrict__ r, const uint32_t* __restrict__ slot_cell,
+__device__ inline void move_flag_item(Geom g, uint32_t N, const double4* __restrict__ r, const uint32_t* __restrict__ slot_cell,
                             uint32_t* __restrict__ mv_dst, uint32_t* __restrict__ cnt_out, uint32_t* __restrict__ cnt_in,
-                            uint32_t* __restrict__ err) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= N) return;
+                            uint32_t* __restrict__ err, uint32_t s) {
   const uint32_t c = slot_cell[s];
   int si, sj, sk;
   cell_ijk(g, c, si, sj, sk);
@@ -153,13 +151,19 @@ __global__ void k_move_flag(Geom g, uint32_t N, const double4* __restrict__ r, c
   atomicAdd(&cnt_in[dc], 1u);
 }
 
+__global__ void k_move_flag(Geom g, uint32_t N, const double4* __restrict__ r, const uint32_t* __restrict__ slot_cell,
+                            uint32_t* __restrict__ mv_dst, uint32_t* __restrict__ cnt_out, uint32_t* __restrict__ cnt_in,
+                            uint32_t* __restrict__ err) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  move_flag_item(g, N, r, slot_cell, mv_dst, cnt_out, cnt_in, err, s);
+}
+
 // one thread per cell: swap-with-last packing (move_particles_across_cells.h:186-209)
-__global__ void k_move_pack(Geom g, const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
+__device__ inline void move_pack_item(Geom g, const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
                             const uint32_t* __restrict__ cnt_out, const uint32_t* __restrict__ cnt_in,
                             const uint32_t* __restrict__ mv_dst, uint32_t* __restrict__ pk, uint32_t* __restrict__ newloc,
-                            uint32_t* __restrict__ new_n) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.ncells) return;
+                            uint32_t* __restrict__ new_n, uint32_t c) {
   int i, j, k;
   cell_ijk(g, c, i, j, k);
   if (is_ghost(g, i, j, k)) {
@@ -183,6 +187,15 @@ __global__ void k_move_pack(Geom g, const uint32_t* __restrict__ cell_off, const
   for (int q = 0; q < m; q++) newloc[off + pk[off + q]] = (uint32_t)q;
 }
 
+__global__ void k_move_pack(Geom g, const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
+                            const uint32_t* __restrict__ cnt_out, const uint32_t* __restrict__ cnt_in,
+                            const uint32_t* __restrict__ mv_dst, uint32_t* __restrict__ pk, uint32_t* __restrict__ newloc,
+                            uint32_t* __restrict__ new_n) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncells) return;
+  move_pack_item(g, cell_off, cell_n, cnt_out, cnt_in, mv_dst, pk, newloc, new_n, c);
+}
+
 __device__ inline void copy_particle(const PState& A, uint32_t s, const PState& B, uint32_t t) {
   B.r[t] = A.r[s];
   B.vx[t] = A.vx[s]; B.vy[t] = A.vy[s]; B.vz[t] = A.vz[s];
@@ -191,13 +204,11 @@ __device__ inline void copy_particle(const PState& A, uint32_t s, const PState& 
   B.id[t] = A.id[s];
 }
 
-__global__ void k_move_scatter(uint32_t N, PState A, PState B, const uint32_t* __restrict__ slot_cell,
+__device__ inline void move_scatter_item(uint32_t N, PState A, PState B, const uint32_t* __restrict__ slot_cell,
                                const uint32_t* __restrict__ mv_dst, const uint32_t* __restrict__ newloc,
                                const uint32_t* __restrict__ new_off, const uint32_t* __restrict__ cell_n,
                                const uint32_t* __restrict__ cnt_out, uint32_t* __restrict__ in_cur,
-                               uint32_t* __restrict__ in_list, uint32_t* __restrict__ slot_cell_B) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= N) return;
+                               uint32_t* __restrict__ in_list, uint32_t* __restrict__ slot_cell_B, uint32_t s) {
   const uint32_t c = slot_cell[s];
   const uint32_t dc = mv_dst[s];
   if (dc == NONE) {
@@ -210,13 +221,21 @@ __global__ void k_move_scatter(uint32_t N, PState A, PState B, const uint32_t* _
   }
 }
 
+__global__ void k_move_scatter(uint32_t N, PState A, PState B, const uint32_t* __restrict__ slot_cell,
+                               const uint32_t* __restrict__ mv_dst, const uint32_t* __restrict__ newloc,
+                               const uint32_t* __restrict__ new_off, const uint32_t* __restrict__ cell_n,
+                               const uint32_t* __restrict__ cnt_out, uint32_t* __restrict__ in_cur,
+                               uint32_t* __restrict__ in_list, uint32_t* __restrict__ slot_cell_B) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  move_scatter_item(N, A, B, slot_cell, mv_dst, newloc, new_off, cell_n, cnt_out, in_cur, in_list, slot_cell_B, s);
+}
+
 // one thread per cell: order incoming movers by source slot, copy them with wrapped positions
-__global__ void k_move_incoming(Geom g, PState A, PState B, const uint32_t* __restrict__ new_off,
+__device__ inline void move_incoming_item(Geom g, PState A, PState B, const uint32_t* __restrict__ new_off,
                                 const uint32_t* __restrict__ cell_n, const uint32_t* __restrict__ cnt_out,
                                 const uint32_t* __restrict__ cnt_in, uint32_t* __restrict__ in_list,
-                                uint32_t* __restrict__ slot_cell_B) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.ncells) return;
+                                uint32_t* __restrict__ slot_cell_B, uint32_t c) {
   const uint32_t nin = cnt_in[c];
   if (nin == 0) return;
   const uint32_t b = new_off[c] + (cell_n[c] - cnt_out[c]);
@@ -240,15 +259,29 @@ __global__ void k_move_incoming(Geom g, PState A, PState B, const uint32_t* __re
   }
 }
 
-__global__ void k_set_inner_cells(uint32_t ncells, const uint32_t* __restrict__ new_n, const uint32_t* __restrict__ new_off,
-                                  Geom g, uint32_t* __restrict__ cell_n, uint32_t* __restrict__ cell_off) {
+__global__ void k_move_incoming(Geom g, PState A, PState B, const uint32_t* __restrict__ new_off,
+                                const uint32_t* __restrict__ cell_n, const uint32_t* __restrict__ cnt_out,
+                                const uint32_t* __restrict__ cnt_in, uint32_t* __restrict__ in_list,
+                                uint32_t* __restrict__ slot_cell_B) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
+  if (c >= g.ncells) return;
+  move_incoming_item(g, A, B, new_off, cell_n, cnt_out, cnt_in, in_list, slot_cell_B, c);
+}
+
+__device__ inline void set_inner_cells_item(uint32_t ncells, const uint32_t* __restrict__ new_n, const uint32_t* __restrict__ new_off,
+                                  Geom g, uint32_t* __restrict__ cell_n, uint32_t* __restrict__ cell_off, uint32_t c) {
   int i, j, k;
   cell_ijk(g, c, i, j, k);
   if (is_ghost(g, i, j, k)) return;
   cell_n[c] = new_n[c];
   cell_off[c] = new_off[c];
+}
+
+__global__ void k_set_inner_cells(uint32_t ncells, const uint32_t* __restrict__ new_n, const uint32_t* __restrict__ new_off,
+                                  Geom g, uint32_t* __restrict__ cell_n, uint32_t* __restrict__ cell_off) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  set_inner_cells_item(ncells, new_n, new_off, g, cell_n, cell_off, c);
 }
 
 // ------------------------------------------------------------------------ ghosts
@@ -280,10 +313,8 @@ __device__ inline bool ghost_keep(const Geom& g, double4 p, int si, int sj, int 
   return q.x >= bx0 && q.x <= bx1 && q.y >= by0 && q.y <= by1 && q.z >= bz0 && q.z <= bz1;
 }
 
-__global__ void k_ghost_count(Geom g, const double4* __restrict__ r, const uint32_t* __restrict__ cell_off,
-                              const uint32_t* __restrict__ cell_n, uint32_t* __restrict__ gcnt) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.ncells) return;
+__device__ inline void ghost_count_item(Geom g, const double4* __restrict__ r, const uint32_t* __restrict__ cell_off,
+                              const uint32_t* __restrict__ cell_n, uint32_t* __restrict__ gcnt, uint32_t c) {
   uint32_t src;
   int si, sj, sk;
   uint32_t cnt = 0;
@@ -297,11 +328,19 @@ __global__ void k_ghost_count(Geom g, const double4* __restrict__ r, const uint3
   gcnt[c] = cnt;
 }
 
-__global__ void k_ghost_fill(Geom g, uint32_t N, PState B, const uint32_t* __restrict__ goff, uint32_t* __restrict__ cell_off,
-                             uint32_t* __restrict__ cell_n, uint32_t* __restrict__ slot_cell, uint32_t* __restrict__ ghost_src,
-                             uint32_t* __restrict__ ghost_shift) {
+__global__ void k_ghost_count(Geom g, const double4* __restrict__ r, const uint32_t* __restrict__ cell_off,
+                              const uint32_t* __restrict__ cell_n, uint32_t* __restrict__ gcnt) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.ncells) return;
+  ghost_count_item(g, r, cell_off, cell_n, gcnt, c);
+}
+
+// Ghost cell c of one box.  Slots past cap are not written: the cell count is still stored
+// and *status gets bit 2, so the caller sizes the buffers and runs the fill again.
+__device__ inline void ghost_fill_item(const Geom& g, uint32_t N, const PState& B, const uint32_t* __restrict__ goff,
+                                       uint32_t* __restrict__ cell_off, uint32_t* __restrict__ cell_n, uint32_t* __restrict__ slot_cell,
+                                       uint32_t* __restrict__ ghost_src, uint32_t* __restrict__ ghost_shift, uint32_t cap,
+                                       uint32_t* __restrict__ status, uint32_t c) {
   int i, j, k;
   cell_ijk(g, c, i, j, k);
   if (!is_ghost(g, i, j, k)) return;
@@ -316,6 +355,11 @@ __global__ void k_ghost_fill(Geom g, uint32_t N, PState B, const uint32_t* __res
       double4 gp;
       if (ghost_keep(g, B.r[off + q], si, sj, sk, gp)) {
         const uint32_t t = base + cnt;
+        ++cnt;
+        if (t >= cap) {
+          atomicOr(status, 4u);
+          continue;
+        }
         B.r[t] = gp;
         B.vx[t] = B.vx[off + q]; B.vy[t] = B.vy[off + q]; B.vz[t] = B.vz[off + q];
         B.fx[t] = 0.; B.fy[t] = 0.; B.fz[t] = 0.; B.ep[t] = 0.;
@@ -323,7 +367,6 @@ __global__ void k_ghost_fill(Geom g, uint32_t N, PState B, const uint32_t* __res
         slot_cell[t] = c;
         ghost_src[t - N] = off + q;
         ghost_shift[t - N] = code;
-        ++cnt;
       }
     }
   }
@@ -331,11 +374,17 @@ __global__ void k_ghost_fill(Geom g, uint32_t N, PState B, const uint32_t* __res
   cell_n[c] = cnt;
 }
 
+__global__ void k_ghost_fill(Geom g, uint32_t N, PState B, const uint32_t* __restrict__ goff, uint32_t* __restrict__ cell_off,
+                             uint32_t* __restrict__ cell_n, uint32_t* __restrict__ slot_cell, uint32_t* __restrict__ ghost_src,
+                             uint32_t* __restrict__ ghost_shift, uint32_t cap, uint32_t* __restrict__ status) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncells) return;
+  ghost_fill_item(g, N, B, goff, cell_off, cell_n, slot_cell, ghost_src, ghost_shift, cap, status, c);
+}
+
 // ghost_update_r (update_ghosts.cu:58): re-copy shifted positions along the stored scheme
-__global__ void k_ghost_update_r(Geom g, uint32_t N, uint32_t G, double4* __restrict__ r, const uint32_t* __restrict__ ghost_src,
-                                 const uint32_t* __restrict__ ghost_shift) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= G) return;
+__device__ inline void ghost_update_r_item(Geom g, uint32_t N, uint32_t G, double4* __restrict__ r, const uint32_t* __restrict__ ghost_src,
+                                 const uint32_t* __restrict__ ghost_shift, uint32_t q) {
   const uint32_t code = ghost_shift[q];
   const double4 p = r[ghost_src[q]];
   double4 o;
@@ -346,18 +395,30 @@ __global__ void k_ghost_update_r(Geom g, uint32_t N, uint32_t G, double4* __rest
   r[N + q] = o;
 }
 
+__global__ void k_ghost_update_r(Geom g, uint32_t N, uint32_t G, double4* __restrict__ r, const uint32_t* __restrict__ ghost_src,
+                                 const uint32_t* __restrict__ ghost_shift) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= G) return;
+  ghost_update_r_item(g, N, G, r, ghost_src, ghost_shift, q);
+}
+
 // --------------------------------------------------------------- backup / trigger
 
-__global__ void k_backup_r(Geom g, uint32_t N, const double4* __restrict__ r, const uint32_t* __restrict__ slot_cell,
-                           uint32_t* __restrict__ backup) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= N) return;
+__device__ inline void backup_r_item(Geom g, uint32_t N, const double4* __restrict__ r, const uint32_t* __restrict__ slot_cell,
+                           uint32_t* __restrict__ backup, uint32_t s) {
   int i, j, k;
   cell_ijk(g, slot_cell[s], i, j, k);
   const double4 p = r[s];
   backup[3 * (size_t)s + 0] = encode_double_u32(p.x, cell_origin(g.ox, g.gl, i, g.cs), g.cs);
   backup[3 * (size_t)s + 1] = encode_double_u32(p.y, cell_origin(g.oy, g.gl, j, g.cs), g.cs);
   backup[3 * (size_t)s + 2] = encode_double_u32(p.z, cell_origin(g.oz, g.gl, k, g.cs), g.cs);
+}
+
+__global__ void k_backup_r(Geom g, uint32_t N, const double4* __restrict__ r, const uint32_t* __restrict__ slot_cell,
+                           uint32_t* __restrict__ backup) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  backup_r_item(g, N, r, slot_cell, backup, s);
 }
 
 // particle_displ_over (particle_displ_over.cu:38-77)
@@ -396,6 +457,27 @@ __global__ void k_push_f_v(uint32_t N, double* __restrict__ vx, double* __restri
 
 // verlet_first_half fused: push_f_v_r(1.0) then push_f_v(0.5), then the
 // particle_displ_over test of the new position (same arithmetic as the separate kernels).
+__device__ inline bool verlet_first_half_item(const Geom& g, double4* __restrict__ r, double* __restrict__ vx,
+                                              double* __restrict__ vy, double* __restrict__ vz, const double* __restrict__ fx,
+                                              const double* __restrict__ fy, const double* __restrict__ fz, double dt, double dt2,
+                                              double dth, const uint32_t* __restrict__ slot_cell, const uint32_t* __restrict__ backup,
+                                              double thr2, uint32_t s) {
+  double4 p = r[s];
+  double ux = vx[s], uy = vy[s], uz = vz[s];
+  const double ax = fx[s], ay = fy[s], az = fz[s];
+  p.x += ux * dt + ax * dt2;
+  p.y += uy * dt + ay * dt2;
+  p.z += uz * dt + az * dt2;
+  r[s] = p;
+  ux += ax * dth;
+  uy += ay * dth;
+  uz += az * dth;
+  vx[s] = ux;
+  vy[s] = uy;
+  vz[s] = uz;
+  return displaced(g, p, slot_cell[s], backup, s, thr2);
+}
+
 __global__ void k_verlet_first_half(Geom g, uint32_t N, double4* __restrict__ r, double* __restrict__ vx,
                                     double* __restrict__ vy, double* __restrict__ vz, const double* __restrict__ fx,
                                     const double* __restrict__ fy, const double* __restrict__ fz, double dt, double dt2,
@@ -403,22 +485,7 @@ __global__ void k_verlet_first_half(Geom g, uint32_t N, double4* __restrict__ r,
                                     double thr2, unsigned long long* __restrict__ count) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   bool over = false;
-  if (s < N) {
-    double4 p = r[s];
-    double ux = vx[s], uy = vy[s], uz = vz[s];
-    const double ax = fx[s], ay = fy[s], az = fz[s];
-    p.x += ux * dt + ax * dt2;
-    p.y += uy * dt + ay * dt2;
-    p.z += uz * dt + az * dt2;
-    r[s] = p;
-    ux += ax * dth;
-    uy += ay * dth;
-    uz += az * dth;
-    vx[s] = ux;
-    vy[s] = uy;
-    vz[s] = uz;
-    over = displaced(g, p, slot_cell[s], backup, s, thr2);
-  }
+  if (s < N) over = verlet_first_half_item(g, r, vx, vy, vz, fx, fy, fz, dt, dt2, dth, slot_cell, backup, thr2, s);
   wave_count_add(over, count);
 }
 
@@ -856,8 +923,8 @@ void launch_ghost_count(const Geom& g, const double4* r, const uint32_t* cell_of
   k_ghost_count<<<nblk(g.ncells, 128), 128, 0, st>>>(g, r, cell_off, cell_n, gcnt);
 }
 void launch_ghost_fill(const Geom& g, uint32_t N, const PState& B, const uint32_t* goff, uint32_t* cell_off, uint32_t* cell_n,
-                       uint32_t* slot_cell, uint32_t* ghost_src, uint32_t* ghost_shift, hipStream_t st) {
-  k_ghost_fill<<<nblk(g.ncells, 128), 128, 0, st>>>(g, N, B, goff, cell_off, cell_n, slot_cell, ghost_src, ghost_shift);
+                       uint32_t* slot_cell, uint32_t* ghost_src, uint32_t* ghost_shift, uint32_t cap, uint32_t* status, hipStream_t st) {
+  k_ghost_fill<<<nblk(g.ncells, 128), 128, 0, st>>>(g, N, B, goff, cell_off, cell_n, slot_cell, ghost_src, ghost_shift, cap, status);
 }
 void launch_ghost_update_r(const Geom& g, uint32_t N, uint32_t G, double4* r, const uint32_t* ghost_src,
                            const uint32_t* ghost_shift, hipStream_t st) {
@@ -966,6 +1033,235 @@ void launch_stream_fill(const Geom& g, uint32_t n_atoms, const uint32_t* slot_ce
 void launch_energies(uint32_t N, const PState& A, double* part, uint32_t nparts, double* out, hipStream_t st) {
   k_energy_partials<<<nparts, RED_THREADS, 0, st>>>(N, A.vx, A.vy, A.vz, A.ep, part);
   k_energy_final<<<1, 64, 0, st>>>(nparts, part, out);
+}
+
+// ================================================================== batched boxes
+// Every batched launch covers many boxes: work[blockIdx.x] names the box and the block
+// within it (NONE: idle workgroup).  Each workgroup serves one box, so a box's frame is read
+// with wave-uniform (scalar) loads, and per-item bodies are those of the one-box kernels.
+
+__device__ inline bool batch_item(const uint32_t* __restrict__ work, uint32_t& box, uint32_t& idx) {
+  const uint32_t w = work[blockIdx.x];
+  if (w == NONE) return false;
+  box = w >> kWorkBoxShift;
+  idx = (w & kWorkBlockMask) * blockDim.x + threadIdx.x;
+  return true;
+}
+
+struct FrameState {
+  PState A, B;
+  uint32_t *scA, *scB;
+};
+
+__device__ inline FrameState frame_state(const BoxFrame& f, uint32_t par) {
+  FrameState s;
+  s.A = f.ab ? f.B : f.A;
+  s.B = f.ab ? f.A : f.B;
+  s.scA = f.slot_cell[f.ab & 1u];
+  s.scB = f.slot_cell[(f.ab & 1u) ^ 1u];
+  const uint32_t cur = (f.rcur ^ par) & 1u;
+  s.A.r = f.rbuf[cur];
+  s.B.r = f.rbuf[cur ^ 1u];
+  return s;
+}
+
+__global__ void __launch_bounds__(kBatchSlotBlock) kb_zero_cells(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work) {
+  uint32_t box, c;
+  if (!batch_item(work, box, c)) return;
+  const BoxFrame& f = F[box];
+  if (c >= f.g.ncells) return;
+  f.cnt_out[c] = 0;
+  f.cnt_in[c] = 0;
+  f.in_cur[c] = 0;
+}
+
+__global__ void __launch_bounds__(kBatchSlotBlock) kb_move_flag(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
+                                                                 uint32_t par) {
+  uint32_t box, s;
+  if (!batch_item(work, box, s)) return;
+  const BoxFrame& f = F[box];
+  if (s >= f.N) return;
+  const FrameState X = frame_state(f, par);
+  move_flag_item(f.g, f.N, X.A.r, X.scA, f.mv_dst, f.cnt_out, f.cnt_in, f.words + 2, s);
+}
+
+__global__ void __launch_bounds__(kBatchSlotBlock) kb_move_pack(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work) {
+  uint32_t box, c;
+  if (!batch_item(work, box, c)) return;
+  const BoxFrame& f = F[box];
+  if (c >= f.g.ncells) return;
+  move_pack_item(f.g, f.cell_off, f.cell_n, f.cnt_out, f.cnt_in, f.mv_dst, f.pk, f.newloc, f.new_n, c);
+}
+
+// One workgroup of 1024 threads per listed box: exclusive scan of the box's per-cell counts.
+// which 0: new_n -> new_off; which 1: gcnt -> goff and the total into the frame's G.
+__global__ void __launch_bounds__(1024) kb_scan_cells(BoxFrame* __restrict__ F, const uint32_t* __restrict__ boxes, int which) {
+  __shared__ uint32_t wsum[16];
+  const uint32_t box = boxes[blockIdx.x];
+  BoxFrame& f = F[box];
+  const uint32_t* in = which ? f.gcnt : f.new_n;
+  uint32_t* out = which ? f.goff : f.new_off;
+  const uint32_t n = f.g.ncells;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < n; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < n ? in[i] : 0u;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+    for (int q = 0; q < 16; q++) {
+      if (q < w) woff += wsum[q];
+      tot += wsum[q];
+    }
+    __syncthreads();
+    if (i < n) out[i] = carry + woff + x - v;
+    carry += tot;
+  }
+  if (which && threadIdx.x == 0) {
+    f.G = carry;
+    f.words[3] = carry;
+  }
+}
+
+__global__ void __launch_bounds__(kBatchSlotBlock) kb_move_scatter(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
+                                                                    uint32_t par) {
+  uint32_t box, s;
+  if (!batch_item(work, box, s)) return;
+  const BoxFrame& f = F[box];
+  if (s >= f.N) return;
+  const FrameState X = frame_state(f, par);
+  move_scatter_item(f.N, X.A, X.B, X.scA, f.mv_dst, f.newloc, f.new_off, f.cell_n, f.cnt_out, f.in_cur, f.in_list, X.scB, s);
+}
+
+__global__ void __launch_bounds__(kBatchSlotBlock) kb_move_incoming(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
+                                                                     uint32_t par) {
+  uint32_t box, c;
+  if (!batch_item(work, box, c)) return;
+  const BoxFrame& f = F[box];
+  if (c >= f.g.ncells) return;
+  const FrameState X = frame_state(f, par);
+  move_incoming_item(f.g, X.A, X.B, f.new_off, f.cell_n, f.cnt_out, f.cnt_in, f.in_list, X.scB, c);
+}
+
+__global__ void __launch_bounds__(kBatchSlotBlock) kb_set_inner_cells(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work) {
+  uint32_t box, c;
+  if (!batch_item(work, box, c)) return;
+  const BoxFrame& f = F[box];
+  if (c >= f.g.ncells) return;
+  set_inner_cells_item(f.g.ncells, f.new_n, f.new_off, f.g, f.cell_n, f.cell_off, c);
+}
+
+// after move_particles: the B arrays become the current ones (the one-box swap_buffers)
+__global__ void kb_flip(BoxFrame* __restrict__ F, const uint32_t* __restrict__ boxes, uint32_t nboxes) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nboxes) return;
+  BoxFrame& f = F[boxes[q]];
+  f.ab ^= 1u;
+  f.rcur ^= 1u;
+  f.G = 0;
+}
+
+__global__ void __launch_bounds__(kBatchSlotBlock) kb_backup_r(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
+                                                                uint32_t par) {
+  uint32_t box, s;
+  if (!batch_item(work, box, s)) return;
+  const BoxFrame& f = F[box];
+  if (s >= f.N) return;
+  const FrameState X = frame_state(f, par);
+  backup_r_item(f.g, f.N, X.A.r, X.scA, f.backup, s);
+}
+
+__global__ void __launch_bounds__(kBatchSlotBlock) kb_ghost_count(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
+                                                                   uint32_t par) {
+  uint32_t box, c;
+  if (!batch_item(work, box, c)) return;
+  const BoxFrame& f = F[box];
+  if (c >= f.g.ncells) return;
+  const FrameState X = frame_state(f, par);
+  ghost_count_item(f.g, X.A.r, f.cell_off, f.cell_n, f.gcnt, c);
+}
+
+__global__ void __launch_bounds__(kBatchSlotBlock) kb_ghost_fill(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
+                                                                  uint32_t par) {
+  uint32_t box, c;
+  if (!batch_item(work, box, c)) return;
+  const BoxFrame& f = F[box];
+  if (c >= f.g.ncells) return;
+  const FrameState X = frame_state(f, par);
+  ghost_fill_item(f.g, f.N, X.A, f.goff, f.cell_off, f.cell_n, X.scA, f.ghost_src, f.ghost_shift, f.cap, f.words + 0, c);
+}
+
+__global__ void __launch_bounds__(kBatchSlotBlock) kb_ghost_update_r(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
+                                                                      uint32_t par) {
+  uint32_t box, q;
+  if (!batch_item(work, box, q)) return;
+  const BoxFrame& f = F[box];
+  if (q >= f.G) return;
+  ghost_update_r_item(f.g, f.N, f.G, f.rbuf[(f.rcur ^ par) & 1u], f.ghost_src, f.ghost_shift, q);
+}
+
+__global__ void __launch_bounds__(kBatchSlotBlock) kb_first_half(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
+                                                                  uint32_t par, uint32_t gstep) {
+  uint32_t box, s;
+  if (!batch_item(work, box, s)) return;  // whole workgroup
+  const BoxFrame& f = F[box];
+  const FrameState X = frame_state(f, par);
+  bool over = false;
+  if (s < f.N)
+    over = verlet_first_half_item(f.g, X.A.r, X.A.vx, X.A.vy, X.A.vz, X.A.fx, X.A.fy, X.A.fz, f.dt, f.dt2, f.dth, X.scA, f.backup,
+                                  f.thr2, s);
+  wave_count_add(over, f.counts + (gstep - f.gstep0));  // local step k = gstep - gstep0 + 1 -> [k - 1]
+}
+
+__global__ void kb_zero_words(uint32_t* __restrict__ words, const uint32_t* __restrict__ boxes, uint32_t n) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 4 * n) return;
+  words[4 * (size_t)boxes[q >> 2] + (q & 3u)] = 0;
+}
+
+void launch_batch_zero_words(uint32_t* words, const uint32_t* boxes, uint32_t n, hipStream_t st) {
+  if (n) kb_zero_words<<<(4 * n + 255) / 256, 256, 0, st>>>(words, boxes, n);
+}
+
+void launch_batch_zero_cells(const BoxFrame* F, const uint32_t* work, uint32_t nwork, hipStream_t st) {
+  if (nwork) kb_zero_cells<<<nwork, kBatchSlotBlock, 0, st>>>(F, work);
+}
+
+void launch_batch_move(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wcells, uint32_t ncells,
+                       const uint32_t* boxes, uint32_t nboxes, uint32_t par, hipStream_t st) {
+  if (!nboxes) return;
+  launch_batch_zero_cells(F, wcells, ncells, st);
+  if (nslots) kb_move_flag<<<nslots, kBatchSlotBlock, 0, st>>>(F, wslots, par);
+  if (ncells) kb_move_pack<<<ncells, kBatchSlotBlock, 0, st>>>(F, wcells);
+  kb_scan_cells<<<nboxes, 1024, 0, st>>>(F, boxes, 0);
+  if (nslots) kb_move_scatter<<<nslots, kBatchSlotBlock, 0, st>>>(F, wslots, par);
+  if (ncells) kb_move_incoming<<<ncells, kBatchSlotBlock, 0, st>>>(F, wcells, par);
+  if (ncells) kb_set_inner_cells<<<ncells, kBatchSlotBlock, 0, st>>>(F, wcells);
+  kb_flip<<<(nboxes + 255) / 256, 256, 0, st>>>(F, boxes, nboxes);
+}
+
+void launch_batch_ghosts(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wcells, uint32_t ncells,
+                         const uint32_t* boxes, uint32_t nboxes, uint32_t par, hipStream_t st) {
+  if (!nboxes) return;
+  if (nslots) kb_backup_r<<<nslots, kBatchSlotBlock, 0, st>>>(F, wslots, par);
+  if (ncells) kb_ghost_count<<<ncells, kBatchSlotBlock, 0, st>>>(F, wcells, par);
+  kb_scan_cells<<<nboxes, 1024, 0, st>>>(F, boxes, 1);
+  if (ncells) kb_ghost_fill<<<ncells, kBatchSlotBlock, 0, st>>>(F, wcells, par);
+}
+
+void launch_batch_ghost_update_r(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t par, hipStream_t st) {
+  if (nwork) kb_ghost_update_r<<<nwork, kBatchSlotBlock, 0, st>>>(F, work, par);
+}
+
+void launch_batch_first_half(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t par, uint32_t gstep, hipStream_t st) {
+  if (nwork) kb_first_half<<<nwork, kBatchSlotBlock, 0, st>>>(F, work, par, gstep);
 }
 
 }  // namespace xnb

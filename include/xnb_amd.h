@@ -185,6 +185,27 @@ int xnb_copy_probe(xnb_ctx* ctx, size_t bytes, int iters, double* gbps);
  * [2^-767, 2^1000], recip_out[i] for x[i] in [2^-383, 2^383]. */
 int xnb_fp64_core(xnb_ctx* ctx, const double* x, size_t n, double* sqrt_out, double* recip_out);
 
+/* ---- batches of boxes ---------------------------------------------------------- */
+/* No counterpart in the reference, which advances one domain per process: many independent
+ * periodic boxes advanced together on one GPU, each kernel of a step being one launch over
+ * all boxes.  A box is an xnb_ctx after xnb_init (brick-path geometry: cell edge >=
+ * nbh_dist); attached with the parameters xnb_run would take (fixed-interval rebuild,
+ * rebuild_every >= 1), it follows xnb_run's operator sequence and ends bitwise equal to the
+ * same box advanced by xnb_run alone.  Boxes may be attached and detached between steps. */
+typedef struct xnb_batch xnb_batch;
+int xnb_batch_create(xnb_batch** out, int device);
+int xnb_batch_destroy(xnb_batch* batch);
+const char* xnb_batch_error(const xnb_batch* batch);
+/* Attach an initialised box for p->steps steps; *slot identifies it in the batch. */
+int xnb_batch_attach(xnb_batch* batch, xnb_ctx* box, const xnb_run_params* p, int* slot);
+/* Advance every unfinished box by up to nsteps steps; *n_active = unfinished boxes left. */
+int xnb_batch_step(xnb_batch* batch, int nsteps, int* n_active);
+/* Steps done and state (0 running, 1 finished, -1 failed) of the box in a slot. */
+int xnb_batch_box_status(xnb_batch* batch, int slot, int64_t* done, int* state);
+/* Detach the box of a slot; its xnb_ctx then holds the state xnb_run would have produced
+ * (for a finished box) and p->n_rebuilds / n_violations are filled as by xnb_run. */
+int xnb_batch_detach(xnb_batch* batch, int slot, xnb_run_params* p);
+
 /* ---- operator decks ---------------------------------------------------------- */
 /* YAML-subset decks with the reference's operator names and slots (the .msp structure of
  * data/config/numerical-scheme.msp, update-particles.msp, compute-loop.msp): named

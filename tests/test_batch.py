@@ -1,0 +1,103 @@
+"""Batched boxes (xnb_batch): every box advanced in a batch, alone or in the mix, attached and
+detached at different steps, ends bitwise equal to the same box advanced by xnb_run alone --
+FP64 and mixed precision alike (same per-atom operations in the same order)."""
+import numpy as np
+import pytest
+
+from exanbody_amd import PRECISION_FP64, PRECISION_MIXED, Batch, Box, cubic_domain, generate_fcc
+
+pytestmark = pytest.mark.gpu
+
+RCUT, SKIN, DT = 2.5, 0.3, 0.005
+
+# (ncell, density, temperature, seed, steps, precision, rebuild_every)
+JOBS = [
+    (4, 0.84, 0.72, 11, 23, PRECISION_FP64, 10),
+    (5, 0.75, 1.20, 12, 31, PRECISION_MIXED, 10),
+    (6, 0.90, 0.50, 13, 17, PRECISION_FP64, 5),
+    (7, 0.80, 1.00, 14, 40, PRECISION_MIXED, 10),
+    (5, 0.70, 1.50, 15, 12, PRECISION_MIXED, 4),
+    (8, 0.85, 0.80, 16, 26, PRECISION_FP64, 10),
+]
+
+
+def _box(job):
+    ncell, rho, temp, seed = job[:4]
+    r, v, ids, L = generate_fcc(ncell, rho, jitter=0.01, temperature=temp, seed=seed)
+    dom = cubic_domain(L, RCUT + SKIN)
+    b = Box(dom["bounds_min"], dom["bounds_max"], dom["grid_dims"], dom["cell_size"])
+    b.nbh_dist(RCUT, SKIN)
+    b.set_particles(r, v, ids)
+    b.init(1.0, 1.0, RCUT, SKIN, job[5])
+    return b
+
+
+def _alone(job):
+    b = _box(job)
+    res = b.run(job[4], DT, 1.0, 1.0, RCUT, SKIN, precision=job[5], rebuild_every=job[6])
+    out = b.export(), res
+    b.close()
+    return out
+
+
+def _same(a, b, what):
+    for k in ("cell", "id"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=f"{what}: {k}")
+    for k in ("r", "v", "f", "ep"):
+        np.testing.assert_array_equal(a[k].view(np.uint64), b[k].view(np.uint64), err_msg=f"{what}: {k} (bitwise)")
+
+
+def test_batch_equals_alone_all_at_once():
+    ref = [_alone(j) for j in JOBS]
+    batch = Batch()
+    boxes, slots = [], []
+    for j in JOBS:
+        b = _box(j)
+        boxes.append(b)
+        slots.append(batch.attach(b, j[4], DT, 1.0, 1.0, RCUT, SKIN, precision=j[5], rebuild_every=j[6]))
+    left = len(JOBS)
+    guard = 0
+    while left:
+        left = batch.step(7)
+        guard += 1
+        assert guard < 100
+    for q, (j, b, s) in enumerate(zip(JOBS, boxes, slots)):
+        assert batch.status(s) == (j[4], 1)
+        res = batch.detach(s)
+        assert res == ref[q][1], f"job {q}: rebuild / violation counts"
+        _same(b.export(), ref[q][0], f"job {q}")
+        b.close()
+    batch.close()
+
+
+def test_batch_continuous_admission():
+    """Jobs arrive while others run (attach between steps) and leave when done; each result
+    is the same as alone."""
+    ref = [_alone(j) for j in JOBS]
+    batch = Batch()
+    pending = list(range(len(JOBS)))
+    running = {}
+    results = {}
+    t = 0
+    while pending or running:
+        # admit one job every 5 steps, at most 3 at a time
+        if pending and len(running) < 3 and t % 5 == 0:
+            q = pending.pop(0)
+            b = _box(JOBS[q])
+            j = JOBS[q]
+            running[batch.attach(b, j[4], DT, 1.0, 1.0, RCUT, SKIN, precision=j[5], rebuild_every=j[6])] = (q, b)
+        batch.step(1)
+        t += 1
+        for s in list(running):
+            done, state = batch.status(s)
+            assert state >= 0
+            if state == 1:
+                q, b = running.pop(s)
+                assert done == JOBS[q][4]
+                results[q] = (batch.detach(s), b.export())
+                b.close()
+        assert t < 1000
+    for q in range(len(JOBS)):
+        assert results[q][0] == ref[q][1], f"job {q}: counts"
+        _same(results[q][1], ref[q][0], f"job {q}")
+    batch.close()
