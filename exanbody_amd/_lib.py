@@ -91,6 +91,8 @@ _SIGS = {
     "xnb_batch_step": (C.c_int, [P, C.c_int, C.POINTER(C.c_int)]),
     "xnb_batch_box_status": (C.c_int, [P, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "xnb_batch_detach": (C.c_int, [P, C.c_int, C.POINTER(RunParams)]),
+    "xnb_job_bytes": (C.c_size_t, [C.c_size_t]),
+    "xnb_serve_run": (C.c_int, [P, C.c_int, P, P, P]),
     "xnb_deck_create": (C.c_int, [C.POINTER(P)]),
     "xnb_deck_destroy": (C.c_int, [P]),
     "xnb_deck_load": (C.c_int, [P, C.c_char_p]),

@@ -17,7 +17,7 @@ LIB = os.path.join(LIBDIR, "libxnb_amd.so")
 ARCH = os.environ.get("XNB_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["xnb_kernels.hip", "xnb_brick.hip", "xnb_api.hip", "xnb_batch.hip"]
-CXX_SOURCES = ["xnb_generate.cpp", "xnb_operators.cpp"]
+CXX_SOURCES = ["xnb_generate.cpp", "xnb_operators.cpp", "xnb_serve.cpp"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 HIP_FLAGS = [

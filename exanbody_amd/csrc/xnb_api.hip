@@ -158,8 +158,15 @@ int ensure_capacity(xnb_ctx* c, size_t ncap, size_t keep) {
     dfree(*a);
     if ((e = dalloc(c, *a, newcap))) return e;
   }
-  dfree(c->backup);
-  if ((e = dalloc(c, c->backup, 3 * newcap))) return e;
+  {  // backup_r of the kept (inner) slots survives: a ghost rebuild may grow the buffers after it
+    DArr<uint32_t> nb;
+    if ((e = dalloc(c, nb, 3 * newcap))) return e;
+    if (c->backup.p && keep)
+      HIPCHK(c, hipMemcpyAsync(nb.p, c->backup.p, 3 * keep * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    dfree(c->backup);
+    c->backup = nb;
+  }
   c->cap = newcap;
   bind_states(c);
   return XNB_OK;

@@ -206,6 +206,50 @@ int xnb_batch_box_status(xnb_batch* batch, int slot, int64_t* done, int* state);
  * (for a finished box) and p->n_rebuilds / n_violations are filled as by xnb_run. */
 int xnb_batch_detach(xnb_batch* batch, int slot, xnb_run_params* p);
 
+/* ---- serving loop ---------------------------------------------------------------- */
+/* No counterpart in the reference.  Continuous batching of MD jobs: jobs become visible at
+ * their arrival time, are admitted in arrival order while the estimated device bytes of the
+ * admitted jobs stay within the budget (and at most max_active run), advance together in one
+ * xnb_batch, and leave when their steps are done; a job admitted while others run overlaps
+ * with them on the device.  max_active = 1 serves one job at a time on the same path. */
+typedef struct {
+  double arrival_s;                    /* seconds after the loop starts */
+  size_t n;                            /* atoms */
+  const double* r;                     /* n x 3 */
+  const double* v;                     /* n x 3, or NULL */
+  const uint64_t* id;                  /* n, or NULL (0..n-1) */
+  xnb_domain domain;
+  xnb_run_params run;                  /* dt, lj, rcut, rcut_inc, precision, rebuild_every, steps */
+  double* r_out;                       /* optional n x 3: final positions, cell order */
+  double* v_out;                       /* optional n x 3 */
+  uint64_t* id_out;                    /* optional n */
+} xnb_job;
+
+typedef struct {
+  double start_s, finish_s, latency_s; /* admission, completion, completion - arrival */
+  int status;                          /* XNB_OK or the error of the job */
+  int64_t n_rebuilds, n_violations;
+} xnb_job_result;
+
+typedef struct {
+  int device;
+  size_t hbm_budget_bytes;             /* 0: no limit */
+  int max_active;                      /* <= 0: no limit */
+  int steps_per_iter;                  /* batch steps between admission checks (>= 1) */
+} xnb_serve_config;
+
+typedef struct {
+  double wall_s, atom_steps, atom_steps_per_s;
+  double p50_latency_s, p99_latency_s, max_latency_s, mean_active;
+  int completed, failed, peak_active;
+  size_t peak_bytes;
+} xnb_serve_stats;
+
+/* Device bytes the loop budgets for a job of n atoms. */
+size_t xnb_job_bytes(size_t n);
+int xnb_serve_run(const xnb_job* jobs, int njobs, const xnb_serve_config* cfg, xnb_job_result* results,
+                  xnb_serve_stats* stats);
+
 /* ---- operator decks ---------------------------------------------------------- */
 /* YAML-subset decks with the reference's operator names and slots (the .msp structure of
  * data/config/numerical-scheme.msp, update-particles.msp, compute-loop.msp): named
