@@ -115,6 +115,79 @@ def cpu_baseline(inputs, dom, cpu_steps):
                        f"FP64 oracle (C/OpenMP restatement of the reference operators), {wall:.1f} s wall")
 
 
+def batched_bench(n_boxes, steps, alone_sample, seed=5):
+    """Many independent boxes (atoms log-uniform in [500, 50000], density U[0.70, 0.90],
+    T* U[0.5, 1.5], mixed precision, rebuild every 10) advanced by one batch, one launch per
+    kernel per step, against one box at a time (xnb_run) on a sample of the same boxes."""
+    from exanbody_amd import Batch, Box, PRECISION_MIXED
+    from exanbody_amd.serve import make_job
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for q in range(n_boxes):
+        n = int(round(np.exp(rng.uniform(np.log(500), np.log(50000)))))
+        jobs.append(make_job(n, rng.uniform(0.70, 0.90), rng.uniform(0.5, 1.5), steps, PRECISION_MIXED, seed=7000 + q))
+
+    def make_box(j):
+        d = j["domain"]
+        b = Box(d["bounds_min"], d["bounds_max"], d["grid_dims"], d["cell_size"])
+        b.nbh_dist(RCUT, SKIN)
+        b.set_particles(j["r"], j["v"], j["id"])
+        b.init(1.0, 1.0, RCUT, SKIN, PRECISION_MIXED)
+        return b
+
+    t = time.perf_counter()
+    boxes = [make_box(j) for j in jobs]
+    setup = time.perf_counter() - t
+    batch = Batch()
+    slots = [batch.attach(b, steps, DT, 1.0, 1.0, RCUT, SKIN, precision=PRECISION_MIXED, rebuild_every=EVERY) for b in boxes]
+    boxes[0].synchronize()
+    t0 = time.perf_counter()
+    left = len(slots)
+    while left:
+        left = batch.step(steps)
+    for s in slots:
+        batch.detach(s)  # synchronises
+    wall = time.perf_counter() - t0
+    atoms = np.array([j["r"].shape[0] for j in jobs])
+    for b in boxes:
+        b.close()
+    batch.close()
+    # one box at a time on a sample of the same boxes
+    pick = np.random.default_rng(seed + 1).choice(n_boxes, size=min(alone_sample, n_boxes), replace=False)
+    t_alone, as_alone = 0.0, 0.0
+    for q in pick:
+        b = make_box(jobs[q])
+        b.synchronize()
+        t1 = time.perf_counter()
+        b.run(steps, DT, 1.0, 1.0, RCUT, SKIN, precision=PRECISION_MIXED, rebuild_every=EVERY)
+        b.synchronize()
+        t_alone += time.perf_counter() - t1
+        as_alone += atoms[q] * steps
+        b.close()
+    batched = float(atoms.sum() * steps / wall)
+    alone = float(as_alone / t_alone)
+    return dict(workload=f"{n_boxes} periodic LJ boxes, atoms log-uniform [500, 50000] (total {int(atoms.sum())}), "
+                         f"rho* U[0.70, 0.90], T* U[0.5, 1.5], {steps} steps each, mixed, rebuild every {EVERY}",
+                atom_steps_per_s=batched, ms_per_step=1e3 * wall / steps, wall_s=wall, setup_s=setup,
+                one_at_a_time_atom_steps_per_s=alone, one_at_a_time_sample=int(len(pick)),
+                speedup_vs_one_at_a_time=batched / alone)
+
+
+def serving_bench(n_jobs, rate, seed=9):
+    """Poisson arrivals of LJ jobs (atoms log-uniform [1000, 200000], rho* U[0.70, 0.90],
+    T* U[0.5, 1.5], steps U[100, 2000], half FP64 / half mixed) through the continuous-batching
+    loop, and the same jobs served one at a time."""
+    from exanbody_amd.serve import job_mix, serve
+    jobs = job_mix(n_jobs, rate, seed=seed)
+    out = dict(workload=f"{n_jobs} jobs, Poisson arrivals at {rate}/s, atoms log-uniform [1000, 200000], "
+                        "rho* U[0.70, 0.90], T* U[0.5, 1.5], steps U[100, 2000], half FP64 / half mixed")
+    for name, max_active in (("continuous_batching", 0), ("one_at_a_time", 1)):
+        _, st = serve(jobs, max_active=max_active)
+        out[name] = {k: (float(v) if isinstance(v, float) else int(v)) for k, v in st.items()}
+    out["throughput_ratio"] = out["continuous_batching"]["atom_steps_per_s"] / out["one_at_a_time"]["atom_steps_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +199,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--batch-boxes", type=int, default=512, help="batched workload size (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--batch-alone-sample", type=int, default=32)
+    ap.add_argument("--serve-jobs", type=int, default=64, help="serving workload size (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -170,6 +248,15 @@ def main():
     b_alg = 144.0
     step_ceiling = min(HBM_PEAK_GBS * 1e9 / b_alg, VALU_PEAK_TFLOPS * 1e12 * (2.0 if head_prec == "mixed" else 1.0) /
                        flops_per_atom)
+    batched = serving = None
+    if args.batch_boxes > 0:
+        log("batched ...")
+        batched = batched_bench(args.batch_boxes, args.batch_steps, args.batch_alone_sample)
+        log("batched", json.dumps(batched))
+    if args.serve_jobs > 0:
+        log("serving ...")
+        serving = serving_bench(args.serve_jobs, args.serve_rate)
+        log("serving", json.dumps(serving))
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -187,7 +274,7 @@ def main():
         results=results, roofline=roofline,
         step_roofline=dict(b_alg_bytes_per_atom_step=b_alg, flops_per_atom_step=flops_per_atom,
                            ceiling_atom_steps_per_s=step_ceiling, frac=head["atom_steps_per_s"] / step_ceiling),
-        copy_probe_gbs=copy_gbs, cpu_baseline=cpu)
+        copy_probe_gbs=copy_gbs, batched=batched, serving=serving, cpu_baseline=cpu)
     print(json.dumps(line), flush=True)
 
 

@@ -77,6 +77,7 @@ _SIGS = {
     "xnb_simulation_energies": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "xnb_init": (C.c_int, [P, C.POINTER(LJParams), C.c_double, C.c_double, C.c_int]),
     "xnb_run": (C.c_int, [P, C.POINTER(RunParams)]),
+    "xnb_run_multimat": (C.c_int, [P, C.POINTER(RunParams), C.c_int, P, P, P]),
     "xnb_timer_enable": (C.c_int, [P, C.c_int]),
     "xnb_timer_reset": (C.c_int, [P]),
     "xnb_timer_read": (C.c_int, [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),

@@ -190,6 +190,16 @@ class Box:
         self._check(self.L.xnb_run(self._h, C.byref(p)), "run")
         return dict(rebuilds=p.n_rebuilds, violations=p.n_violations)
 
+    def run_multimat(self, steps, dt, ntypes, eps_pair, sigma_pair, rcut_pair, rcut_inc, precision=_lib.PRECISION_FP64,
+                     rebuild_every=0):
+        e = np.ascontiguousarray(eps_pair, np.float64)
+        s = np.ascontiguousarray(sigma_pair, np.float64)
+        rc = np.ascontiguousarray(rcut_pair, np.float64)
+        p = _lib.RunParams(dt, _lib.LJParams(1.0, 1.0), float(rc.max()), rcut_inc, int(precision), int(rebuild_every), int(steps),
+                           0, 0)
+        self._check(self.L.xnb_run_multimat(self._h, C.byref(p), int(ntypes), _ptr(e), _ptr(s), _ptr(rc)), "run_multimat")
+        return dict(rebuilds=p.n_rebuilds, violations=p.n_violations)
+
     # ---- measurement
     def timer_enable(self, on=True):
         self.L.xnb_timer_enable(self._h, int(on))

@@ -416,6 +416,25 @@ int launch_force(xnb_ctx* c, int precision, bool accum, bool push, bool ghost, d
   return XNB_OK;
 }
 
+// Multimat force of a run-loop step: zero + compute_pair_multimat (+ the epilogue) in one brick
+// kernel, or, on generic lists, the separate kernels (no fusion).
+int launch_force_multimat(xnb_ctx* c, int precision, const LJMulti& mm, int ntypes, double dth, const NextStep* next, bool lean) {
+  if (c->list_kind == 1) {
+    const int epi = next ? (lean ? EPI_NEXT_LEAN : EPI_NEXT) : EPI_KICK;
+    hipError_t he = launch_ljmm_brick(precision, ntypes, epi, c->g, c->bp, c->A, c->cell_off.p, c->cell_n.p, c->blist.p, c->bcnt.p, mm,
+                                      dth, next, c->st);
+    if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick multimat kernel: ") + hipGetErrorString(he));
+    return XNB_OK;
+  }
+  if (next) return fail(c, XNB_E_STATE, "fused next-step push needs brick lists");
+  launch_zero_force(c->N, c->A, c->st);
+  if (launch_lj_multimat(precision, c->N, c->A, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, ntypes, mm, c->st))
+    return fail(c, XNB_E_ARG, "too many type pairs");
+  launch_push_f_v(c->N, c->A, dth, c->st);
+  HIPCHK(c, hipGetLastError());
+  return XNB_OK;
+}
+
 int full_update(xnb_ctx* c) {
   int e;
   if ((e = op_move(c))) return e;
@@ -844,10 +863,18 @@ int xnb_lennard_jones_force_multimat(xnb_ctx* c, int ntypes, const double* eps, 
   if ((e = check_ctx(c))) return e;
   if (ntypes <= 0 || ntypes > 4 || !eps || !sigma || !rcut) return fail(c, XNB_E_ARG, "bad multimat arguments (1..4 types)");
   if (!c->lists_valid) return fail(c, XNB_E_STATE, "no neighbour lists");
-  if ((e = ensure_slot_lists(c))) return e;
+  LJMulti mm;
+  if (make_lj_multi(ntypes, eps, sigma, rcut, mm)) return fail(c, XNB_E_ARG, "too many type pairs");
   tstart(c, XNB_TIMER_FORCE);
-  if (launch_lj_multimat(precision, c->N, c->A, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, ntypes, eps, sigma, rcut, c->st))
-    return fail(c, XNB_E_ARG, "too many type pairs");
+  if (c->list_kind == 1) {
+    hipError_t he = launch_ljmm_brick(precision, ntypes, EPI_NONE, c->g, c->bp, c->A, c->cell_off.p, c->cell_n.p, c->blist.p, c->bcnt.p,
+                                      mm, 0.0, nullptr, c->st);
+    if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick multimat kernel: ") + hipGetErrorString(he));
+  } else {
+    if ((e = ensure_slot_lists(c))) return e;
+    if (launch_lj_multimat(precision, c->N, c->A, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, ntypes, mm, c->st))
+      return fail(c, XNB_E_ARG, "too many type pairs");
+  }
   tstop(c);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->st));
@@ -914,7 +941,12 @@ int xnb_init(xnb_ctx* c, const xnb_lj_params* p, double rcut, double rcut_inc, i
 // push kernel's.  Forces and per-atom energies are stored on the last step and on steps
 // followed by a rebuild; the state at return equals the unfused loop's bit for bit.
 // Displacement counts go to step_counts[step - 1].
-int xnb_run(xnb_ctx* c, xnb_run_params* p) {
+}  // extern "C"
+
+namespace xnb_impl {
+
+// The step loop of xnb_run / xnb_run_multimat (mm == nullptr: single-material LJ).
+int run_loop(xnb_ctx* c, xnb_run_params* p, const LJMulti* mm, int ntypes) {
   int e;
   if ((e = check_ctx(c))) return e;
   if (!p || p->steps < 0) return fail(c, XNB_E_ARG, "bad run parameters");
@@ -961,7 +993,8 @@ int xnb_run(xnb_ctx* c, xnb_run_params* p) {
     const bool lean = fuse && interval && ((step + 1) % p->rebuild_every) != 0;
     NextStep nx{c->rB.p, c->backup.p, dt, dt2, thr2, cnt + 1};
     tstart(c, XNB_TIMER_FORCE);
-    e = launch_force(c, p->precision, false, true, false, p->lj.epsilon, p->lj.sigma, p->rcut, dth, fuse ? &nx : nullptr, lean);
+    if (mm) e = launch_force_multimat(c, p->precision, *mm, ntypes, dth, fuse ? &nx : nullptr, lean);
+    else e = launch_force(c, p->precision, false, true, false, p->lj.epsilon, p->lj.sigma, p->rcut, dth, fuse ? &nx : nullptr, lean);
     tstop(c);
     if (e) return e;
     if (fuse) {
@@ -980,6 +1013,21 @@ int xnb_run(xnb_ctx* c, xnb_run_params* p) {
   }
   HIPCHK(c, hipStreamSynchronize(c->st));
   return XNB_OK;
+}
+
+}  // namespace xnb_impl
+
+extern "C" {
+
+int xnb_run(xnb_ctx* c, xnb_run_params* p) { return run_loop(c, p, nullptr, 0); }
+
+int xnb_run_multimat(xnb_ctx* c, xnb_run_params* p, int ntypes, const double* eps, const double* sigma, const double* rcut) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (ntypes <= 0 || ntypes > 4 || !eps || !sigma || !rcut) return fail(c, XNB_E_ARG, "bad multimat arguments (1..4 types)");
+  LJMulti mm;
+  if (make_lj_multi(ntypes, eps, sigma, rcut, mm)) return fail(c, XNB_E_ARG, "too many type pairs");
+  return run_loop(c, p, &mm, ntypes);
 }
 
 int xnb_timer_enable(xnb_ctx* c, int enable) {

@@ -62,6 +62,7 @@ __device__ inline size_t lpos(uint32_t q, uint32_t t, uint32_t T) { return ((siz
 struct BrickLds {
   uint32_t *rcnt, *rpref, *roff, *bpref;
   double *px, *py, *pz;
+  uint8_t* pt;  // particle type (r.w) of each region atom
 };
 
 __device__ inline BrickLds brick_lds(const BrickParams& bp, char* lds) {
@@ -73,6 +74,7 @@ __device__ inline BrickLds brick_lds(const BrickParams& bp, char* lds) {
   L.px = (double*)(lds + bp.off_pos);  // interleaved x, y, z: atom i at [3 i, 3 i + 2]
   L.py = L.px + 1;
   L.pz = L.px + 2;
+  L.pt = (uint8_t*)(lds + bp.off_type);
   return L;
 }
 
@@ -152,11 +154,13 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
       L.px[3 * idx] = p.x;
       L.py[3 * idx] = p.y;
       L.pz[3 * idx] = p.z;
+      L.pt[idx] = (uint8_t)p.w;
     }
     if (t == 0) {  // sentinel atom R: every distance to it exceeds any cutoff
       L.px[3 * R] = 1e30;
       L.py[3 * R] = 1e30;
       L.pz[3 * R] = 1e30;
+      L.pt[R] = 0;
     }
     __syncthreads();
   }
@@ -431,6 +435,195 @@ __global__ void __launch_bounds__(1024) k_lj_brick(Geom g, BrickParams bp, const
   lj_brick_body<MIXED, ACCUM, EPI, U>(g, bp, b, r, cell_off, cell_n, blist, bcnt, lj, A, dth, nx);
 }
 
+// ------------------------------------------------------------------ multimat force
+// compute_pair_multimat (compute_pair_multimat.h:100-253) on brick lists: a pair (a, b) is
+// accepted iff d2 <= rc(pid)^2, pid = unique_pair_id(type a, type b); the per-atom sums are
+// grouped by pair id, each group summed in stream order from 0 and added to f in increasing
+// pair id, a group without accepted pairs not added.  For a fixed type a, pid increases with
+// type b, so group t is the group of neighbours of type t.  Every lane updates all NT groups
+// per pair, the non-matching ones by +0.0 (sums start at +0.0 and stay out of -0.0, so this
+// is bitwise neutral).  ACCUM adds to the stored f (after zero_particle_force); otherwise f
+// starts from 0.0 like a zeroed force.  EPI as in k_lj_brick.
+template <bool MIXED, int NT, bool ACCUM, int EPI, int U>
+__device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams& bp, uint32_t b, const double4* __restrict__ r,
+                                                const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
+                                                const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt,
+                                                const LJMulti& mm, const PState& A, double dth, const NextStep& nx) {
+  extern __shared__ __align__(16) char lds[];
+  const BrickLds L = brick_lds(bp, lds);
+  const BrickPos P = brick_pos(g, bp, b);
+  if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, true)) return;
+  const LaneAtom At = lane_atom(bp, L);
+  const uint32_t t = threadIdx.x, T = bp.T;
+  const uint32_t n = At.live ? bcnt[(size_t)b * T + t] : 0u;
+  const uint32_t nmax = wave_max_b(n);
+  const double ax0 = L.px[3 * At.ia], ay0 = L.py[3 * At.ia], az0 = L.pz[3 * At.ia];
+  const unsigned ta = L.pt[At.ia];
+  // this lane's parameters per neighbour type
+  double sg[NT], e4[NT], m24[NT], rc2[NT];
+  float s2f[NT], e4f[NT], m24f[NT];
+#pragma unroll
+  for (int q = 0; q < NT; q++) {
+    const unsigned pid = unique_pair_id(ta, (unsigned)q);
+    sg[q] = mm.sigma[pid];
+    e4[q] = mm.eps4[pid];
+    m24[q] = mm.m24eps[pid];
+    rc2[q] = mm.rc2[pid];
+    s2f[q] = mm.sigma2f[pid];
+    e4f[q] = mm.eps4f[pid];
+    m24f[q] = mm.m24epsf[pid];
+  }
+  constexpr bool ENERGY = EPI != EPI_NEXT_LEAN;
+  double gx[NT], gy[NT], gz[NT], ge[NT];
+  bool any[NT];
+#pragma unroll
+  for (int q = 0; q < NT; q++) {
+    gx[q] = gy[q] = gz[q] = ge[q] = 0.0;
+    any[q] = false;
+  }
+  const uint2* lst = reinterpret_cast<const uint2*>(blist + (size_t)b * bp.cap * T) + t;
+  constexpr int W = U / 4;
+  uint2 wn[W];
+  if (nmax > 0) {
+#pragma unroll
+    for (int w = 0; w < W; w++) wn[w] = lst[(size_t)w * T];
+  }
+  for (uint32_t q0 = 0; q0 < nmax; q0 += U) {
+    uint2 wc[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) wc[w] = wn[w];
+    if (q0 + U < nmax) {
+      const uint2* nxt = lst + (size_t)((q0 + U) >> 2) * T;
+#pragma unroll
+      for (int w = 0; w < W; w++) wn[w] = nxt[(size_t)w * T];
+    }
+    uint32_t j[U];
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      j[4 * w + 0] = wc[w].x & 0xffffu;
+      j[4 * w + 1] = wc[w].x >> 16;
+      j[4 * w + 2] = wc[w].y & 0xffffu;
+      j[4 * w + 3] = wc[w].y >> 16;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const double dx = L.px[3 * j[u]] - ax0;
+      const double dy = L.py[3 * j[u]] - ay0;
+      const double dz = L.pz[3 * j[u]] - az0;
+      const unsigned tb = min((unsigned)L.pt[j[u]], (unsigned)(NT - 1));
+      double psg = sg[0], pe4 = e4[0], pm24 = m24[0], prc2 = rc2[0];
+      float ps2f = s2f[0], pe4f = e4f[0], pm24f = m24f[0];
+#pragma unroll
+      for (int q = 1; q < NT; q++)
+        if (tb == (unsigned)q) {
+          psg = sg[q];
+          pe4 = e4[q];
+          pm24 = m24[q];
+          prc2 = rc2[q];
+          ps2f = s2f[q];
+          pe4f = e4f[q];
+          pm24f = m24f[q];
+        }
+      double coef, ce;  // force coefficient (f += coef * d) and pair energy
+      bool in;
+      if (MIXED) {
+        const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        in = d2 <= prc2;
+        const float inv_r2 = in ? __builtin_amdgcn_rcpf((float)d2) : 0.0f;  // 0 outside: coefficient and energy 0
+        const float s2 = ps2f * inv_r2;
+        const float s6 = s2 * s2 * s2;
+        const float s12 = s6 * s6;
+        coef = (double)(pm24f * (2.0f * s12 - s6) * inv_r2);
+        ce = ENERGY ? (double)(pe4f * (s12 - s6)) : 0.0;
+      } else {
+        const double d2 = (dx * dx + dy * dy) + dz * dz;
+        in = d2 <= prc2;
+        const double inv_r = inv_sqrt_rounded(in ? d2 : 1.0);
+        const double ratio = psg * inv_r;
+        const double ratio2 = ratio * ratio;
+        const double ratio6 = ratio2 * ratio2 * ratio2;
+        const double ratio12 = ratio6 * ratio6;
+        double de = (pm24 * (2. * ratio12 - ratio6)) * inv_r;
+        de *= inv_r;
+        coef = in ? de : 0.0;
+        ce = (ENERGY && in) ? pe4 * (ratio12 - ratio6) : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < NT; q++) {
+        const bool mine = in && tb == (unsigned)q;
+        const double cq = mine ? coef : 0.0;
+        if (MIXED) {  // f = fma(c, d, f), the singlemat mixed form
+          gx[q] = fma(cq, dx, gx[q]);
+          gy[q] = fma(cq, dy, gy[q]);
+          gz[q] = fma(cq, dz, gz[q]);
+        } else {  // f += (de / r) * d, the plain form
+          gx[q] += cq * dx;
+          gy[q] += cq * dy;
+          gz[q] += cq * dz;
+        }
+        if (ENERGY) ge[q] += mine ? ce : 0.0;
+        any[q] = any[q] || mine;
+      }
+    }
+  }
+  constexpr bool NEXT = EPI == EPI_NEXT || EPI == EPI_NEXT_LEAN;
+  bool over = false;
+  if (At.live) {
+    const uint32_t s = At.slot;
+    double fx = ACCUM ? A.fx[s] : 0.0, fy = ACCUM ? A.fy[s] : 0.0, fz = ACCUM ? A.fz[s] : 0.0, fe = ACCUM ? A.ep[s] : 0.0;
+#pragma unroll
+    for (int q = 0; q < NT; q++)
+      if (any[q]) {
+        fx += gx[q];
+        fy += gy[q];
+        fz += gz[q];
+        fe += ge[q];
+      }
+    if (EPI != EPI_NEXT_LEAN) {
+      A.fx[s] = fx;
+      A.fy[s] = fy;
+      A.fz[s] = fz;
+      A.ep[s] = fe;
+    }
+    if (EPI != EPI_NONE) {
+      double ux = A.vx[s] + fx * dth;
+      double uy = A.vy[s] + fy * dth;
+      double uz = A.vz[s] + fz * dth;
+      if (NEXT) {
+        double4 p;
+        p.x = ax0;
+        p.y = ay0;
+        p.z = az0;
+        p.w = reinterpret_cast<const double*>(r)[4 * (size_t)s + 3];
+        p.x += ux * nx.dt + fx * nx.dt2;
+        p.y += uy * nx.dt + fy * nx.dt2;
+        p.z += uz * nx.dt + fz * nx.dt2;
+        nx.r_next[s] = p;
+        ux += fx * dth;
+        uy += fy * dth;
+        uz += fz * dth;
+        const int ri = (int)(At.rc % (uint32_t)bp.RX), rj = (int)((At.rc / (uint32_t)bp.RX) % (uint32_t)bp.RY),
+                  rk = (int)(At.rc / ((uint32_t)bp.RX * (uint32_t)bp.RY));
+        over = displaced_ijk(g, p, P.gi0 - 1 + ri, P.gj0 - 1 + rj, P.gk0 - 1 + rk, nx.backup, s, nx.thr2);
+      }
+      A.vx[s] = ux;
+      A.vy[s] = uy;
+      A.vz[s] = uz;
+    }
+  }
+  if (NEXT) wave_count_add(over, nx.count);
+}
+
+template <bool MIXED, int NT, bool ACCUM, int EPI>
+__global__ void __launch_bounds__(1024) k_ljmm_brick(Geom g, BrickParams bp, const double4* __restrict__ r,
+                                                     const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
+                                                     const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt, LJMulti mm,
+                                                     PState A, double dth, NextStep nx) {
+  const uint32_t b = brick_of_block(bp);
+  if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
+  ljmm_brick_body<MIXED, NT, ACCUM, EPI, LJB_UNROLL_B>(g, bp, b, r, cell_off, cell_n, blist, bcnt, mm, A, dth, nx);
+}
+
 // ------------------------------------------------------------------ conversion
 // brick list -> generic per-slot u32 list (slot indices), for stream export and the generic
 // traversals.
@@ -492,6 +685,8 @@ uint32_t brick_layout(BrickParams& bp) {
   o = align8(o + 4u * (bp.nbc + 1));
   bp.off_pos = o;
   o += 24u * (bp.Rmax + 1);  // region atoms + the sentinel
+  bp.off_type = o;
+  o = align8(o + bp.Rmax + 1);
   return align8(o);
 }
 
@@ -555,6 +750,45 @@ hipError_t launch_lj_brick(int precision, bool accum, int epi, const Geom& g, Br
   if (accum) return epi ? ljb_launch<false, true, EPI_KICK>(XNB_LJB_ARGS) : ljb_launch<false, true, EPI_NONE>(XNB_LJB_ARGS);
   return epi ? ljb_launch<false, false, EPI_KICK>(XNB_LJB_ARGS) : ljb_launch<false, false, EPI_NONE>(XNB_LJB_ARGS);
 #undef XNB_LJB_ARGS
+}
+
+template <bool MIXED, int NT, bool ACCUM, int EPI>
+static hipError_t ljmm_launch(const Geom& g, const BrickParams& bp, uint32_t lds, const uint32_t* cell_off, const uint32_t* cell_n,
+                              const uint16_t* blist, const uint32_t* bcnt, const LJMulti& mm, const PState& A, double dth,
+                              const NextStep& nx, hipStream_t st) {
+  hipError_t e = set_lds((const void*)k_ljmm_brick<MIXED, NT, ACCUM, EPI>, lds);
+  if (e != hipSuccess) return e;
+  k_ljmm_brick<MIXED, NT, ACCUM, EPI><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, A.r, cell_off, cell_n, blist, bcnt, mm, A, dth, nx);
+  return hipGetLastError();
+}
+
+template <bool MIXED, int NT>
+static hipError_t ljmm_dispatch(int epi, const Geom& g, const BrickParams& bp, uint32_t lds, const uint32_t* cell_off,
+                                const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const LJMulti& mm, const PState& A,
+                                double dth, const NextStep& nx, hipStream_t st) {
+  switch (epi) {
+    case EPI_NONE: return ljmm_launch<MIXED, NT, true, EPI_NONE>(g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
+    case EPI_KICK: return ljmm_launch<MIXED, NT, false, EPI_KICK>(g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
+    case EPI_NEXT: return ljmm_launch<MIXED, NT, false, EPI_NEXT>(g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
+    default: return ljmm_launch<MIXED, NT, false, EPI_NEXT_LEAN>(g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
+  }
+}
+
+// Multimat force on brick lists.  epi EPI_NONE accumulates into f (after zero_particle_force);
+// the other epilogues start f from 0, as the run loop's zero + force.  ntypes 1..4.
+hipError_t launch_ljmm_brick(int precision, int ntypes, int epi, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
+                             const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const LJMulti& mm, double dth,
+                             const NextStep* next, hipStream_t st) {
+  if (ntypes < 1 || ntypes > 4) return hipErrorInvalidValue;
+  if ((epi == EPI_NEXT || epi == EPI_NEXT_LEAN) && !next) return hipErrorInvalidValue;
+  const uint32_t lds = brick_layout(bp);
+  const NextStep nx = next ? *next : NextStep{};
+  if (precision) {
+    if (ntypes <= 2) return ljmm_dispatch<true, 2>(epi, g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
+    return ljmm_dispatch<true, 4>(epi, g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
+  }
+  if (ntypes <= 2) return ljmm_dispatch<false, 2>(epi, g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
+  return ljmm_dispatch<false, 4>(epi, g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
 }
 
 hipError_t launch_brick_to_slots(const Geom& g, BrickParams bp, const uint32_t* cell_off, const uint32_t* cell_n, const uint16_t* blist,

@@ -29,7 +29,7 @@ struct BrickParams {
   uint32_t Rmax;  // largest region atom count
   uint32_t cap;   // list rows per brick slab
   uint32_t* status;  // device word: bit 0 set when a region exceeds Rmax, bit 1 when a brick exceeds T
-  uint32_t off_rcnt, off_rpref, off_roff, off_bpref, off_pos;  // LDS byte offsets
+  uint32_t off_rcnt, off_rpref, off_roff, off_bpref, off_pos, off_type;  // LDS byte offsets
 };
 
 // Particle arrays of one storage buffer (slots [0, N) inner, [N, N+G) ghosts).
@@ -137,7 +137,7 @@ uint32_t lj_list_granule();
 void launch_lj(int precision, bool accum, bool push, uint32_t n_atoms, const PState& A, const uint32_t* nbr,
                const uint32_t* nbr_cnt, uint32_t cap, double eps, double sigma, double rcut, double dth, hipStream_t st);
 int launch_lj_multimat(int precision, uint32_t n_atoms, const PState& A, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap,
-                       int ntypes, const double* eps, const double* sigma, const double* rcut, hipStream_t st);
+                       int ntypes, const LJMulti& m, hipStream_t st);
 void launch_pair_counts(uint32_t n_atoms, const double4* r, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap, double rc2,
                         unsigned long long* out, hipStream_t st);
 void launch_stream_plen(uint32_t n_atoms, const uint32_t* slot_cell, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap,
@@ -159,6 +159,9 @@ hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, con
 hipError_t launch_lj_brick(int precision, bool accum, int epi, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
                            const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, double eps, double sigma, double rcut,
                            double dth, const NextStep* next, hipStream_t st);
+hipError_t launch_ljmm_brick(int precision, int ntypes, int epi, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
+                             const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const LJMulti& mm, double dth,
+                             const NextStep* next, hipStream_t st);
 hipError_t launch_brick_to_slots(const Geom& g, BrickParams bp, const uint32_t* cell_off, const uint32_t* cell_n, const uint16_t* blist,
                                  const uint32_t* bcnt, uint32_t* nbr, uint32_t* nbr_cnt, uint32_t cap, hipStream_t st);
 void launch_energies(uint32_t N, const PState& A, double* part, uint32_t nparts, double* out, hipStream_t st);

@@ -596,22 +596,6 @@ __global__ void __launch_bounds__(256) k_lj(uint32_t n_atoms, const double4* __r
 // Multi-material LJ (compute_pair_multimat.h:100-253): accepted iff d2 <= rc(pair)^2, with
 // the per-atom sums grouped by pair id (each group summed in stream order from 0, then added
 // to f in increasing pair id).  At most MM_PAIRS pair ids (up to 4 types).
-static constexpr int MM_PAIRS = 10;
-struct LJMulti {
-  int npairs;
-  double sigma[MM_PAIRS], eps4[MM_PAIRS], m24eps[MM_PAIRS], rc2[MM_PAIRS];
-  float sigma2f[MM_PAIRS], eps4f[MM_PAIRS], m24epsf[MM_PAIRS];
-};
-
-__device__ inline unsigned unique_pair_id(unsigned a, unsigned b) {
-  if (a > b) {
-    unsigned t = a;
-    a = b;
-    b = t;
-  }
-  return (b * (b + 1)) / 2 + a;
-}
-
 template <bool MIXED>
 __global__ void __launch_bounds__(256) k_lj_multimat(uint32_t n_atoms, const double4* __restrict__ r, const uint32_t* __restrict__ nbr,
                                                      const uint32_t* __restrict__ nbr_cnt, uint32_t cap, LJMulti lj,
@@ -990,20 +974,8 @@ void launch_lj(int precision, bool accum, bool push, uint32_t n_atoms, const PSt
 }
 
 int launch_lj_multimat(int precision, uint32_t n_atoms, const PState& A, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap,
-                       int ntypes, const double* eps, const double* sigma, const double* rcut, hipStream_t st) {
-  LJMulti m;
-  const int np = ntypes * (ntypes + 1) / 2;
-  if (np > MM_PAIRS) return -1;
-  m.npairs = np;
-  for (int q = 0; q < np; q++) {
-    m.sigma[q] = sigma[q];
-    m.eps4[q] = 4. * eps[q];
-    m.m24eps[q] = -24. * eps[q];
-    m.rc2[q] = rcut[q] * rcut[q];
-    m.sigma2f[q] = (float)(sigma[q] * sigma[q]);
-    m.eps4f[q] = (float)(4. * eps[q]);
-    m.m24epsf[q] = (float)(-24. * eps[q]);
-  }
+                       int ntypes, const LJMulti& m, hipStream_t st) {
+  if (ntypes < 1 || m.npairs != ntypes * (ntypes + 1) / 2) return -1;
   const uint32_t nb = nblk(n_atoms, 256);
   if (precision == 0) k_lj_multimat<false><<<nb, 256, 0, st>>>(n_atoms, A.r, nbr, nbr_cnt, cap, m, A.fx, A.fy, A.fz, A.ep);
   else k_lj_multimat<true><<<nb, 256, 0, st>>>(n_atoms, A.r, nbr, nbr_cnt, cap, m, A.fx, A.fy, A.fz, A.ep);

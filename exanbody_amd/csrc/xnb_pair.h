@@ -133,6 +133,40 @@ __device__ inline void lj_pair_mixed(const LJConst& lj, const double4& ra, const
   if (ENERGY) te += (double)(lj.eps4f * (s12 - s6));
 }
 
+// Multi-material LJ parameters per unique pair id (particle_type_pair.h: id of types a <= b
+// is b (b + 1) / 2 + a); up to 4 types.
+static constexpr int MM_PAIRS = 10;
+struct LJMulti {
+  int npairs;
+  double sigma[MM_PAIRS], eps4[MM_PAIRS], m24eps[MM_PAIRS], rc2[MM_PAIRS];
+  float sigma2f[MM_PAIRS], eps4f[MM_PAIRS], m24epsf[MM_PAIRS];
+};
+
+__host__ __device__ inline unsigned unique_pair_id(unsigned a, unsigned b) {
+  if (a > b) {
+    unsigned t = a;
+    a = b;
+    b = t;
+  }
+  return (b * (b + 1)) / 2 + a;
+}
+
+inline int make_lj_multi(int ntypes, const double* eps, const double* sigma, const double* rcut, LJMulti& m) {
+  const int np = ntypes * (ntypes + 1) / 2;
+  if (ntypes < 1 || np > MM_PAIRS) return -1;
+  m.npairs = np;
+  for (int q = 0; q < np; q++) {
+    m.sigma[q] = sigma[q];
+    m.eps4[q] = 4. * eps[q];
+    m.m24eps[q] = -24. * eps[q];
+    m.rc2[q] = rcut[q] * rcut[q];
+    m.sigma2f[q] = (float)(sigma[q] * sigma[q]);
+    m.eps4f[q] = (float)(4. * eps[q]);
+    m.m24epsf[q] = (float)(-24. * eps[q]);
+  }
+  return 0;
+}
+
 inline LJConst make_lj_const(double eps, double sigma, double rcut) {
   LJConst c;
   c.sigma = sigma;

@@ -160,6 +160,11 @@ typedef struct {
 /* numerical_scheme loop (data/config/numerical-scheme.msp:8-25, update-particles.msp:42-79):
  * push_f_v_r, push_f_v, trigger, full update or ghost_update_r, forces, push_f_v. */
 int xnb_run(xnb_ctx* ctx, xnb_run_params* params);
+/* xnb_run with compute_pair_multimat LJ forces (compute_pair_multimat.h:256-316; parameters
+ * per unique type pair as for xnb_lennard_jones_force_multimat; params->lj and ->rcut are not
+ * used; nbh_dist must cover the largest pair cutoff). */
+int xnb_run_multimat(xnb_ctx* ctx, xnb_run_params* params, int ntypes, const double* eps_pair, const double* sigma_pair,
+                     const double* rcut_pair);
 
 /* ---- measurement ----------------------------------------------------------- */
 #define XNB_TIMER_FORCE 0
