@@ -1,10 +1,14 @@
 """Benchmark: LJ atom-steps/s (rc=2.5 sigma), mixed vs FP64 on one MI355X.
 
-Workload: FCC 128^3 unit cells (8,388,608 atoms), rho*=0.8442, T*=0.72 seeded
+Headline workload: FCC 128^3 unit cells (8,388,608 atoms), rho*=0.8442, T*=0.72 seeded
 Maxwell-Boltzmann velocities, +-0.01 sigma jitter, rc=2.5, skin=0.3, dt=0.005, neighbour
 rebuild every 10 steps, velocity-Verlet.  Both precisions run the same device-resident
 step loop (push, trigger, ghost refresh or full rebuild, force + push).  value = mixed
 atom-steps/s; the FP64 plain form is timed beside it.
+
+Beside it, in the same JSON line: many boxes in one batch against one box at a time
+("batched"), the continuous-batching serving loop with p50/p99 job latency ("serving"), and
+the two-species slab on the multimat path ("two_species_slab").
 
 Prints ONE JSON line on stdout (the last line); progress goes to stderr.
 """
@@ -188,6 +192,52 @@ def serving_bench(n_jobs, rate, seed=9):
     return out
 
 
+def slab_bench(steps, dt=0.004, every=10):
+    """Two-species slab + droplets (4,194,304 atoms, A:B 80:20, sigma_BB 1.4, LB mixing,
+    rc = 2.5 sigma_ij), multimat pair path, FP64 and mixed, with the mixed force error."""
+    from exanbody_amd import Box, PRECISION_FP64, PRECISION_MIXED
+    from exanbody_amd._lib import TIMER_FORCE, TIMER_NBH
+    from exanbody_amd.workloads import lorentz_berthelot, slab_droplets
+    eps, sig, rc = lorentz_berthelot()
+    r, v, ids, types, dom = slab_droplets()
+    n = r.shape[0]
+    out = dict(workload=f"two-species slab + 64 droplets, {n} atoms, A:B 80:20, sigma_BB 1.4, LB mixing, rc 2.5 sigma_ij, "
+                        f"T* 0.8, dt {dt}, {steps} steps, rebuild every {every}, multimat")
+    fref = None
+    for name, prec in (("fp64", PRECISION_FP64), ("mixed", PRECISION_MIXED)):
+        b = Box(dom["bounds_min"], dom["bounds_max"], dom["grid_dims"], dom["cell_size"])
+        b.nbh_dist(float(rc.max()), SKIN)
+        b.set_particles(r, v, ids, types)
+        b.init(1.0, 1.0, float(rc.max()), SKIN, prec)
+        b.zero_particle_force()
+        b.lennard_jones_force_multimat(2, eps, sig, rc, precision=prec)
+        f0 = b.export()["f"]
+        if fref is None:
+            fref = f0
+        else:
+            d = np.linalg.norm(f0 - fref, axis=1)
+            rms = np.sqrt(np.mean(np.sum(fref * fref, axis=1)))
+            out["mixed_force_error_t0"] = dict(rms_rel=float(np.sqrt(np.mean(d * d)) / rms), max_rel=float(d.max() / rms))
+        ke0, pe0 = b.energies()
+        b.synchronize()
+        b.timer_reset()
+        b.timer_enable(True)
+        t0 = time.perf_counter()
+        res = b.run_multimat(steps, dt, 2, eps, sig, rc, SKIN, precision=prec, rebuild_every=every)
+        b.synchronize()
+        wall = time.perf_counter() - t0
+        b.timer_enable(False)
+        ke1, pe1 = b.energies()
+        fms, fn = b.timer_read(TIMER_FORCE)
+        nms, nn = b.timer_read(TIMER_NBH)
+        out[name] = dict(atom_steps_per_s=n * steps / wall, ms_per_step=1e3 * wall / steps, rebuilds=res["rebuilds"],
+                         guard_violations=res["violations"], force_avg_ms=fms / max(fn, 1), nbh_build_avg_ms=nms / max(nn, 1),
+                         energy_drift_per_atom_per_1000_steps=((ke1 + pe1) - (ke0 + pe0)) / n * 1000.0 / steps)
+        b.close()
+    out["mixed_over_fp64"] = out["mixed"]["atom_steps_per_s"] / out["fp64"]["atom_steps_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,6 +254,7 @@ def main():
     ap.add_argument("--batch-alone-sample", type=int, default=32)
     ap.add_argument("--serve-jobs", type=int, default=64, help="serving workload size (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--slab-steps", type=int, default=1000, help="two-species slab workload steps (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -257,6 +308,11 @@ def main():
         log("serving ...")
         serving = serving_bench(args.serve_jobs, args.serve_rate)
         log("serving", json.dumps(serving))
+    slab = None
+    if args.slab_steps > 0:
+        log("slab ...")
+        slab = slab_bench(args.slab_steps)
+        log("slab", json.dumps(slab))
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -274,7 +330,7 @@ def main():
         results=results, roofline=roofline,
         step_roofline=dict(b_alg_bytes_per_atom_step=b_alg, flops_per_atom_step=flops_per_atom,
                            ceiling_atom_steps_per_s=step_ceiling, frac=head["atom_steps_per_s"] / step_ceiling),
-        copy_probe_gbs=copy_gbs, batched=batched, serving=serving, cpu_baseline=cpu)
+        copy_probe_gbs=copy_gbs, batched=batched, serving=serving, two_species_slab=slab, cpu_baseline=cpu)
     print(json.dumps(line), flush=True)
 
 
