@@ -263,6 +263,8 @@ int xnb_batch_create(xnb_batch** out, int device) {
   ok = ok && hipHostMalloc((void**)&B->hwords, 4 * kMaxBatchBoxes * sizeof(uint32_t), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipEventCreate(&B->hev[0]) == hipSuccess && hipEventCreate(&B->hev[1]) == hipSuccess;
   if (ok) ok = hipMemset(B->dwords.p, 0, 4 * kMaxBatchBoxes * sizeof(uint32_t)) == hipSuccess;
+  // both staging events start recorded, so the first waits on them return at once
+  if (ok) ok = hipEventRecord(B->hev[0], B->st) == hipSuccess && hipEventRecord(B->hev[1], B->st) == hipSuccess;
   if (!ok) {
     delete B;
     return XNB_E_HIP;
