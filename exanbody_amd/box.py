@@ -200,6 +200,19 @@ class Box:
         self._check(self.L.xnb_run_multimat(self._h, C.byref(p), int(ntypes), _ptr(e), _ptr(s), _ptr(rc)), "run_multimat")
         return dict(rebuilds=p.n_rebuilds, violations=p.n_violations)
 
+    # ---- snapshots
+    def snapshot(self):
+        """Bytes of the box's full step-loop state (see xnb_snapshot_save)."""
+        n = C.c_size_t()
+        self._check(self.L.xnb_snapshot_size(self._h, C.byref(n)), "snapshot_size")
+        buf = np.zeros(n.value, np.uint8)
+        self._check(self.L.xnb_snapshot_save(self._h, _ptr(buf), buf.size), "snapshot_save")
+        return buf
+
+    def restore(self, snap):
+        snap = np.ascontiguousarray(snap, np.uint8)
+        self._check(self.L.xnb_snapshot_load(self._h, _ptr(snap), snap.size), "snapshot_load")
+
     # ---- measurement
     def timer_enable(self, on=True):
         self.L.xnb_timer_enable(self._h, int(on))

@@ -666,6 +666,142 @@ int xnb_set_particles(xnb_ctx* c, size_t n, const double* r, const double* v, co
   return XNB_OK;
 }
 
+// ---- snapshots: the inner particles in slot order with everything the step loop carries
+// from one step to the next (r with type, v, f, ep, id, per-cell counts, backup_r), so a box
+// restored from one continues bitwise as if it had not stopped.
+namespace {
+struct SnapHeader {
+  char magic[8];  // "XNBSNAP1"
+  uint32_t version, n, ncells, pad;
+  double nbh_dist_lab, max_displ, ghost_dist;
+  xnb_domain domain;
+};
+constexpr uint32_t kSnapVersion = 1;
+
+bool same_domain(const xnb_domain& a, const xnb_domain& b) {
+  for (int q = 0; q < 3; q++)
+    if (a.bounds_min[q] != b.bounds_min[q] || a.bounds_max[q] != b.bounds_max[q] || a.grid_dims[q] != b.grid_dims[q] ||
+        (a.periodic[q] != 0) != (b.periodic[q] != 0))
+      return false;
+  return a.cell_size == b.cell_size;
+}
+
+size_t snap_bytes(uint32_t n, uint32_t ncells) {
+  return sizeof(SnapHeader) + (size_t)ncells * 4 + (size_t)n * (32 + 24 + 24 + 8 + 8 + 12);
+}
+}  // namespace
+
+int xnb_snapshot_size(xnb_ctx* c, size_t* bytes) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (!bytes) return fail(c, XNB_E_ARG, "null argument");
+  *bytes = snap_bytes(c->N, c->g.ncells);
+  return XNB_OK;
+}
+
+int xnb_snapshot_save(xnb_ctx* c, void* buf, size_t bytes) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (!buf || bytes < snap_bytes(c->N, c->g.ncells)) return fail(c, XNB_E_CAPACITY, "snapshot buffer too small");
+  if (!c->backup_valid || !c->nbh_set) return fail(c, XNB_E_STATE, "snapshot needs an initialised box (xnb_init)");
+  const uint32_t n = c->N, nc = c->g.ncells;
+  SnapHeader h;
+  std::memset(&h, 0, sizeof(h));
+  std::memcpy(h.magic, "XNBSNAP1", 8);
+  h.version = kSnapVersion;
+  h.n = n;
+  h.ncells = nc;
+  h.nbh_dist_lab = c->nbh_dist_lab;
+  h.max_displ = c->max_displ;
+  h.ghost_dist = c->ghost_dist;
+  h.domain = c->dom;
+  char* o = (char*)buf;
+  std::memcpy(o, &h, sizeof(h));
+  o += sizeof(h);
+  std::vector<uint32_t> cn(nc);
+  HIPCHK(c, hipMemcpyAsync(cn.data(), c->cell_n.p, nc * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (uint32_t q = 0; q < nc; q++) {  // ghost cells are rebuilt on load
+    int i, j, k;
+    cell_ijk(c->g, q, i, j, k);
+    if (is_ghost(c->g, i, j, k)) cn[q] = 0;
+  }
+  std::memcpy(o, cn.data(), nc * sizeof(uint32_t));
+  o += (size_t)nc * 4;
+  auto d2h = [&](const void* src, size_t sz) -> int {
+    HIPCHK(c, hipMemcpyAsync(o, src, sz, hipMemcpyDeviceToHost, c->st));
+    o += sz;
+    return XNB_OK;
+  };
+  if ((e = d2h(c->A.r, (size_t)n * 32)) || (e = d2h(c->A.vx, (size_t)n * 8)) || (e = d2h(c->A.vy, (size_t)n * 8)) ||
+      (e = d2h(c->A.vz, (size_t)n * 8)) || (e = d2h(c->A.fx, (size_t)n * 8)) || (e = d2h(c->A.fy, (size_t)n * 8)) ||
+      (e = d2h(c->A.fz, (size_t)n * 8)) || (e = d2h(c->A.ep, (size_t)n * 8)) || (e = d2h(c->A.id, (size_t)n * 8)) ||
+      (e = d2h(c->backup.p, (size_t)n * 12)))
+    return e;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
+
+// Load into a context created (xnb_create) with the snapshot's domain; replaces its particles,
+// rebuilds ghosts and neighbour lists from the restored slots (no re-binning).
+int xnb_snapshot_load(xnb_ctx* c, const void* buf, size_t bytes) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (!buf || bytes < sizeof(SnapHeader)) return fail(c, XNB_E_ARG, "bad snapshot buffer");
+  SnapHeader h;
+  std::memcpy(&h, buf, sizeof(h));
+  if (std::memcmp(h.magic, "XNBSNAP1", 8) != 0 || h.version != kSnapVersion) return fail(c, XNB_E_ARG, "not a snapshot");
+  if (!same_domain(h.domain, c->dom)) return fail(c, XNB_E_DOMAIN, "snapshot of another domain");
+  if (bytes < snap_bytes(h.n, h.ncells)) return fail(c, XNB_E_ARG, "truncated snapshot");
+  if ((e = set_geometry(c, h.ghost_dist))) return e;
+  if (h.ncells != c->g.ncells) return fail(c, XNB_E_DOMAIN, "snapshot grid differs");
+  c->nbh_set = true;
+  c->nbh_dist_lab = h.nbh_dist_lab;
+  c->max_displ = h.max_displ;
+  c->ghost_dist = h.ghost_dist;
+  const uint32_t n = h.n, nc = h.ncells;
+  const char* o = (const char*)buf + sizeof(h);
+  std::vector<uint32_t> cn(nc), off(nc + 1), sc(n);
+  std::memcpy(cn.data(), o, nc * sizeof(uint32_t));
+  o += (size_t)nc * 4;
+  uint32_t acc = 0;
+  for (uint32_t q = 0; q < nc; q++) {
+    off[q] = acc;
+    acc += cn[q];  // ghost cells hold 0
+    for (uint32_t p = 0; p < cn[q] && off[q] + p < n; p++) sc[off[q] + p] = q;
+  }
+  if (acc != n) return fail(c, XNB_E_ARG, "snapshot cell counts do not match its particle count");
+  for (uint32_t q = 0; q < nc; q++) {  // ghost cells: empty, after the inner slots
+    int i, j, k;
+    cell_ijk(c->g, q, i, j, k);
+    if (is_ghost(c->g, i, j, k)) off[q] = n;
+  }
+  c->N = n;
+  c->G = 0;
+  if ((e = ensure_capacity(c, (size_t)n + std::max<size_t>(n / 2, 4096), 0))) return e;
+  auto h2d = [&](void* dst, size_t sz) -> int {
+    HIPCHK(c, hipMemcpyAsync(dst, o, sz, hipMemcpyHostToDevice, c->st));
+    o += sz;
+    return XNB_OK;
+  };
+  if ((e = h2d(c->A.r, (size_t)n * 32)) || (e = h2d(c->A.vx, (size_t)n * 8)) || (e = h2d(c->A.vy, (size_t)n * 8)) ||
+      (e = h2d(c->A.vz, (size_t)n * 8)) || (e = h2d(c->A.fx, (size_t)n * 8)) || (e = h2d(c->A.fy, (size_t)n * 8)) ||
+      (e = h2d(c->A.fz, (size_t)n * 8)) || (e = h2d(c->A.ep, (size_t)n * 8)) || (e = h2d(c->A.id, (size_t)n * 8)) ||
+      (e = h2d(c->backup.p, (size_t)n * 12)))
+    return e;
+  HIPCHK(c, hipMemcpyAsync(c->slot_cell_A.p, sc.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->cell_n.p, cn.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->cell_off.p, off.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->backup_valid = true;
+  c->lists_valid = false;
+  if ((e = op_ghost_all(c))) return e;
+  xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
+  if ((e = op_nbh(c, c->nbh_dist_lab, cfg, 0))) return e;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
+
 int xnb_num_particles(const xnb_ctx* c, size_t* n_inner, size_t* n_ghost) {
   if (!c) return XNB_E_ARG;
   if (n_inner) *n_inner = c->N;

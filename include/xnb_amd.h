@@ -190,6 +190,16 @@ int xnb_copy_probe(xnb_ctx* ctx, size_t bytes, int iters, double* gbps);
  * [2^-767, 2^1000], recip_out[i] for x[i] in [2^-383, 2^383]. */
 int xnb_fp64_core(xnb_ctx* ctx, const double* x, size_t n, double* sqrt_out, double* recip_out);
 
+/* ---- snapshots ------------------------------------------------------------------- */
+/* Device-resident box -> host bytes and back, for preemption and restart (the fields of
+ * write_dump / read_dump, write_dump.cpp:38,43: positions, velocities, ids, types), plus what
+ * the step loop carries between steps (forces, per-atom energies, slot order, backup_r), so a
+ * box restored into a fresh context (xnb_create with the same domain) continues bitwise.
+ * Little-endian layout: header, per-cell counts, then r (x, y, z, type), v, f, ep, id, backup. */
+int xnb_snapshot_size(xnb_ctx* ctx, size_t* bytes);
+int xnb_snapshot_save(xnb_ctx* ctx, void* buf, size_t bytes);
+int xnb_snapshot_load(xnb_ctx* ctx, const void* buf, size_t bytes);
+
 /* ---- batches of boxes ---------------------------------------------------------- */
 /* No counterpart in the reference, which advances one domain per process: many independent
  * periodic boxes advanced together on one GPU, each kernel of a step being one launch over

@@ -196,3 +196,26 @@ def test_fp64_sqrt_recip_sequences_are_ieee():
     _, rc = b.fp64_core(y)
     np.testing.assert_array_equal(rc.view(np.uint64), (1.0 / y).view(np.uint64))
     b.close()
+
+
+@pytest.mark.parametrize("precision", [PRECISION_FP64, PRECISION_MIXED])
+def test_snapshot_restart_is_bitwise(precision):
+    """run(12); snapshot; restore into a fresh box; run(12) == run(24) without interruption
+    (rebuild every 6: the restart falls on a rebuild step boundary, so the schedule is kept)."""
+    from exanbody_amd import Box
+    r, v, ids, L = fcc_inputs(7, temperature=1.0, jitter=0.05, seed=41)
+    _, a, dom = make_pair(r, v, ids, L)
+    _, b, _ = make_pair(r, v, ids, L)
+    for x in (a, b):
+        x.init(1.0, 1.0, RCUT, SKIN, precision)
+    a.run(24, 0.005, 1.0, 1.0, RCUT, SKIN, precision=precision, rebuild_every=6)
+    b.run(12, 0.005, 1.0, 1.0, RCUT, SKIN, precision=precision, rebuild_every=6)
+    snap = b.snapshot()
+    b.close()
+    c = Box(dom["bounds_min"], dom["bounds_max"], dom["grid_dims"], dom["cell_size"])
+    c.restore(snap)
+    c.run(12, 0.005, 1.0, 1.0, RCUT, SKIN, precision=precision, rebuild_every=6)
+    ea, ec = a.export(), c.export()
+    assert_same_layout(ea, ec, "restart")
+    for k in ("v", "f", "ep"):
+        np.testing.assert_array_equal(ea[k].view(np.uint64), ec[k].view(np.uint64), err_msg=k)
