@@ -40,6 +40,11 @@ class Batch:
         self._check(self.L.xnb_batch_box_status(self._h, int(slot), C.byref(done), C.byref(state)), "status")
         return done.value, state.value
 
+    def recoveries(self, slot):
+        n = C.c_int64()
+        self._check(self.L.xnb_batch_box_recoveries(self._h, int(slot), C.byref(n)), "recoveries")
+        return n.value
+
     def detach(self, slot):
         """Detach a box; returns dict(rebuilds, violations) like Box.run."""
         p = _lib.RunParams()

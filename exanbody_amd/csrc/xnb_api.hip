@@ -154,9 +154,19 @@ int ensure_capacity(xnb_ctx* c, size_t ncap, size_t keep) {
   if ((e = dalloc(c, c->eB, newcap))) return e;
   if ((e = dalloc(c, c->idB, newcap))) return e;
   if ((e = dalloc(c, c->slot_cell_B, newcap))) return e;
-  for (DArr<uint32_t>* a : {&c->mv_dst, &c->pk, &c->newloc, &c->in_list, &c->plen, &c->pstart, &c->ghost_src, &c->ghost_shift}) {
+  for (DArr<uint32_t>* a : {&c->mv_dst, &c->pk, &c->newloc, &c->in_list, &c->plen, &c->pstart}) {
     dfree(*a);
     if ((e = dalloc(c, *a, newcap))) return e;
+  }
+  // the ghost scheme (one entry per ghost) stays valid across a growth: ghost_update_r reads it
+  for (DArr<uint32_t>* a : {&c->ghost_src, &c->ghost_shift}) {
+    DArr<uint32_t> na;
+    if ((e = dalloc(c, na, newcap))) return e;
+    const size_t ng = std::min<size_t>(c->G, a->n);
+    if (a->p && ng) HIPCHK(c, hipMemcpyAsync(na.p, a->p, ng * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    dfree(*a);
+    *a = na;
   }
   {  // backup_r of the kept (inner) slots survives: a ghost rebuild may grow the buffers after it
     DArr<uint32_t> nb;

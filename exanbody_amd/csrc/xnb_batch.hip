@@ -37,7 +37,7 @@ struct BatchBox {
   int every = 10;
   bool fresh = true;      // the next step's first half still has to be launched
   uint32_t last_par = 0;  // parity of the box's last step
-  int64_t n_rebuilds = 0, n_violations = 0;
+  int64_t n_rebuilds = 0, n_violations = 0, n_recover = 0;
   std::vector<char> rebuilt;
   DArr<unsigned long long> counts;
   xnb_run_params p{};
@@ -416,6 +416,7 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
           continue;
         }
         if (w[0] || w[1] > bb.f.bp.cap) {
+          bb.n_recover++;
           if ((e = recover_box(B, i, bb.p))) {
             bb.failed = true;
             bb.err = std::string("rebuild recovery failed: ") + bb.c->err;
@@ -485,6 +486,14 @@ int xnb_batch_box_status(xnb_batch* B, int slot, int64_t* done, int* state) {
   if (done) *done = bb.done;
   if (state) *state = bb.failed ? -1 : (bb.finished ? 1 : 0);
   if (bb.failed) B->err = bb.err;
+  return XNB_OK;
+}
+
+// Rebuilds of the box in a slot that outgrew its frame and were redone with the one-box
+// operators.
+int xnb_batch_box_recoveries(xnb_batch* B, int slot, int64_t* n) {
+  if (!B || slot < 0 || slot >= (int)kMaxBatchBoxes || !B->box[slot].used || !n) return XNB_E_ARG;
+  *n = B->box[slot].n_recover;
   return XNB_OK;
 }
 

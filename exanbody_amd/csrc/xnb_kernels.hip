@@ -386,7 +386,9 @@ __global__ void k_ghost_fill(Geom g, uint32_t N, PState B, const uint32_t* __res
 __device__ inline void ghost_update_r_item(Geom g, uint32_t N, uint32_t G, double4* __restrict__ r, const uint32_t* __restrict__ ghost_src,
                                  const uint32_t* __restrict__ ghost_shift, uint32_t q) {
   const uint32_t code = ghost_shift[q];
-  const double4 p = r[ghost_src[q]];
+  const uint32_t src = ghost_src[q];
+  if (src >= N) return;  // a ghost's source is an inner slot; anything else is a stale scheme
+  const double4 p = r[src];
   double4 o;
   o.x = shift_coord(p.x, (int)(code & 3u) - 1, g.ox, g.ex);
   o.y = shift_coord(p.y, (int)((code >> 2) & 3u) - 1, g.oy, g.ey);

@@ -217,6 +217,9 @@ int xnb_batch_attach(xnb_batch* batch, xnb_ctx* box, const xnb_run_params* p, in
 int xnb_batch_step(xnb_batch* batch, int nsteps, int* n_active);
 /* Steps done and state (0 running, 1 finished, -1 failed) of the box in a slot. */
 int xnb_batch_box_status(xnb_batch* batch, int slot, int64_t* done, int* state);
+/* Rebuilds of the box that outgrew the sizes it was attached with and were redone by the
+ * one-box operators (which re-size its arrays). */
+int xnb_batch_box_recoveries(xnb_batch* batch, int slot, int64_t* n);
 /* Detach the box of a slot; its xnb_ctx then holds the state xnb_run would have produced
  * (for a finished box) and p->n_rebuilds / n_violations are filled as by xnb_run. */
 int xnb_batch_detach(xnb_batch* batch, int slot, xnb_run_params* p);
