@@ -310,7 +310,7 @@ int choose_bricks(xnb_ctx* c, uint32_t cap, BrickParams& out) {
     bp.Rmax = std::max<uint32_t>(r_max, 1u);
     bp.cap = cap;
     bp.status = c->u32s.p + 6;
-    if (brick_layout(bp) + kBrickStaticLds > c->lds_limit) continue;
+    if (brick_layout(bp) > c->lds_limit) continue;
     out = bp;
     return XNB_OK;
   }
@@ -327,12 +327,11 @@ int op_nbh_brick(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& 
     if ((e = choose_bricks(c, cap, bp))) return e;
     if ((e = dalloc(c, c->blist, (size_t)bp.nbricks * bp.cap * bp.T))) return e;
     if ((e = dalloc(c, c->bcnt, (size_t)bp.nbricks * bp.T))) return e;
-    if ((e = dalloc(c, c->bperm, (size_t)bp.nbricks * bp.T))) return e;
     HIPCHK(c, hipMemsetAsync(c->u32s.p + 2, 0, sizeof(uint32_t), c->st));
     HIPCHK(c, hipMemsetAsync(c->u32s.p + 6, 0, sizeof(uint32_t), c->st));
     tstart(c, XNB_TIMER_NBH);
     hipError_t he = launch_nbh_brick(c->g, bp, c->A.r, c->cell_off.p, c->cell_n.p, max2, cfg.half_symmetric, cfg.skip_ghosts,
-                                     c->blist.p, c->bcnt.p, c->u32s.p + 2, c->bperm.p, c->st);
+                                     c->blist.p, c->bcnt.p, c->u32s.p + 2, c->st);
     tstop(c);
     if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick neighbour build: ") + hipGetErrorString(he));
     uint32_t mx = 0, bad = 0;
@@ -414,8 +413,8 @@ int launch_force(xnb_ctx* c, int precision, bool accum, bool push, bool ghost, d
                  const NextStep* next = nullptr, bool lean = false) {
   if (c->list_kind == 1 && !ghost) {
     const int epi = next ? (lean ? EPI_NEXT_LEAN : EPI_NEXT) : (push ? EPI_KICK : EPI_NONE);
-    hipError_t he = launch_lj_brick(precision, accum, epi, c->g, c->bp, c->A, c->cell_off.p, c->cell_n.p, c->blist.p, c->bcnt.p,
-                                    c->bperm.p, eps, sigma, rcut, dth, next, c->st);
+    hipError_t he = launch_lj_brick(precision, accum, epi, c->g, c->bp, c->A, c->cell_off.p, c->cell_n.p, c->blist.p, c->bcnt.p, eps,
+                                    sigma, rcut, dth, next, c->st);
     if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick LJ kernel: ") + hipGetErrorString(he));
     return XNB_OK;
   }
@@ -432,8 +431,8 @@ int launch_force(xnb_ctx* c, int precision, bool accum, bool push, bool ghost, d
 int launch_force_multimat(xnb_ctx* c, int precision, const LJMulti& mm, int ntypes, double dth, const NextStep* next, bool lean) {
   if (c->list_kind == 1) {
     const int epi = next ? (lean ? EPI_NEXT_LEAN : EPI_NEXT) : EPI_KICK;
-    hipError_t he = launch_ljmm_brick(precision, ntypes, epi, c->g, c->bp, c->A, c->cell_off.p, c->cell_n.p, c->blist.p, c->bcnt.p,
-                                      c->bperm.p, mm, dth, next, c->st);
+    hipError_t he = launch_ljmm_brick(precision, ntypes, epi, c->g, c->bp, c->A, c->cell_off.p, c->cell_n.p, c->blist.p, c->bcnt.p, mm,
+                                      dth, next, c->st);
     if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick multimat kernel: ") + hipGetErrorString(he));
     return XNB_OK;
   }
@@ -559,7 +558,7 @@ int xnb_destroy(xnb_ctx* c) {
                             &c->clen, &c->coff, &c->mv_dst, &c->pk, &c->newloc, &c->in_list, &c->plen, &c->pstart, &c->ghost_src,
                             &c->ghost_shift, &c->backup, &c->nbr, &c->nbr_cnt, &c->bsum, &c->u32s})
     dfree(*a);
-  dfree(c->blist); dfree(c->bcnt); dfree(c->bperm);
+  dfree(c->blist); dfree(c->bcnt);
   dfree(c->stream_buf); dfree(c->u64s); dfree(c->step_counts); dfree(c->red_part); dfree(c->red_out);
   if (c->h_u32) hipHostFree(c->h_u32);
   if (c->h_u64) hipHostFree(c->h_u64);
@@ -1015,7 +1014,7 @@ int xnb_lennard_jones_force_multimat(xnb_ctx* c, int ntypes, const double* eps, 
   tstart(c, XNB_TIMER_FORCE);
   if (c->list_kind == 1) {
     hipError_t he = launch_ljmm_brick(precision, ntypes, EPI_NONE, c->g, c->bp, c->A, c->cell_off.p, c->cell_n.p, c->blist.p, c->bcnt.p,
-                                      c->bperm.p, mm, 0.0, nullptr, c->st);
+                                      mm, 0.0, nullptr, c->st);
     if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick multimat kernel: ") + hipGetErrorString(he));
   } else {
     if ((e = ensure_slot_lists(c))) return e;

@@ -120,7 +120,6 @@ void frame_from_ctx(xnb_batch* B, int i, const xnb_run_params& p) {
   f.backup = c->backup.p;
   f.blist = c->blist.p;
   f.bcnt = c->bcnt.p;
-  f.bperm = c->bperm.p;
   f.lj = make_lj_const(p.lj.epsilon, p.lj.sigma, p.rcut);
   f.dt = p.dt * 1.0;  // the same expressions as xnb_run
   f.dt2 = f.dt * f.dt * 0.5;
@@ -315,7 +314,7 @@ int xnb_batch_attach(xnb_batch* B, xnb_ctx* c, const xnb_run_params* p, int* slo
   xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
   if ((e = op_nbh(c, c->nbh_dist_lab, cfg, 0))) return e;
   if (c->list_kind != 1) return bfail(B, XNB_E_STATE, "box geometry is not on the brick path (cell edge < nbh_dist)");
-  if (brick_lds_bytes(c->bp) + kBrickStaticLds > B->lds_limit) return bfail(B, XNB_E_CAPACITY, "box bricks exceed LDS");
+  if (brick_lds_bytes(c->bp) > B->lds_limit) return bfail(B, XNB_E_CAPACITY, "box bricks exceed LDS");
   if (hipStreamSynchronize(c->st) != hipSuccess) return bfail(B, XNB_E_HIP, "box stream");
   BatchBox& bb = B->box[i];
   bb = BatchBox{};

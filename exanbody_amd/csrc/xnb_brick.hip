@@ -175,9 +175,9 @@ struct LaneAtom {
   uint32_t slot;  // global slot
 };
 
-// the atom of brick lane t (lanes are the brick's atoms in cell order)
-__device__ inline LaneAtom lane_atom(const BrickParams& bp, const BrickLds& L, uint32_t t) {
+__device__ inline LaneAtom lane_atom(const BrickParams& bp, const BrickLds& L) {
   LaneAtom A;
+  const uint32_t t = threadIdx.x;
   A.live = t < L.bpref[bp.nbc];
   if (!A.live) {
     A.rc = 0;
@@ -226,58 +226,16 @@ __global__ void k_brick_stats(Geom g, BrickParams bp, const uint32_t* __restrict
 // overwritten.  FILTER enables the half_symmetric / skip_ghosts filters.
 // Brick b of one box; the workgroup may be wider than bp.T (batched launches), lanes from
 // bp.T on hold no atom and touch no list.
-// Lane order for the force kernels: the brick's lanes sorted by list length, longest first
-// (counting sort in LDS; the order among equal lengths is arbitrary, which changes no result
-// since each atom is summed by one lane).  Waves of the force kernels then hold lanes of
-// similar trip counts.  Returns, for this lane, the longest list of the force wave it lands
-// in (the first lane of that wave in the sorted order).  Every thread must call it.
-__device__ inline uint32_t sort_lanes_by_count(uint32_t cnt, uint32_t T, uint16_t* __restrict__ perm) {
-  constexpr uint32_t NB = 1024;
-  __shared__ uint32_t hist[NB];
-  __shared__ uint32_t scnt[1024];  // count at each sorted position
-  const uint32_t t = threadIdx.x;
-  for (uint32_t q = t; q < NB; q += blockDim.x) hist[q] = 0;
-  __syncthreads();
-  const uint32_t key = NB - 1 - min(cnt, NB - 1);
-  uint32_t rank = 0;
-  if (t < T) rank = atomicAdd(&hist[key], 1u);
-  __syncthreads();
-  if (t < 64) {  // exclusive scan of the bins by the first wave
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < NB; base += 64) {
-      const uint32_t v = hist[base + t];
-      uint32_t x = v;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 64);
-        if ((int)t >= o) x += y;
-      }
-      hist[base + t] = carry + x - v;
-      carry += __shfl(x, 63, 64);
-    }
-  }
-  __syncthreads();
-  uint32_t pos = 0;
-  if (t < T) {
-    pos = hist[key] + rank;
-    perm[pos] = (uint16_t)t;
-    scnt[pos] = cnt;
-  }
-  __syncthreads();
-  return t < T ? scnt[pos & ~63u] : 0u;
-}
-
 template <bool FILTER>
 __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams& bp, uint32_t b, const double4* __restrict__ r,
                                                const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
                                                double max2, int half_symmetric, int skip_ghosts, uint16_t* __restrict__ blist,
-                                               uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt,
-                                               uint16_t* __restrict__ bperm) {
+                                               uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt) {
   extern __shared__ __align__(16) char lds[];
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
   if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, true)) return;
-  const LaneAtom A = lane_atom(bp, L, threadIdx.x);
+  const LaneAtom A = lane_atom(bp, L);
   const uint32_t t = threadIdx.x, T = bp.T;
   uint16_t* out = blist + (size_t)b * bp.cap * T;
   // every lane, live or not, tests against its own LDS atom when past the candidate cell's
@@ -341,24 +299,21 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
   if (t < T) bcnt[(size_t)b * T + t] = cnt;
   const uint32_t wmax = wave_max_b(cnt);
   if ((t & 63) == 0 && wmax > 0) atomicMax(max_cnt, wmax);
-  // pad each list with the sentinel up to the batch boundary above the longest list of its
-  // force wave, so the force kernels need no per-lane bound inside a batch
-  const uint32_t head = sort_lanes_by_count(cnt, T, bperm + (size_t)b * T);
-  const uint32_t pad_end = min((head + kListBatch - 1) / kListBatch * kListBatch, bp.cap);
+  // pad to the wave's batch boundary with the sentinel, so force kernels need no per-lane
+  // bound inside a batch
+  const uint32_t pad_end = min((wmax + kListBatch - 1) / kListBatch * kListBatch, bp.cap);
   const uint16_t sentinel = (uint16_t)L.rpref[bp.nrc];
-  if (t < T)
-    for (uint32_t q = min(cnt, bp.cap); q < pad_end; q++) out[lpos(q, t, T)] = sentinel;
+  for (uint32_t q = min(cnt, bp.cap); q < pad_end; q++) out[lpos(q, t, T)] = sentinel;
 }
 
 template <bool FILTER>
 __global__ void __launch_bounds__(1024) k_nbh_brick_mask(Geom g, BrickParams bp, const double4* __restrict__ r,
                                                          const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
                                                          double max2, int half_symmetric, int skip_ghosts, uint16_t* __restrict__ blist,
-                                                         uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt,
-                                                         uint16_t* __restrict__ bperm) {
+                                                         uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt) {
   const uint32_t b = brick_of_block(bp);
   if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
-  nbh_brick_body<FILTER>(g, bp, b, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist, bcnt, max_cnt, bperm);
+  nbh_brick_body<FILTER>(g, bp, b, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist, bcnt, max_cnt);
 }
 
 // ------------------------------------------------------------------ force
@@ -377,16 +332,15 @@ __global__ void __launch_bounds__(1024) k_nbh_brick_mask(Geom g, BrickParams bp,
 template <bool MIXED, bool ACCUM, int EPI, int U>
 __device__ __forceinline__ void lj_brick_body(const Geom& g, const BrickParams& bp, uint32_t b, const double4* __restrict__ r,
                                               const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
-                                              const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt, const uint16_t* __restrict__ bperm,
+                                              const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt,
                                               const LJConst& lj, const PState& A, double dth, const NextStep& nx) {
   extern __shared__ __align__(16) char lds[];
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
   if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, true)) return;
+  const LaneAtom At = lane_atom(bp, L);
   const uint32_t t = threadIdx.x, T = bp.T;
-  const uint32_t lane = t < T ? (uint32_t)bperm[(size_t)b * T + t] : t;  // lanes sorted by list length
-  const LaneAtom At = lane_atom(bp, L, lane);
-  const uint32_t n = At.live ? bcnt[(size_t)b * T + lane] : 0u;
+  const uint32_t n = At.live ? bcnt[(size_t)b * T + t] : 0u;
   const uint32_t nmax = wave_max_b(n);
   double4 ra;
   ra.x = L.px[3 * At.ia];
@@ -398,7 +352,7 @@ __device__ __forceinline__ void lj_brick_body(const Geom& g, const BrickParams& 
   // build padded every list of the wave with the sentinel up to a multiple of U.
   static_assert(U % 4 == 0 && U == kListBatch, "batches are whole groups of four entries, as padded");
   constexpr int W = U / 4;
-  const uint2* lst = reinterpret_cast<const uint2*>(blist + (size_t)b * bp.cap * T) + lane;
+  const uint2* lst = reinterpret_cast<const uint2*>(blist + (size_t)b * bp.cap * T) + t;
   double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
   uint2 wn[W];
   if (nmax > 0) {
@@ -475,10 +429,10 @@ template <bool MIXED, bool ACCUM, int EPI, int U>
 __global__ void __launch_bounds__(1024) k_lj_brick(Geom g, BrickParams bp, const double4* __restrict__ r,
                                                    const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
                                                    const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt,
-                                                   const uint16_t* __restrict__ bperm, LJConst lj, PState A, double dth, NextStep nx) {
+                                                   LJConst lj, PState A, double dth, NextStep nx) {
   const uint32_t b = brick_of_block(bp);
   if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
-  lj_brick_body<MIXED, ACCUM, EPI, U>(g, bp, b, r, cell_off, cell_n, blist, bcnt, bperm, lj, A, dth, nx);
+  lj_brick_body<MIXED, ACCUM, EPI, U>(g, bp, b, r, cell_off, cell_n, blist, bcnt, lj, A, dth, nx);
 }
 
 // ------------------------------------------------------------------ multimat force
@@ -493,16 +447,15 @@ __global__ void __launch_bounds__(1024) k_lj_brick(Geom g, BrickParams bp, const
 template <bool MIXED, int NT, bool ACCUM, int EPI, int U>
 __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams& bp, uint32_t b, const double4* __restrict__ r,
                                                 const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
-                                                const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt, const uint16_t* __restrict__ bperm,
+                                                const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt,
                                                 const LJMulti& mm, const PState& A, double dth, const NextStep& nx) {
   extern __shared__ __align__(16) char lds[];
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
   if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, true)) return;
+  const LaneAtom At = lane_atom(bp, L);
   const uint32_t t = threadIdx.x, T = bp.T;
-  const uint32_t lane = t < T ? (uint32_t)bperm[(size_t)b * T + t] : t;  // lanes sorted by list length
-  const LaneAtom At = lane_atom(bp, L, lane);
-  const uint32_t n = At.live ? bcnt[(size_t)b * T + lane] : 0u;
+  const uint32_t n = At.live ? bcnt[(size_t)b * T + t] : 0u;
   const uint32_t nmax = wave_max_b(n);
   const double ax0 = L.px[3 * At.ia], ay0 = L.py[3 * At.ia], az0 = L.pz[3 * At.ia];
   const unsigned ta = L.pt[At.ia];
@@ -528,7 +481,7 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
     gx[q] = gy[q] = gz[q] = ge[q] = 0.0;
     any[q] = false;
   }
-  const uint2* lst = reinterpret_cast<const uint2*>(blist + (size_t)b * bp.cap * T) + lane;
+  const uint2* lst = reinterpret_cast<const uint2*>(blist + (size_t)b * bp.cap * T) + t;
   constexpr int W = U / 4;
   uint2 wn[W];
   if (nmax > 0) {
@@ -664,11 +617,11 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
 template <bool MIXED, int NT, bool ACCUM, int EPI>
 __global__ void __launch_bounds__(1024) k_ljmm_brick(Geom g, BrickParams bp, const double4* __restrict__ r,
                                                      const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
-                                                     const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt,
-                                                     const uint16_t* __restrict__ bperm, LJMulti mm, PState A, double dth, NextStep nx) {
+                                                     const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt, LJMulti mm,
+                                                     PState A, double dth, NextStep nx) {
   const uint32_t b = brick_of_block(bp);
   if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
-  ljmm_brick_body<MIXED, NT, ACCUM, EPI, LJB_UNROLL_B>(g, bp, b, r, cell_off, cell_n, blist, bcnt, bperm, mm, A, dth, nx);
+  ljmm_brick_body<MIXED, NT, ACCUM, EPI, LJB_UNROLL_B>(g, bp, b, r, cell_off, cell_n, blist, bcnt, mm, A, dth, nx);
 }
 
 // ------------------------------------------------------------------ conversion
@@ -684,7 +637,7 @@ __global__ void __launch_bounds__(1024) k_brick_to_slots(Geom g, BrickParams bp,
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
   if (!brick_setup(g, bp, P, L, nullptr, cell_off, cell_n, false)) return;
-  const LaneAtom A = lane_atom(bp, L, threadIdx.x);
+  const LaneAtom A = lane_atom(bp, L);
   if (!A.live) return;
   const uint32_t t = threadIdx.x, T = bp.T;
   const uint32_t n = min(bcnt[(size_t)b * T + t], bp.cap);
@@ -747,19 +700,18 @@ static hipError_t set_lds(const void* fn, uint32_t bytes) {
 
 hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n,
                             double max2, int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt, uint32_t* max_cnt,
-                            uint16_t* bperm, hipStream_t st) {
+                            hipStream_t st) {
   const uint32_t lds = brick_layout(bp);
   hipError_t e;
   if (half_symmetric || skip_ghosts) {
     e = set_lds((const void*)k_nbh_brick_mask<true>, lds);
     if (e != hipSuccess) return e;
     k_nbh_brick_mask<true><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist, bcnt,
-                                                          max_cnt, bperm);
+                                                          max_cnt);
   } else {
     e = set_lds((const void*)k_nbh_brick_mask<false>, lds);
     if (e != hipSuccess) return e;
-    k_nbh_brick_mask<false><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, 0, 0, blist, bcnt, max_cnt,
-                                                                    bperm);
+    k_nbh_brick_mask<false><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, 0, 0, blist, bcnt, max_cnt);
   }
   return hipGetLastError();
 }
@@ -768,26 +720,25 @@ static constexpr int LJB_UNROLL = LJB_UNROLL_B;
 
 template <bool MIXED, bool ACCUM, int EPI>
 static hipError_t ljb_launch(const Geom& g, const BrickParams& bp, uint32_t lds, const double4* r, const uint32_t* cell_off,
-                             const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const uint16_t* bperm, const LJConst& lj,
-                             const PState& A, double dth, const NextStep& nx, hipStream_t st) {
+                             const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const LJConst& lj, const PState& A,
+                             double dth, const NextStep& nx, hipStream_t st) {
   hipError_t e = set_lds((const void*)k_lj_brick<MIXED, ACCUM, EPI, LJB_UNROLL>, lds);
   if (e != hipSuccess) return e;
-  k_lj_brick<MIXED, ACCUM, EPI, LJB_UNROLL><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, blist, bcnt, bperm, lj, A,
-                                                                                     dth, nx);
+  k_lj_brick<MIXED, ACCUM, EPI, LJB_UNROLL><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, blist, bcnt, lj, A, dth, nx);
   return hipGetLastError();
 }
 
 uint32_t brick_list_granule() { return LJB_UNROLL; }
 
 hipError_t launch_lj_brick(int precision, bool accum, int epi, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
-                           const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const uint16_t* bperm, double eps,
-                           double sigma, double rcut, double dth, const NextStep* next, hipStream_t st) {
+                           const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, double eps, double sigma, double rcut,
+                           double dth, const NextStep* next, hipStream_t st) {
   const LJConst lj = make_lj_const(eps, sigma, rcut);
   const uint32_t lds = brick_layout(bp);
   const NextStep nx = next ? *next : NextStep{};
   const bool next_epi = epi == EPI_NEXT || epi == EPI_NEXT_LEAN;
   if (next_epi && (accum || !next)) return hipErrorInvalidValue;
-#define XNB_LJB_ARGS g, bp, lds, A.r, cell_off, cell_n, blist, bcnt, bperm, lj, A, dth, nx, st
+#define XNB_LJB_ARGS g, bp, lds, A.r, cell_off, cell_n, blist, bcnt, lj, A, dth, nx, st
   if (precision) {
     if (epi == EPI_NEXT) return ljb_launch<true, false, EPI_NEXT>(XNB_LJB_ARGS);
     if (epi == EPI_NEXT_LEAN) return ljb_launch<true, false, EPI_NEXT_LEAN>(XNB_LJB_ARGS);
@@ -803,42 +754,41 @@ hipError_t launch_lj_brick(int precision, bool accum, int epi, const Geom& g, Br
 
 template <bool MIXED, int NT, bool ACCUM, int EPI>
 static hipError_t ljmm_launch(const Geom& g, const BrickParams& bp, uint32_t lds, const uint32_t* cell_off, const uint32_t* cell_n,
-                              const uint16_t* blist, const uint32_t* bcnt, const uint16_t* bperm, const LJMulti& mm, const PState& A,
-                              double dth, const NextStep& nx, hipStream_t st) {
+                              const uint16_t* blist, const uint32_t* bcnt, const LJMulti& mm, const PState& A, double dth,
+                              const NextStep& nx, hipStream_t st) {
   hipError_t e = set_lds((const void*)k_ljmm_brick<MIXED, NT, ACCUM, EPI>, lds);
   if (e != hipSuccess) return e;
-  k_ljmm_brick<MIXED, NT, ACCUM, EPI><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, A.r, cell_off, cell_n, blist, bcnt, bperm, mm, A,
-                                                                                      dth, nx);
+  k_ljmm_brick<MIXED, NT, ACCUM, EPI><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, A.r, cell_off, cell_n, blist, bcnt, mm, A, dth, nx);
   return hipGetLastError();
 }
 
 template <bool MIXED, int NT>
 static hipError_t ljmm_dispatch(int epi, const Geom& g, const BrickParams& bp, uint32_t lds, const uint32_t* cell_off,
-                                const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const uint16_t* bperm,
-                                const LJMulti& mm, const PState& A, double dth, const NextStep& nx, hipStream_t st) {
+                                const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const LJMulti& mm, const PState& A,
+                                double dth, const NextStep& nx, hipStream_t st) {
   switch (epi) {
-    case EPI_NONE: return ljmm_launch<MIXED, NT, true, EPI_NONE>(g, bp, lds, cell_off, cell_n, blist, bcnt, bperm, mm, A, dth, nx, st);
-    case EPI_KICK: return ljmm_launch<MIXED, NT, false, EPI_KICK>(g, bp, lds, cell_off, cell_n, blist, bcnt, bperm, mm, A, dth, nx, st);
-    case EPI_NEXT: return ljmm_launch<MIXED, NT, false, EPI_NEXT>(g, bp, lds, cell_off, cell_n, blist, bcnt, bperm, mm, A, dth, nx, st);
-    default: return ljmm_launch<MIXED, NT, false, EPI_NEXT_LEAN>(g, bp, lds, cell_off, cell_n, blist, bcnt, bperm, mm, A, dth, nx, st);
+    case EPI_NONE: return ljmm_launch<MIXED, NT, true, EPI_NONE>(g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
+    case EPI_KICK: return ljmm_launch<MIXED, NT, false, EPI_KICK>(g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
+    case EPI_NEXT: return ljmm_launch<MIXED, NT, false, EPI_NEXT>(g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
+    default: return ljmm_launch<MIXED, NT, false, EPI_NEXT_LEAN>(g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
   }
 }
 
 // Multimat force on brick lists.  epi EPI_NONE accumulates into f (after zero_particle_force);
 // the other epilogues start f from 0, as the run loop's zero + force.  ntypes 1..4.
 hipError_t launch_ljmm_brick(int precision, int ntypes, int epi, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
-                             const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const uint16_t* bperm, const LJMulti& mm,
-                             double dth, const NextStep* next, hipStream_t st) {
+                             const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const LJMulti& mm, double dth,
+                             const NextStep* next, hipStream_t st) {
   if (ntypes < 1 || ntypes > 4) return hipErrorInvalidValue;
   if ((epi == EPI_NEXT || epi == EPI_NEXT_LEAN) && !next) return hipErrorInvalidValue;
   const uint32_t lds = brick_layout(bp);
   const NextStep nx = next ? *next : NextStep{};
   if (precision) {
-    if (ntypes <= 2) return ljmm_dispatch<true, 2>(epi, g, bp, lds, cell_off, cell_n, blist, bcnt, bperm, mm, A, dth, nx, st);
-    return ljmm_dispatch<true, 4>(epi, g, bp, lds, cell_off, cell_n, blist, bcnt, bperm, mm, A, dth, nx, st);
+    if (ntypes <= 2) return ljmm_dispatch<true, 2>(epi, g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
+    return ljmm_dispatch<true, 4>(epi, g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
   }
-  if (ntypes <= 2) return ljmm_dispatch<false, 2>(epi, g, bp, lds, cell_off, cell_n, blist, bcnt, bperm, mm, A, dth, nx, st);
-  return ljmm_dispatch<false, 4>(epi, g, bp, lds, cell_off, cell_n, blist, bcnt, bperm, mm, A, dth, nx, st);
+  if (ntypes <= 2) return ljmm_dispatch<false, 2>(epi, g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
+  return ljmm_dispatch<false, 4>(epi, g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
 }
 
 hipError_t launch_brick_to_slots(const Geom& g, BrickParams bp, const uint32_t* cell_off, const uint32_t* cell_n, const uint16_t* blist,
@@ -867,7 +817,7 @@ __global__ void __launch_bounds__(1024) kb_nbh_brick(const BoxFrame* __restrict_
   if (!batch_brick(work, box, b)) return;  // whole workgroup
   const BoxFrame& f = F[box];
   const double4* r = f.rbuf[(f.rcur ^ par) & 1u];
-  nbh_brick_body<false>(f.g, f.bp, b, r, f.cell_off, f.cell_n, f.max2, 0, 0, f.blist, f.bcnt, f.words + 1, f.bperm);
+  nbh_brick_body<false>(f.g, f.bp, b, r, f.cell_off, f.cell_n, f.max2, 0, 0, f.blist, f.bcnt, f.words + 1);
 }
 
 template <bool MIXED, int EPI>
@@ -877,7 +827,7 @@ __device__ __forceinline__ void lj_batch_case(const BoxFrame& f, uint32_t b, uin
   A.r = f.rbuf[cur];
   // the next step's count: local step k = gstep - gstep0 + 1, count of step k + 1 at [k]
   const NextStep nx{f.rbuf[cur ^ 1u], f.backup, f.dt, f.dt2, f.thr2, f.counts + (gstep - f.gstep0 + 1)};
-  lj_brick_body<MIXED, false, EPI, LJB_UNROLL_B>(f.g, f.bp, b, A.r, f.cell_off, f.cell_n, f.blist, f.bcnt, f.bperm, f.lj, A, f.dth, nx);
+  lj_brick_body<MIXED, false, EPI, LJB_UNROLL_B>(f.g, f.bp, b, A.r, f.cell_off, f.cell_n, f.blist, f.bcnt, f.lj, A, f.dth, nx);
 }
 
 // Force of every brick of every box; precision and epilogue are per box (uniform per

@@ -84,7 +84,6 @@ struct xnb_ctx {
   int brick_dims[3] = {0, 0, 0};  // 0: choose automatically
   DArr<uint16_t> blist;
   DArr<uint32_t> bcnt;
-  DArr<uint16_t> bperm;  // per brick: lanes sorted by list length (force lane order)
   bool slot_lists_current = false;  // nbr/nbr_cnt mirror the brick lists
   bool fuse_push = true;            // force kernel performs the next step's first half (brick lists)
   uint32_t lds_limit = 65536;

@@ -70,7 +70,6 @@ struct BoxFrame {
   uint32_t *mv_dst, *pk, *newloc, *in_list, *ghost_src, *ghost_shift, *backup;
   uint16_t* blist;
   uint32_t* bcnt;
-  uint16_t* bperm;
   LJConst lj;
   double dt, dt2, dth, thr2, max2;
   uint32_t* words;             // [0] status bits (1, 2 brick sizes; 4 ghost capacity), [1] max list count, [2] left the domain, [3] ghost count
@@ -153,19 +152,16 @@ void launch_fp64_core(const double* x, size_t n, double* sq, double* rc, hipStre
 
 BrickParams brick_params(const Geom& g, int Bi, int Bj, int Bk);
 uint32_t brick_layout(BrickParams& bp);
-// static LDS of the brick build kernel (lane sort and scans), beside the dynamic layout
-constexpr uint32_t kBrickStaticLds = 9216;
 uint32_t brick_list_granule();
 void launch_brick_stats(const Geom& g, const BrickParams& bp, const uint32_t* cell_n, uint32_t* out, hipStream_t st);
 hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, double max2,
-                            int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt, uint32_t* max_cnt, uint16_t* bperm,
-                            hipStream_t st);
+                            int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt, uint32_t* max_cnt, hipStream_t st);
 hipError_t launch_lj_brick(int precision, bool accum, int epi, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
-                           const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const uint16_t* bperm, double eps,
-                           double sigma, double rcut, double dth, const NextStep* next, hipStream_t st);
+                           const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, double eps, double sigma, double rcut,
+                           double dth, const NextStep* next, hipStream_t st);
 hipError_t launch_ljmm_brick(int precision, int ntypes, int epi, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
-                             const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const uint16_t* bperm, const LJMulti& mm,
-                             double dth, const NextStep* next, hipStream_t st);
+                             const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const LJMulti& mm, double dth,
+                             const NextStep* next, hipStream_t st);
 hipError_t launch_brick_to_slots(const Geom& g, BrickParams bp, const uint32_t* cell_off, const uint32_t* cell_n, const uint16_t* blist,
                                  const uint32_t* bcnt, uint32_t* nbr, uint32_t* nbr_cnt, uint32_t cap, hipStream_t st);
 void launch_energies(uint32_t N, const PState& A, double* part, uint32_t nparts, double* out, hipStream_t st);
