@@ -85,6 +85,7 @@ _SIGS = {
     "xnb_synchronize": (C.c_int, [P]),
     "xnb_copy_probe": (C.c_int, [P, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "xnb_fp64_core": (C.c_int, [P, P, C.c_size_t, P, P]),
+    "xnb_autotune": (C.c_int, [P, C.POINTER(LJParams), C.c_double, C.c_int, P]),
     "xnb_snapshot_size": (C.c_int, [P, C.POINTER(C.c_size_t)]),
     "xnb_snapshot_save": (C.c_int, [P, P, C.c_size_t]),
     "xnb_snapshot_load": (C.c_int, [P, P, C.c_size_t]),

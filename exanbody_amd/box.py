@@ -200,6 +200,13 @@ class Box:
         self._check(self.L.xnb_run_multimat(self._h, C.byref(p), int(ntypes), _ptr(e), _ptr(s), _ptr(rc)), "run_multimat")
         return dict(rebuilds=p.n_rebuilds, violations=p.n_violations)
 
+    def autotune(self, epsilon, sigma, rcut, precision=_lib.PRECISION_FP64):
+        """Brick shape chosen by timing (cached per box class); returns it (0s off the brick path)."""
+        p = _lib.LJParams(epsilon, sigma)
+        dims = np.zeros(3, np.int32)
+        self._check(self.L.xnb_autotune(self._h, C.byref(p), rcut, int(precision), _ptr(dims)), "autotune")
+        return tuple(int(x) for x in dims)
+
     # ---- snapshots
     def snapshot(self):
         """Bytes of the box's full step-loop state (see xnb_snapshot_save)."""

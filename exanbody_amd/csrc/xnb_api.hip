@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1174,6 +1177,101 @@ int xnb_run_multimat(xnb_ctx* c, xnb_run_params* p, int ntypes, const double* ep
   LJMulti mm;
   if (make_lj_multi(ntypes, eps, sigma, rcut, mm)) return fail(c, XNB_E_ARG, "too many type pairs");
   return run_loop(c, p, &mm, ntypes);
+}
+
+// ---- autotuner: brick shape (cells per workgroup tile) by measurement, cached per class
+namespace {
+struct TuneKey {
+  int di, dj, dk, occ, precision;
+  bool operator<(const TuneKey& o) const {
+    if (di != o.di) return di < o.di;
+    if (dj != o.dj) return dj < o.dj;
+    if (dk != o.dk) return dk < o.dk;
+    if (occ != o.occ) return occ < o.occ;
+    return precision < o.precision;
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::array<int, 3>> g_tune;
+}  // namespace
+
+int xnb_autotune(xnb_ctx* c, const xnb_lj_params* p, double rcut, int precision, int* dims_out) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (!p) return fail(c, XNB_E_ARG, "null parameters");
+  if (!c->lists_valid || !c->ghosts_valid) return fail(c, XNB_E_STATE, "xnb_init must run first");
+  if (c->list_kind != 1) {  // not on the brick path: nothing to tune
+    if (dims_out) dims_out[0] = dims_out[1] = dims_out[2] = 0;
+    return XNB_OK;
+  }
+  const Geom& g = c->g;
+  // class: cells per side (large boxes alike), mean atoms per inner cell, precision
+  const double occ = (double)c->N / ((double)g.Di * g.Dj * g.Dk);
+  const TuneKey key{std::min(g.Di, 16), std::min(g.Dj, 16), std::min(g.Dk, 16), (int)std::lround(occ), precision};
+  std::array<int, 3> best{0, 0, 0};
+  bool cached = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      best = it->second;
+      cached = true;
+    }
+  }
+  const int saved[3] = {c->brick_dims[0], c->brick_dims[1], c->brick_dims[2]};
+  xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
+  if (!cached) {
+    static const int cand[][3] = {{4, 4, 2}, {4, 2, 2}, {2, 4, 2}, {4, 4, 1}, {2, 2, 2}, {4, 2, 1}};
+    hipEvent_t t0, t1;
+    HIPCHK(c, hipEventCreate(&t0));
+    HIPCHK(c, hipEventCreate(&t1));
+    float best_ms = 1e30f;
+    for (const auto& d : cand) {
+      c->brick_dims[0] = d[0];
+      c->brick_dims[1] = d[1];
+      c->brick_dims[2] = d[2];
+      if ((e = op_nbh(c, c->nbh_dist_lab, cfg, 0))) break;
+      if (c->list_kind != 1 || c->bp.Bi != d[0] || c->bp.Bj != d[1] || c->bp.Bk != d[2]) continue;  // did not fit
+      // forces are rewritten with the values they had (the shape does not change results)
+      if ((e = launch_force(c, precision, false, false, false, p->epsilon, p->sigma, rcut, 0.0))) break;
+      HIPCHK(c, hipEventRecord(t0, c->st));
+      for (int q = 0; q < 3 && !e; q++) e = launch_force(c, precision, false, false, false, p->epsilon, p->sigma, rcut, 0.0);
+      HIPCHK(c, hipEventRecord(t1, c->st));
+      HIPCHK(c, hipEventSynchronize(t1));
+      if (e) break;
+      float ms = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&ms, t0, t1));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = {d[0], d[1], d[2]};
+      }
+    }
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    if (e) return e;
+    if (best[0] > 0) {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune[key] = best;
+    }
+  }
+  if (best[0] > 0) {
+    c->brick_dims[0] = best[0];
+    c->brick_dims[1] = best[1];
+    c->brick_dims[2] = best[2];
+  } else {
+    c->brick_dims[0] = saved[0];
+    c->brick_dims[1] = saved[1];
+    c->brick_dims[2] = saved[2];
+  }
+  if ((e = op_nbh(c, c->nbh_dist_lab, cfg, 0))) return e;
+  if ((e = launch_force(c, precision, false, false, false, p->epsilon, p->sigma, rcut, 0.0))) return e;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (dims_out) {
+    dims_out[0] = c->list_kind == 1 ? c->bp.Bi : 0;
+    dims_out[1] = c->list_kind == 1 ? c->bp.Bj : 0;
+    dims_out[2] = c->list_kind == 1 ? c->bp.Bk : 0;
+  }
+  return XNB_OK;
 }
 
 int xnb_timer_enable(xnb_ctx* c, int enable) {

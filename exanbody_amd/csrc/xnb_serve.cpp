@@ -84,6 +84,7 @@ int xnb_serve_run(const xnb_job* jobs, int njobs, const xnb_serve_config* cfg, x
       if (!rc) rc = xnb_nbh_dist(a.ctx, J.run.rcut, J.run.rcut_inc, nullptr, nullptr, nullptr);
       if (!rc) rc = xnb_set_particles(a.ctx, J.n, J.r, J.v, J.id, nullptr);
       if (!rc) rc = xnb_init(a.ctx, &J.run.lj, J.run.rcut, J.run.rcut_inc, J.run.precision);
+      if (!rc && cfg->autotune) rc = xnb_autotune(a.ctx, &J.run.lj, J.run.rcut, J.run.precision, nullptr);
       if (!rc) rc = xnb_batch_attach(B, a.ctx, &J.run, &a.slot);
       if (rc) {
         if (a.ctx) xnb_destroy(a.ctx);

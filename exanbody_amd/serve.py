@@ -20,7 +20,8 @@ class JobResult(C.Structure):
 
 
 class ServeConfig(C.Structure):
-    _fields_ = [("device", C.c_int), ("hbm_budget_bytes", C.c_size_t), ("max_active", C.c_int), ("steps_per_iter", C.c_int)]
+    _fields_ = [("device", C.c_int), ("hbm_budget_bytes", C.c_size_t), ("max_active", C.c_int), ("steps_per_iter", C.c_int),
+                ("autotune", C.c_int)]
 
 
 class ServeStats(C.Structure):
@@ -58,7 +59,7 @@ def job_mix(n_jobs, rate, atoms=(1000, 200000), rho=(0.70, 0.90), temperature=(0
     return jobs
 
 
-def serve(jobs, device=0, hbm_budget_bytes=0, max_active=0, steps_per_iter=1, outputs=False):
+def serve(jobs, device=0, hbm_budget_bytes=0, max_active=0, steps_per_iter=1, outputs=False, autotune=True):
     """Run the loop over job dicts (arrival-ordered).  Returns (results, stats); with outputs,
     each result carries the final r, v, id in cell order."""
     L = _lib.load()
@@ -89,7 +90,7 @@ def serve(jobs, device=0, hbm_budget_bytes=0, max_active=0, steps_per_iter=1, ou
                      _ptr(o["v"]) if o else None, _ptr(o["id"]) if o else None)
     res = (JobResult * max(n, 1))()
     st = ServeStats()
-    cfg = ServeConfig(int(device), int(hbm_budget_bytes), int(max_active), int(steps_per_iter))
+    cfg = ServeConfig(int(device), int(hbm_budget_bytes), int(max_active), int(steps_per_iter), int(bool(autotune)))
     rc = L.xnb_serve_run(arr, n, C.byref(cfg), res, C.byref(st))
     if rc != 0:
         raise XnbError(rc, "xnb_serve_run failed")

@@ -190,6 +190,15 @@ int xnb_copy_probe(xnb_ctx* ctx, size_t bytes, int iters, double* gbps);
  * [2^-767, 2^1000], recip_out[i] for x[i] in [2^-383, 2^383]. */
 int xnb_fp64_core(xnb_ctx* ctx, const double* x, size_t n, double* sqrt_out, double* recip_out);
 
+/* ---- autotuning ------------------------------------------------------------------ */
+/* No counterpart in the reference.  Picks the brick shape (cells per workgroup tile; the tile's
+ * atoms are the workgroup's lanes) of an initialised box by timing its LJ force kernel for each
+ * candidate shape, and rebuilds its lists with the fastest; results are cached per process by
+ * (cells per side up to 16, mean atoms per cell, precision), so boxes of a seen class are not
+ * timed again.  Forces and energies are recomputed (bitwise the same: the shape does not enter
+ * the arithmetic).  dims_out (optional) receives the shape, 0s off the brick path. */
+int xnb_autotune(xnb_ctx* ctx, const xnb_lj_params* params, double rcut, int precision, int* dims_out);
+
 /* ---- snapshots ------------------------------------------------------------------- */
 /* Device-resident box -> host bytes and back, for preemption and restart (the fields of
  * write_dump / read_dump, write_dump.cpp:38,43: positions, velocities, ids, types), plus what
@@ -254,6 +263,7 @@ typedef struct {
   size_t hbm_budget_bytes;             /* 0: no limit */
   int max_active;                      /* <= 0: no limit */
   int steps_per_iter;                  /* batch steps between admission checks (>= 1) */
+  int autotune;                        /* 1: xnb_autotune each admitted job (cached per class) */
 } xnb_serve_config;
 
 typedef struct {

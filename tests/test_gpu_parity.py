@@ -219,3 +219,25 @@ def test_snapshot_restart_is_bitwise(precision):
     assert_same_layout(ea, ec, "restart")
     for k in ("v", "f", "ep"):
         np.testing.assert_array_equal(ea[k].view(np.uint64), ec[k].view(np.uint64), err_msg=k)
+
+
+def test_autotune_keeps_results():
+    """The autotuner only changes the brick shape: forces, energies and a later run are bitwise
+    those of an untuned box; a second box of the same class reuses the cached shape."""
+    r, v, ids, L = fcc_inputs(8, temperature=1.0, jitter=0.05, seed=51)
+    _, a, _ = make_pair(r, v, ids, L)
+    _, b, _ = make_pair(r, v, ids, L)
+    for x in (a, b):
+        x.init(1.0, 1.0, RCUT, SKIN, PRECISION_FP64)
+    f0 = a.export()
+    dims = a.autotune(1.0, 1.0, RCUT, PRECISION_FP64)
+    assert dims in {(4, 4, 2), (4, 2, 2), (2, 4, 2), (4, 4, 1), (2, 2, 2), (4, 2, 1)}
+    f1 = a.export()
+    for k in ("r", "f", "ep"):
+        np.testing.assert_array_equal(f0[k].view(np.uint64), f1[k].view(np.uint64), err_msg=k)
+    assert b.autotune(1.0, 1.0, RCUT, PRECISION_FP64) == dims
+    a.run(15, 0.005, 1.0, 1.0, RCUT, SKIN, precision=PRECISION_FP64, rebuild_every=5)
+    b.run(15, 0.005, 1.0, 1.0, RCUT, SKIN, precision=PRECISION_FP64, rebuild_every=5)
+    ea, eb = a.export(), b.export()
+    assert_same_layout(ea, eb, "tuned")
+    np.testing.assert_array_equal(ea["v"].view(np.uint64), eb["v"].view(np.uint64))
