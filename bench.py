@@ -141,7 +141,6 @@ def batched_bench(n_boxes, steps, alone_sample, seed=5):
         b.nbh_dist(RCUT, SKIN)
         b.set_particles(j["r"], j["v"], j["id"])
         b.init(1.0, 1.0, RCUT, SKIN, PRECISION_MIXED)
-        b.autotune(1.0, 1.0, RCUT, PRECISION_MIXED)  # brick shape by timing, cached per box class
         return b
 
     t = time.perf_counter()

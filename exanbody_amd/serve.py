@@ -59,7 +59,7 @@ def job_mix(n_jobs, rate, atoms=(1000, 200000), rho=(0.70, 0.90), temperature=(0
     return jobs
 
 
-def serve(jobs, device=0, hbm_budget_bytes=0, max_active=0, steps_per_iter=1, outputs=False, autotune=True):
+def serve(jobs, device=0, hbm_budget_bytes=0, max_active=0, steps_per_iter=1, outputs=False, autotune=False):
     """Run the loop over job dicts (arrival-ordered).  Returns (results, stats); with outputs,
     each result carries the final r, v, id in cell order."""
     L = _lib.load()
