@@ -384,10 +384,11 @@ __device__ __forceinline__ void lj_brick_body(const Geom& g, const BrickParams& 
       rb[u].z = L.pz[3 * j[u]];
       rb[u].w = 0.;
     }
+    if (MIXED) {
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      if (MIXED) lj_pair_mixed<EPI != EPI_NEXT_LEAN>(lj, ra, rb[u], tx, ty, tz, te);
-      else lj_pair_fp64<EPI != EPI_NEXT_LEAN>(lj, ra, rb[u], tx, ty, tz, te);
+      for (int u = 0; u < U; u++) lj_pair_mixed<EPI != EPI_NEXT_LEAN>(lj, ra, rb[u], tx, ty, tz, te);
+    } else {
+      lj_batch_fp64<U, EPI != EPI_NEXT_LEAN>(lj, ra, rb, tx, ty, tz, te);
     }
   }
   constexpr bool NEXT = EPI == EPI_NEXT || EPI == EPI_NEXT_LEAN;

@@ -167,6 +167,52 @@ inline int make_lj_multi(int ntypes, const double* eps, const double* sigma, con
   return 0;
 }
 
+// U pairs of the plain FP64 form at once, summed in u order (the same sums as U calls of
+// lj_pair_fp64): the distance tests first, one wave-uniform range decision for all U roots, then
+// the U independent root / reciprocal chains back to back, which the compiler interleaves.
+template <int U, bool ENERGY = true>
+__device__ inline void lj_batch_fp64(const LJConst& lj, const double4& ra, const double4 (&rb)[U], double& tx, double& ty,
+                                     double& tz, double& te) {
+  double dx[U], dy[U], dz[U], x[U];
+  bool in[U];
+  bool small = false;
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    dx[u] = rb[u].x - ra.x;
+    dy[u] = rb[u].y - ra.y;
+    dz[u] = rb[u].z - ra.z;
+    const double d2 = (dx[u] * dx[u] + dy[u] * dy[u]) + dz[u] * dz[u];
+    in[u] = d2 > 0.0 && d2 <= lj.rc2;
+    x[u] = in[u] ? d2 : 1.0;
+    small = small || x[u] < 0x1p-767;
+  }
+  double inv_r[U];
+  if (__builtin_expect(__any(small), 0)) {
+#pragma unroll
+    for (int u = 0; u < U; u++) inv_r[u] = 1.0 / sqrt(x[u]);
+  } else {
+#pragma unroll
+    for (int u = 0; u < U; u++) inv_r[u] = recip_normal(sqrt_normal(x[u]));
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const double ratio = lj.sigma * inv_r[u];
+    const double ratio2 = ratio * ratio;
+    const double ratio6 = ratio2 * ratio2 * ratio2;
+    const double ratio12 = ratio6 * ratio6;
+    double de = (lj.m24eps * (2. * ratio12 - ratio6)) * inv_r[u];
+    de *= inv_r[u];
+    de = in[u] ? de : 0.0;
+    tx += de * dx[u];
+    ty += de * dy[u];
+    tz += de * dz[u];
+    if (ENERGY) {
+      const double e = lj.eps4 * (ratio12 - ratio6);
+      te += in[u] ? e : 0.0;
+    }
+  }
+}
+
 inline LJConst make_lj_const(double eps, double sigma, double rcut) {
   LJConst c;
   c.sigma = sigma;
