@@ -475,6 +475,7 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
     m24f[q] = mm.m24epsf[pid];
   }
   constexpr bool ENERGY = EPI != EPI_NEXT_LEAN;
+  constexpr int H = MIXED ? U : 2;
   double gx[NT], gy[NT], gz[NT], ge[NT];
   bool any[NT];
 #pragma unroll
@@ -506,64 +507,87 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
       j[4 * w + 2] = wc[w].y & 0xffffu;
       j[4 * w + 3] = wc[w].y >> 16;
     }
+    // pairs in sub-batches of H: one range decision per sub-batch for the FP64 roots.  FP64
+    // takes pairs by two: its per-type tables and sums leave too few registers for more in
+    // flight (timed on the two-species slab: force 3.39 ms at H = 2, 3.44 at 4, 3.54 at 8)
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const double dx = L.px[3 * j[u]] - ax0;
-      const double dy = L.py[3 * j[u]] - ay0;
-      const double dz = L.pz[3 * j[u]] - az0;
-      const unsigned tb = min((unsigned)L.pt[j[u]], (unsigned)(NT - 1));
-      double psg = sg[0], pe4 = e4[0], pm24 = m24[0], prc2 = rc2[0];
-      float ps2f = s2f[0], pe4f = e4f[0], pm24f = m24f[0];
+    for (int h = 0; h < U; h += H) {
+      // per pair: displacement, neighbour type, inside its pair's cutoff
+      double dx[H], dy[H], dz[H], d2[H];
+      unsigned tb[H];
+      bool in[H];
+      bool small = false;
 #pragma unroll
-      for (int q = 1; q < NT; q++)
-        if (tb == (unsigned)q) {
-          psg = sg[q];
-          pe4 = e4[q];
-          pm24 = m24[q];
-          prc2 = rc2[q];
-          ps2f = s2f[q];
-          pe4f = e4f[q];
-          pm24f = m24f[q];
+      for (int u = 0; u < H; u++) {
+        dx[u] = L.px[3 * j[h + u]] - ax0;
+        dy[u] = L.py[3 * j[h + u]] - ay0;
+        dz[u] = L.pz[3 * j[h + u]] - az0;
+        tb[u] = min((unsigned)L.pt[j[h + u]], (unsigned)(NT - 1));
+        double prc2 = rc2[0];
+#pragma unroll
+        for (int q = 1; q < NT; q++) prc2 = tb[u] == (unsigned)q ? rc2[q] : prc2;
+        d2[u] = MIXED ? fma(dz[u], dz[u], fma(dy[u], dy[u], dx[u] * dx[u])) : (dx[u] * dx[u] + dy[u] * dy[u]) + dz[u] * dz[u];
+        in[u] = d2[u] <= prc2;
+        small = small || (in[u] && d2[u] < 0x1p-767);
+      }
+      double inv_r[H];  // plain form: 1.0 / sqrt(d2), each correctly rounded (1.0 outside the cutoff)
+      if (!MIXED) {
+        if (__builtin_expect(__any(small), 0)) {
+#pragma unroll
+          for (int u = 0; u < H; u++) inv_r[u] = 1.0 / sqrt(in[u] ? d2[u] : 1.0);
+        } else {
+#pragma unroll
+          for (int u = 0; u < H; u++) inv_r[u] = recip_normal(sqrt_normal(in[u] ? d2[u] : 1.0));
         }
-      double coef, ce;  // force coefficient (f += coef * d) and pair energy
-      bool in;
-      if (MIXED) {
-        const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        in = d2 <= prc2;
-        const float inv_r2 = in ? __builtin_amdgcn_rcpf((float)d2) : 0.0f;  // 0 outside: coefficient and energy 0
-        const float s2 = ps2f * inv_r2;
-        const float s6 = s2 * s2 * s2;
-        const float s12 = s6 * s6;
-        coef = (double)(pm24f * (2.0f * s12 - s6) * inv_r2);
-        ce = ENERGY ? (double)(pe4f * (s12 - s6)) : 0.0;
-      } else {
-        const double d2 = (dx * dx + dy * dy) + dz * dz;
-        in = d2 <= prc2;
-        const double inv_r = inv_sqrt_rounded(in ? d2 : 1.0);
-        const double ratio = psg * inv_r;
-        const double ratio2 = ratio * ratio;
-        const double ratio6 = ratio2 * ratio2 * ratio2;
-        const double ratio12 = ratio6 * ratio6;
-        double de = (pm24 * (2. * ratio12 - ratio6)) * inv_r;
-        de *= inv_r;
-        coef = in ? de : 0.0;
-        ce = (ENERGY && in) ? pe4 * (ratio12 - ratio6) : 0.0;
       }
 #pragma unroll
-      for (int q = 0; q < NT; q++) {
-        const bool mine = in && tb == (unsigned)q;
-        const double cq = mine ? coef : 0.0;
-        if (MIXED) {  // f = fma(c, d, f), the singlemat mixed form
-          gx[q] = fma(cq, dx, gx[q]);
-          gy[q] = fma(cq, dy, gy[q]);
-          gz[q] = fma(cq, dz, gz[q]);
-        } else {  // f += (de / r) * d, the plain form
-          gx[q] += cq * dx;
-          gy[q] += cq * dy;
-          gz[q] += cq * dz;
+      for (int u = 0; u < H; u++) {
+        double psg = sg[0], pe4 = e4[0], pm24 = m24[0];
+        float ps2f = s2f[0], pe4f = e4f[0], pm24f = m24f[0];
+#pragma unroll
+        for (int q = 1; q < NT; q++)
+          if (tb[u] == (unsigned)q) {
+            psg = sg[q];
+            pe4 = e4[q];
+            pm24 = m24[q];
+            ps2f = s2f[q];
+            pe4f = e4f[q];
+            pm24f = m24f[q];
+          }
+        double coef, ce;  // force coefficient (f += coef * d) and pair energy
+        if (MIXED) {
+          const float inv_r2 = in[u] ? __builtin_amdgcn_rcpf((float)d2[u]) : 0.0f;  // 0 outside: coefficient and energy 0
+          const float s2 = ps2f * inv_r2;
+          const float s6 = s2 * s2 * s2;
+          const float s12 = s6 * s6;
+          coef = (double)(pm24f * (2.0f * s12 - s6) * inv_r2);
+          ce = ENERGY ? (double)(pe4f * (s12 - s6)) : 0.0;
+        } else {
+          const double ratio = psg * inv_r[u];
+          const double ratio2 = ratio * ratio;
+          const double ratio6 = ratio2 * ratio2 * ratio2;
+          const double ratio12 = ratio6 * ratio6;
+          double de = (pm24 * (2. * ratio12 - ratio6)) * inv_r[u];
+          de *= inv_r[u];
+          coef = in[u] ? de : 0.0;
+          ce = (ENERGY && in[u]) ? pe4 * (ratio12 - ratio6) : 0.0;
         }
-        if (ENERGY) ge[q] += mine ? ce : 0.0;
-        any[q] = any[q] || mine;
+#pragma unroll
+        for (int q = 0; q < NT; q++) {
+          const bool mine = in[u] && tb[u] == (unsigned)q;
+          const double cq = mine ? coef : 0.0;
+          if (MIXED) {  // f = fma(c, d, f), the singlemat mixed form
+            gx[q] = fma(cq, dx[u], gx[q]);
+            gy[q] = fma(cq, dy[u], gy[q]);
+            gz[q] = fma(cq, dz[u], gz[q]);
+          } else {  // f += (de / r) * d, the plain form
+            gx[q] += cq * dx[u];
+            gy[q] += cq * dy[u];
+            gz[q] += cq * dz[u];
+          }
+          if (ENERGY) ge[q] += mine ? ce : 0.0;
+          any[q] = any[q] || mine;
+        }
       }
     }
   }
