@@ -55,6 +55,9 @@ __device__ inline uint32_t upper_index(const uint32_t* pref, uint32_t n, uint32_
 }
 
 static constexpr int LJB_UNROLL_B = (int)kListBatch;  // list entries per force batch
+// sentinel atoms staged after a region's atoms: index R is the force kernels' list padding;
+// the build reads whole candidate chunks of 32 from a row start and needs 32 of them
+static constexpr uint32_t kSentinels = 32;
 
 // position of entry q of lane t in a brick's list slab (T lanes)
 __device__ inline size_t lpos(uint32_t q, uint32_t t, uint32_t T) { return ((size_t)(q >> 2) * T + t) * 4 + (q & 3u); }
@@ -156,11 +159,11 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
       L.pz[3 * idx] = p.z;
       L.pt[idx] = (uint8_t)p.w;
     }
-    if (t == 0) {  // sentinel atom R: every distance to it exceeds any cutoff
-      L.px[3 * R] = 1e30;
-      L.py[3 * R] = 1e30;
-      L.pz[3 * R] = 1e30;
-      L.pt[R] = 0;
+    if (t < kSentinels) {  // sentinel atoms R .. R + kSentinels - 1: every distance to them exceeds any cutoff
+      L.px[3 * (R + t)] = 1e30;
+      L.py[3 * (R + t)] = 1e30;
+      L.pz[3 * (R + t)] = 1e30;
+      L.pt[R + t] = 0;
     }
     __syncthreads();
   }
@@ -219,11 +222,11 @@ __global__ void k_brick_stats(Geom g, BrickParams bp, const uint32_t* __restrict
 // ------------------------------------------------------------------ build
 // chunk_neighbors (chunk_neighbors_execute.h:221-239): d = r_a - r_b, FP64, kept iff
 // (a != b) && 0 < d2 <= nbh_dist^2, then the NeighborFilterHalfSymGhost filters.
-// Candidates are tested in chunks of 32 without branches (bit q of m set iff candidate
-// p0 + q is kept), then each lane expands its mask in ascending bit order, which appends
-// entries in candidate order.  Stores go to row
-// min(cnt, cap - 1): an overflowing list is rebuilt with a larger cap, so its last row may be
-// overwritten.  FILTER enables the half_symmetric / skip_ghosts filters.
+// Candidates are tested in chunks of 32 without branches into a bit mask, then each lane
+// expands its mask in candidate order, which appends entries in candidate order.  An
+// overflowing list (count > cap) is rebuilt with a larger cap, so what it stored past the
+// point of overflow does not matter.  FILTER enables the half_symmetric / skip_ghosts filters
+// (a per-cell loop; the unfiltered build walks rows of three cells).
 // Brick b of one box; the workgroup may be wider than bp.T (batched launches), lanes from
 // bp.T on hold no atom and touch no list.
 template <bool FILTER>
@@ -251,6 +254,53 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
   }
   const uint32_t last = bp.cap - 1;
   uint32_t cnt = 0;
+  if constexpr (!FILTER) {
+    // Rows: the cells di = -1, 0, 1 of one (dk, dj) are consecutive in LDS, so a row is one
+    // candidate range [rbase, rbase + rlen).  A chunk tests 4-aligned candidates p0 + k
+    // (k < np, np uniform) against every lane; bit 31 - k of m is candidate k.  A lane whose
+    // row is exhausted reads the sentinel block instead.  Reads stay below R + 32: a lane with
+    // p0 < rlen reads up to rbase + p0 + 31 <= R + 30.  Candidates past the lane's own rlen
+    // are masked off after the chunk.  A lane whose list would pass cap stores nothing more
+    // (its count still grows, so the caller rebuilds with a larger cap).
+    const uint32_t R = L.rpref[bp.nrc];
+    for (int dk = -1; dk <= 1; dk++)
+      for (int dj = -1; dj <= 1; dj++) {
+        const int rc0 = (int)A.rc + (dk * bp.RY + dj) * bp.RX - 1;
+        const uint32_t rbase = A.live ? L.rpref[rc0] : R;
+        const uint32_t rlen = A.live ? L.rpref[rc0 + 3] - rbase : 0u;
+        const uint32_t lmax = wave_max_b(rlen);
+        for (uint32_t p0 = 0; p0 < lmax; p0 += 32) {
+          const uint32_t np = min((lmax - p0 + 3) & ~3u, 32u);
+          const uint32_t lb = p0 < rlen ? rbase + p0 : R;
+          const double* pp = L.px + 3 * lb;
+          uint32_t m = 0;
+          for (uint32_t q = 0; q < np; q += 4) {
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++) {
+              const double dx = ax - pp[3 * (q + u)];
+              const double dy = ay - pp[3 * (q + u) + 1];
+              const double dz = az - pp[3 * (q + u) + 2];
+              const double d2 = (dx * dx + dy * dy) + dz * dz;
+              m = m + m + ((d2 > 0.0 && d2 <= max2) ? 1u : 0u);
+            }
+          }
+          if (np < 32) m <<= 32 - np;
+          const uint32_t own = p0 < rlen ? rlen - p0 : 0u;
+          if (own < 32) m &= ~(~0u >> own);
+          const uint32_t pc = (uint32_t)__popc(m);
+          if (cnt + pc > bp.cap) {
+            cnt += pc;
+            m = 0;
+          }
+          while (m) {
+            const uint32_t k = (uint32_t)__builtin_clz(m);
+            m ^= 0x80000000u >> k;
+            out[lpos(cnt, t, T)] = (uint16_t)(lb + k);
+            ++cnt;
+          }
+        }
+      }
+  } else {
   for (int dk = -1; dk <= 1; dk++)
     for (int dj = -1; dj <= 1; dj++)
       for (int di = -1; di <= 1; di++) {
@@ -295,6 +345,7 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
           }
         }
       }
+  }
   if (!A.live) cnt = 0;
   if (t < T) bcnt[(size_t)b * T + t] = cnt;
   const uint32_t wmax = wave_max_b(cnt);
@@ -709,9 +760,9 @@ uint32_t brick_layout(BrickParams& bp) {
   bp.off_bpref = o;
   o = align8(o + 4u * (bp.nbc + 1));
   bp.off_pos = o;
-  o += 24u * (bp.Rmax + 1);  // region atoms + the sentinel
+  o += 24u * (bp.Rmax + kSentinels);  // region atoms + the sentinel block
   bp.off_type = o;
-  o = align8(o + bp.Rmax + 1);
+  o = align8(o + bp.Rmax + kSentinels);
   return align8(o);
 }
 
