@@ -74,7 +74,7 @@ __device__ inline BrickLds brick_lds(const BrickParams& bp, char* lds) {
   L.rpref = (uint32_t*)(lds + bp.off_rpref);
   L.roff = (uint32_t*)(lds + bp.off_roff);
   L.bpref = (uint32_t*)(lds + bp.off_bpref);
-  L.px = (double*)(lds + bp.off_pos);  // interleaved x, y, z: atom i at [3 i, 3 i + 2]
+  L.px = (double*)lds;  // at off_pos == 0; interleaved x, y, z: atom i at [3 i, 3 i + 2]
   L.py = L.px + 1;
   L.pz = L.px + 2;
   L.pt = (uint8_t*)(lds + bp.off_type);
@@ -513,7 +513,7 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
   const unsigned ta = L.pt[At.ia];
   // this lane's parameters per neighbour type
   double sg[NT], e4[NT], m24[NT], rc2[NT];
-  float s2f[NT], e4f[NT], m24f[NT];
+  float s6f[NT], e4f[NT], m24f[NT];
 #pragma unroll
   for (int q = 0; q < NT; q++) {
     const unsigned pid = unique_pair_id(ta, (unsigned)q);
@@ -521,7 +521,7 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
     e4[q] = mm.eps4[pid];
     m24[q] = mm.m24eps[pid];
     rc2[q] = mm.rc2[pid];
-    s2f[q] = mm.sigma2f[pid];
+    s6f[q] = mm.sigma6f[pid];
     e4f[q] = mm.eps4f[pid];
     m24f[q] = mm.m24epsf[pid];
   }
@@ -594,25 +594,23 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
 #pragma unroll
       for (int u = 0; u < H; u++) {
         double psg = sg[0], pe4 = e4[0], pm24 = m24[0];
-        float ps2f = s2f[0], pe4f = e4f[0], pm24f = m24f[0];
+        float ps6f = s6f[0], pe4f = e4f[0], pm24f = m24f[0];
 #pragma unroll
         for (int q = 1; q < NT; q++)
           if (tb[u] == (unsigned)q) {
             psg = sg[q];
             pe4 = e4[q];
             pm24 = m24[q];
-            ps2f = s2f[q];
+            ps6f = s6f[q];
             pe4f = e4f[q];
             pm24f = m24f[q];
           }
         double coef, ce;  // force coefficient (f += coef * d) and pair energy
         if (MIXED) {
           const float inv_r2 = in[u] ? __builtin_amdgcn_rcpf((float)d2[u]) : 0.0f;  // 0 outside: coefficient and energy 0
-          const float s2 = ps2f * inv_r2;
-          const float s6 = s2 * s2 * s2;
-          const float s12 = s6 * s6;
-          coef = (double)(pm24f * (2.0f * s12 - s6) * inv_r2);
-          ce = ENERGY ? (double)(pe4f * (s12 - s6)) : 0.0;
+          float s6;
+          coef = (double)lj_coef_f32(inv_r2, ps6f, pm24f, s6);
+          ce = ENERGY ? (double)lj_energy_f32(s6, pe4f) : 0.0;
         } else {
           const double ratio = psg * inv_r[u];
           const double ratio2 = ratio * ratio;
@@ -750,7 +748,12 @@ static inline uint32_t align8(uint32_t x) { return (x + 7u) & ~7u; }
 
 // LDS layout of the build, force and conversion kernels; returns its bytes
 uint32_t brick_layout(BrickParams& bp) {
+  // positions first, at the start of dynamic LDS, so their addresses need no runtime base
   uint32_t o = 0;
+  bp.off_pos = o;
+  o += 24u * (bp.Rmax + kSentinels);  // region atoms + the sentinel block
+  bp.off_type = o;
+  o = align8(o + bp.Rmax + kSentinels);
   bp.off_rcnt = o;
   o = align8(o + 4u * bp.nrc);
   bp.off_rpref = o;
@@ -759,10 +762,6 @@ uint32_t brick_layout(BrickParams& bp) {
   o = align8(o + 4u * bp.nrc);
   bp.off_bpref = o;
   o = align8(o + 4u * (bp.nbc + 1));
-  bp.off_pos = o;
-  o += 24u * (bp.Rmax + kSentinels);  // region atoms + the sentinel block
-  bp.off_type = o;
-  o = align8(o + bp.Rmax + kSentinels);
   return align8(o);
 }
 

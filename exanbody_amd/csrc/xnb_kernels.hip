@@ -627,15 +627,12 @@ __global__ void __launch_bounds__(256) k_lj_multimat(uint32_t n_atoms, const dou
         const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
         if (d2 <= lj.rc2[pid]) {
           any = true;
-          const float inv_r2 = __builtin_amdgcn_rcpf((float)d2);
-          const float s2 = lj.sigma2f[pid] * inv_r2;
-          const float s6 = s2 * s2 * s2;
-          const float s12 = s6 * s6;
-          const double cd = (double)(lj.m24epsf[pid] * (2.0f * s12 - s6) * inv_r2);
+          float s6;
+          const double cd = (double)lj_coef_f32(__builtin_amdgcn_rcpf((float)d2), lj.sigma6f[pid], lj.m24epsf[pid], s6);
           tx = fma(cd, dx, tx);
           ty = fma(cd, dy, ty);
           tz = fma(cd, dz, tz);
-          te += (double)(lj.eps4f[pid] * (s12 - s6));
+          te += (double)lj_energy_f32(s6, lj.eps4f[pid]);
         }
       } else {
         const double d2 = (dx * dx + dy * dy) + dz * dz;

@@ -15,8 +15,17 @@ namespace xnb {
 
 struct LJConst {
   double sigma, eps4, m24eps, rc2;
-  float sigma2f, eps4f, m24epsf;
+  float sigma6f, eps4f, m24epsf;  // sigma^6, 4 eps, -24 eps for the mixed form
 };
+
+// FP32 part of the mixed form from x = 1/r^2 (0 outside the cutoff): s6 = sigma^6 x^3, the
+// coefficient de/r = -24 eps (2 s6 - 1) s6 x and the energy 4 eps (s6 - 1) s6, with fused
+// multiply-adds (the mixed form has a tolerance, not the reference's rounding).
+__device__ inline float lj_coef_f32(float x, float sigma6, float m24eps, float& s6) {
+  s6 = sigma6 * (x * x * x);
+  return (s6 * x) * fmaf(s6, 2.0f * m24eps, -m24eps);
+}
+__device__ inline float lj_energy_f32(float s6, float eps4) { return eps4 * fmaf(s6, s6, -s6); }
 
 template <bool ACCUM>
 __device__ inline void store_force(uint32_t s, double tx, double ty, double tz, double te, double* __restrict__ fx,
@@ -111,8 +120,8 @@ __device__ inline void lj_pair_fp64(const LJConst& lj, const double4& ra, const 
 }
 
 // One pair of the mixed form: FP64 displacement and cutoff test, FP32 coefficient
-// de/r = -24 eps (2 s^12 - s^6) / r^2 from (float)d2 with a hardware reciprocal, FP64
-// accumulation f = fma(c, d, f).
+// de/r = -24 eps (2 s^12 - s^6) / r^2 (lj_coef_f32) from (float)d2 with a hardware
+// reciprocal, FP64 accumulation f = fma(c, d, f).
 template <bool ENERGY = true>
 __device__ inline void lj_pair_mixed(const LJConst& lj, const double4& ra, const double4& rb, double& tx, double& ty, double& tz,
                                      double& te) {
@@ -123,14 +132,12 @@ __device__ inline void lj_pair_mixed(const LJConst& lj, const double4& ra, const
   const bool in = d2 > 0.0 && d2 <= lj.rc2;
   // outside the cutoff 1/r^2 = 0 makes the coefficient and the energy +-0
   const float inv_r2 = in ? __builtin_amdgcn_rcpf((float)d2) : 0.0f;
-  const float s2 = lj.sigma2f * inv_r2;
-  const float s6 = s2 * s2 * s2;
-  const float s12 = s6 * s6;
-  const double cd = (double)(lj.m24epsf * (2.0f * s12 - s6) * inv_r2);
+  float s6;
+  const double cd = (double)lj_coef_f32(inv_r2, lj.sigma6f, lj.m24epsf, s6);
   tx = fma(cd, dx, tx);
   ty = fma(cd, dy, ty);
   tz = fma(cd, dz, tz);
-  if (ENERGY) te += (double)(lj.eps4f * (s12 - s6));
+  if (ENERGY) te += (double)lj_energy_f32(s6, lj.eps4f);
 }
 
 // Multi-material LJ parameters per unique pair id (particle_type_pair.h: id of types a <= b
@@ -139,7 +146,7 @@ static constexpr int MM_PAIRS = 10;
 struct LJMulti {
   int npairs;
   double sigma[MM_PAIRS], eps4[MM_PAIRS], m24eps[MM_PAIRS], rc2[MM_PAIRS];
-  float sigma2f[MM_PAIRS], eps4f[MM_PAIRS], m24epsf[MM_PAIRS];
+  float sigma6f[MM_PAIRS], eps4f[MM_PAIRS], m24epsf[MM_PAIRS];
 };
 
 __host__ __device__ inline unsigned unique_pair_id(unsigned a, unsigned b) {
@@ -160,7 +167,7 @@ inline int make_lj_multi(int ntypes, const double* eps, const double* sigma, con
     m.eps4[q] = 4. * eps[q];
     m.m24eps[q] = -24. * eps[q];
     m.rc2[q] = rcut[q] * rcut[q];
-    m.sigma2f[q] = (float)(sigma[q] * sigma[q]);
+    m.sigma6f[q] = (float)(sigma[q] * sigma[q] * sigma[q] * sigma[q] * sigma[q] * sigma[q]);
     m.eps4f[q] = (float)(4. * eps[q]);
     m.m24epsf[q] = (float)(-24. * eps[q]);
   }
@@ -219,7 +226,7 @@ inline LJConst make_lj_const(double eps, double sigma, double rcut) {
   c.eps4 = 4. * eps;
   c.m24eps = -24. * eps;
   c.rc2 = rcut * rcut;
-  c.sigma2f = (float)(sigma * sigma);
+  c.sigma6f = (float)(sigma * sigma * sigma * sigma * sigma * sigma);
   c.eps4f = (float)(4. * eps);
   c.m24epsf = (float)(-24. * eps);
   return c;
