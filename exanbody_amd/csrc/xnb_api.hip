@@ -214,7 +214,7 @@ int scan(xnb_ctx* c, const uint32_t* in, uint32_t n, uint32_t add, uint32_t* out
 
 // ---- operators
 
-int op_move(xnb_ctx* c) {
+int op_move(xnb_ctx* c, bool forces = true) {
   const Geom& g = c->g;
   c->lists_valid = false;
   c->ghosts_valid = false;
@@ -230,8 +230,9 @@ int op_move(xnb_ctx* c) {
   int e;
   if ((e = scan(c, c->new_n.p, g.ncells, 0, c->new_off.p))) return e;
   launch_move_scatter(c->N, c->A, c->B, c->slot_cell_A.p, c->mv_dst.p, c->newloc.p, c->new_off.p, c->cell_n.p, c->cnt_out.p,
-                      c->in_cur.p, c->in_list.p, c->slot_cell_B.p, c->st);
-  launch_move_incoming(g, c->A, c->B, c->new_off.p, c->cell_n.p, c->cnt_out.p, c->cnt_in.p, c->in_list.p, c->slot_cell_B.p, c->st);
+                      c->in_cur.p, c->in_list.p, c->slot_cell_B.p, forces, c->st);
+  launch_move_incoming(g, c->A, c->B, c->new_off.p, c->cell_n.p, c->cnt_out.p, c->cnt_in.p, c->in_list.p, c->slot_cell_B.p, forces,
+                       c->st);
   launch_set_inner_cells(g, c->new_n.p, c->new_off.p, c->cell_n.p, c->cell_off.p, c->st);
   tstop(c);
   HIPCHK(c, hipGetLastError());
@@ -448,9 +449,9 @@ int launch_force_multimat(xnb_ctx* c, int precision, const LJMulti& mm, int ntyp
   return XNB_OK;
 }
 
-int full_update(xnb_ctx* c) {
+int full_update(xnb_ctx* c, bool forces) {
   int e;
-  if ((e = op_move(c))) return e;
+  if ((e = op_move(c, forces))) return e;
   if ((e = op_backup(c))) return e;
   if ((e = op_ghost_all(c))) return e;
   xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
@@ -1128,7 +1129,7 @@ int run_loop(xnb_ctx* c, xnb_run_params* p, const LJMulti* mm, int ntypes) {
       trigger = c->h_u64[0] > 0;
     }
     if (trigger) {
-      if ((e = full_update(c))) return e;
+      if ((e = full_update(c, false))) return e;  // f, ep: overwritten by this step's force kernel
       p->n_rebuilds++;
       if (interval) rebuilt[(size_t)step - 1] = 1;
     } else {
@@ -1137,9 +1138,12 @@ int run_loop(xnb_ctx* c, xnb_run_params* p, const LJMulti* mm, int ntypes) {
       tstop(c);
     }
     const bool fuse = c->fuse_push && step < p->steps && c->list_kind == 1;
-    // forces and energies of this step are read before being recomputed only when the next
-    // step rebuilds (move_particles and the ghost refresh copy them): known in interval mode
-    const bool lean = fuse && interval && ((step + 1) % p->rebuild_every) != 0;
+    // A fused step need not store forces and energies: the next step overwrites them before
+    // anything reads them (its force kernel sets f = 0 + sum for every inner atom; a rebuild in
+    // between only carries the stale values along, and the first half-step of the next step is
+    // done by this kernel from the force in registers).  The last step of a run is not fused
+    // and stores both.
+    const bool lean = fuse;
     NextStep nx{c->rB.p, c->backup.p, dt, dt2, thr2, cnt + 1};
     tstart(c, XNB_TIMER_FORCE);
     if (mm) e = launch_force_multimat(c, p->precision, *mm, ntypes, dth, fuse ? &nx : nullptr, lean);

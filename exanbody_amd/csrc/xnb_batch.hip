@@ -442,11 +442,11 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
       BatchBox& bb = B->box[i];
       if (bb.failed) continue;
       const int64_t k = bb.done + 1;
+      // a fused step stores no forces or energies: only the last step's are read (see run_loop)
       const bool fuse = k < bb.steps;
-      const bool lean = fuse && ((k + 1) % bb.every) != 0;
       frc.push_back(i);
       nbr.push_back(bb.f.bp.nbricks);
-      tag.push_back(fuse ? (lean ? (uint32_t)EPI_NEXT_LEAN : (uint32_t)EPI_NEXT) : (uint32_t)EPI_KICK);
+      tag.push_back(fuse ? (uint32_t)EPI_NEXT_LEAN : (uint32_t)EPI_KICK);
       Tf = std::max(Tf, bb.f.bp.T);
       ldsf = std::max(ldsf, brick_lds_bytes(bb.f.bp));
     }

@@ -136,7 +136,7 @@ int check_ctx(xnb_ctx* c);
 int op_ghost_all(xnb_ctx* c);
 int op_backup(xnb_ctx* c);
 int op_nbh(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& cfg, int all_cells);
-int full_update(xnb_ctx* c);
+int full_update(xnb_ctx* c, bool forces = true);  // forces false: f and ep not carried by the move
 void tstart(xnb_ctx* c, int id);
 void tstop(xnb_ctx* c);
 

@@ -196,11 +196,15 @@ __global__ void k_move_pack(Geom g, const uint32_t* __restrict__ cell_off, const
   move_pack_item(g, cell_off, cell_n, cnt_out, cnt_in, mv_dst, pk, newloc, new_n, c);
 }
 
-__device__ inline void copy_particle(const PState& A, uint32_t s, const PState& B, uint32_t t) {
+// forces false: f and ep are not carried (inside a run loop, where the next force kernel
+// overwrites both before anything reads them)
+__device__ inline void copy_particle(const PState& A, uint32_t s, const PState& B, uint32_t t, bool forces) {
   B.r[t] = A.r[s];
   B.vx[t] = A.vx[s]; B.vy[t] = A.vy[s]; B.vz[t] = A.vz[s];
-  B.fx[t] = A.fx[s]; B.fy[t] = A.fy[s]; B.fz[t] = A.fz[s];
-  B.ep[t] = A.ep[s];
+  if (forces) {
+    B.fx[t] = A.fx[s]; B.fy[t] = A.fy[s]; B.fz[t] = A.fz[s];
+    B.ep[t] = A.ep[s];
+  }
   B.id[t] = A.id[s];
 }
 
@@ -208,12 +212,12 @@ __device__ inline void move_scatter_item(uint32_t N, PState A, PState B, const u
                                const uint32_t* __restrict__ mv_dst, const uint32_t* __restrict__ newloc,
                                const uint32_t* __restrict__ new_off, const uint32_t* __restrict__ cell_n,
                                const uint32_t* __restrict__ cnt_out, uint32_t* __restrict__ in_cur,
-                               uint32_t* __restrict__ in_list, uint32_t* __restrict__ slot_cell_B, uint32_t s) {
+                               uint32_t* __restrict__ in_list, uint32_t* __restrict__ slot_cell_B, uint32_t s, bool forces) {
   const uint32_t c = slot_cell[s];
   const uint32_t dc = mv_dst[s];
   if (dc == NONE) {
     const uint32_t t = new_off[c] + newloc[s];
-    copy_particle(A, s, B, t);
+    copy_particle(A, s, B, t, forces);
     slot_cell_B[t] = c;
   } else {
     const uint32_t pos = atomicAdd(&in_cur[dc], 1u);
@@ -225,17 +229,17 @@ __global__ void k_move_scatter(uint32_t N, PState A, PState B, const uint32_t* _
                                const uint32_t* __restrict__ mv_dst, const uint32_t* __restrict__ newloc,
                                const uint32_t* __restrict__ new_off, const uint32_t* __restrict__ cell_n,
                                const uint32_t* __restrict__ cnt_out, uint32_t* __restrict__ in_cur,
-                               uint32_t* __restrict__ in_list, uint32_t* __restrict__ slot_cell_B) {
+                               uint32_t* __restrict__ in_list, uint32_t* __restrict__ slot_cell_B, int forces) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
-  move_scatter_item(N, A, B, slot_cell, mv_dst, newloc, new_off, cell_n, cnt_out, in_cur, in_list, slot_cell_B, s);
+  move_scatter_item(N, A, B, slot_cell, mv_dst, newloc, new_off, cell_n, cnt_out, in_cur, in_list, slot_cell_B, s, forces != 0);
 }
 
 // one thread per cell: order incoming movers by source slot, copy them with wrapped positions
 __device__ inline void move_incoming_item(Geom g, PState A, PState B, const uint32_t* __restrict__ new_off,
                                 const uint32_t* __restrict__ cell_n, const uint32_t* __restrict__ cnt_out,
                                 const uint32_t* __restrict__ cnt_in, uint32_t* __restrict__ in_list,
-                                uint32_t* __restrict__ slot_cell_B, uint32_t c) {
+                                uint32_t* __restrict__ slot_cell_B, uint32_t c, bool forces) {
   const uint32_t nin = cnt_in[c];
   if (nin == 0) return;
   const uint32_t b = new_off[c] + (cell_n[c] - cnt_out[c]);
@@ -250,7 +254,7 @@ __device__ inline void move_incoming_item(Geom g, PState A, PState B, const uint
   }
   for (uint32_t q = 0; q < nin; q++) {
     const uint32_t s = in_list[b + q], t = b + q;
-    copy_particle(A, s, B, t);
+    copy_particle(A, s, B, t, forces);
     double4 p = A.r[s];
     int li, lj, lk;
     periodic_location(g, p.x, p.y, p.z, li, lj, lk);
@@ -262,10 +266,10 @@ __device__ inline void move_incoming_item(Geom g, PState A, PState B, const uint
 __global__ void k_move_incoming(Geom g, PState A, PState B, const uint32_t* __restrict__ new_off,
                                 const uint32_t* __restrict__ cell_n, const uint32_t* __restrict__ cnt_out,
                                 const uint32_t* __restrict__ cnt_in, uint32_t* __restrict__ in_list,
-                                uint32_t* __restrict__ slot_cell_B) {
+                                uint32_t* __restrict__ slot_cell_B, int forces) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.ncells) return;
-  move_incoming_item(g, A, B, new_off, cell_n, cnt_out, cnt_in, in_list, slot_cell_B, c);
+  move_incoming_item(g, A, B, new_off, cell_n, cnt_out, cnt_in, in_list, slot_cell_B, c, forces != 0);
 }
 
 __device__ inline void set_inner_cells_item(uint32_t ncells, const uint32_t* __restrict__ new_n, const uint32_t* __restrict__ new_off,
@@ -888,14 +892,15 @@ void launch_move_pack(const Geom& g, const uint32_t* cell_off, const uint32_t* c
 }
 void launch_move_scatter(uint32_t N, const PState& A, const PState& B, const uint32_t* slot_cell, const uint32_t* mv_dst,
                          const uint32_t* newloc, const uint32_t* new_off, const uint32_t* cell_n, const uint32_t* cnt_out,
-                         uint32_t* in_cur, uint32_t* in_list, uint32_t* slot_cell_B, hipStream_t st) {
+                         uint32_t* in_cur, uint32_t* in_list, uint32_t* slot_cell_B, bool forces, hipStream_t st) {
   k_move_scatter<<<nblk(N, 256), 256, 0, st>>>(N, A, B, slot_cell, mv_dst, newloc, new_off, cell_n, cnt_out, in_cur, in_list,
-                                               slot_cell_B);
+                                               slot_cell_B, forces ? 1 : 0);
 }
 void launch_move_incoming(const Geom& g, const PState& A, const PState& B, const uint32_t* new_off, const uint32_t* cell_n,
                           const uint32_t* cnt_out, const uint32_t* cnt_in, uint32_t* in_list, uint32_t* slot_cell_B,
-                          hipStream_t st) {
-  k_move_incoming<<<nblk(g.ncells, 128), 128, 0, st>>>(g, A, B, new_off, cell_n, cnt_out, cnt_in, in_list, slot_cell_B);
+                          bool forces, hipStream_t st) {
+  k_move_incoming<<<nblk(g.ncells, 128), 128, 0, st>>>(g, A, B, new_off, cell_n, cnt_out, cnt_in, in_list, slot_cell_B,
+                                                        forces ? 1 : 0);
 }
 void launch_set_inner_cells(const Geom& g, const uint32_t* new_n, const uint32_t* new_off, uint32_t* cell_n, uint32_t* cell_off,
                             hipStream_t st) {
@@ -1108,7 +1113,8 @@ __global__ void __launch_bounds__(kBatchSlotBlock) kb_move_scatter(const BoxFram
   const BoxFrame& f = F[box];
   if (s >= f.N) return;
   const FrameState X = frame_state(f, par);
-  move_scatter_item(f.N, X.A, X.B, X.scA, f.mv_dst, f.newloc, f.new_off, f.cell_n, f.cnt_out, f.in_cur, f.in_list, X.scB, s);
+  move_scatter_item(f.N, X.A, X.B, X.scA, f.mv_dst, f.newloc, f.new_off, f.cell_n, f.cnt_out, f.in_cur, f.in_list, X.scB, s,
+                    false);  // inside a run: forces not carried
 }
 
 __global__ void __launch_bounds__(kBatchSlotBlock) kb_move_incoming(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
@@ -1118,7 +1124,7 @@ __global__ void __launch_bounds__(kBatchSlotBlock) kb_move_incoming(const BoxFra
   const BoxFrame& f = F[box];
   if (c >= f.g.ncells) return;
   const FrameState X = frame_state(f, par);
-  move_incoming_item(f.g, X.A, X.B, f.new_off, f.cell_n, f.cnt_out, f.cnt_in, f.in_list, X.scB, c);
+  move_incoming_item(f.g, X.A, X.B, f.new_off, f.cell_n, f.cnt_out, f.cnt_in, f.in_list, X.scB, c, false);
 }
 
 __global__ void __launch_bounds__(kBatchSlotBlock) kb_set_inner_cells(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work) {
