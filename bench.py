@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the headline force kernel k_lj_brick<MIXED, false, 3, 8> on this
 # workload: 2 x FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc, calibrated on the copy probe
 # (profiles/lj_c2_force_pmc.md)
-FORCE_TRAFFIC_BYTES = {"mixed": 3.105e9, "fp64": 3.074e9}
+FORCE_TRAFFIC_BYTES = {"mixed": 3.106e9, "fp64": 3.064e9}
 VALU_PEAK_TFLOPS = 78.6  # FP64 vector; scalar FP32 issues at the same rate (packed FP32 disabled)
 
 WORKLOADS = {
@@ -297,8 +297,8 @@ def main():
     roofline = dict(bound="mfma", achieved=achieved_tflops, peak=VALU_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=(achieved_tflops / VALU_PEAK_TFLOPS) if achieved_tflops else None, traffic=traffic,
                     traffic_gbs=traffic / (fk["avg_ms"] * 1e-3) / 1e9 if (traffic and fk["avg_ms"]) else None,
-                    kernel=f"k_lj_brick<{'true' if head_prec == 'mixed' else 'false'}, false, 2, 8> (LJ force + push_f_v + "
-                           "next step's push_f_v_r/push_f_v and displacement count)",
+                    kernel=f"k_lj_brick<{'true' if head_prec == 'mixed' else 'false'}, false, 3, 8> (LJ force + push_f_v + "
+                           "next step's push_f_v_r/push_f_v and displacement count, f and ep not stored)",
                     algorithmic_flops_per_atom=flops_per_atom, avg_launch_ms=fk["avg_ms"],
                     note="VALU pair kernel (no MFMA): flops = 8/list entry + 24/in-cutoff pair; peak = FP64 vector "
                          "rate, which scalar FP32 shares with packed FP32 disabled; traffic = measured HBM bytes per "

@@ -105,6 +105,8 @@ int set_geometry(xnb_ctx* c, double ghost_dist) {
   for (DArr<uint32_t>* a : {&c->cell_off, &c->cell_n, &c->cnt_out, &c->cnt_in, &c->new_n, &c->new_off, &c->in_cur, &c->gcnt,
                             &c->goff, &c->clen, &c->coff})
     if ((e = dalloc(c, *a, nc + 1))) return e;
+  // ghost counts are written for ghost cells only; inner cells keep this 0
+  HIPCHK(c, hipMemsetAsync(c->gcnt.p, 0, (nc + 1) * sizeof(uint32_t), c->st));
   return XNB_OK;
 }
 

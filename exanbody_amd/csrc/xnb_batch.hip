@@ -359,17 +359,18 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
       else quiet.push_back(i);
     }
     Tables T1;
-    std::vector<uint32_t> nN, nC, nBr;
+    std::vector<uint32_t> nN, nC, nGh, nBr;
     for (int i : fresh) nN.push_back(B->box[i].f.N);
     const size_t o_fresh = T1.add(blocks_table(fresh, nN));
     const size_t n_fresh = T1.v.size() - o_fresh;
-    size_t o_rs = 0, n_rs = 0, o_rc = 0, n_rc = 0, o_rb = 0, n_rb = 0, o_list = 0;
+    size_t o_rs = 0, n_rs = 0, o_rc = 0, n_rc = 0, o_rg = 0, n_rg = 0, o_rb = 0, n_rb = 0, o_list = 0;
     uint32_t Treb = 64, ldsreb = 0;
     if (!reb.empty()) {
       nN.clear();
       for (int i : reb) {
         nN.push_back(B->box[i].f.N);
         nC.push_back(B->box[i].f.g.ncells);
+        nGh.push_back(ghost_cell_count(B->box[i].f.g) * 64u);  // one wave per ghost cell
         nBr.push_back(B->box[i].f.bp.nbricks);
         Treb = std::max(Treb, B->box[i].f.bp.T);
         ldsreb = std::max(ldsreb, brick_lds_bytes(B->box[i].f.bp));
@@ -378,6 +379,8 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
       n_rs = T1.v.size() - o_rs;
       o_rc = T1.add(blocks_table(reb, nC));
       n_rc = T1.v.size() - o_rc;
+      o_rg = T1.add(blocks_table(reb, nGh));
+      n_rg = T1.v.size() - o_rg;
       std::vector<uint32_t> zero(reb.size(), 0u);
       o_rb = T1.add(bricks_table(reb, nBr, zero));
       n_rb = T1.v.size() - o_rb;
@@ -397,7 +400,7 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
         B->box[i].f.rcur ^= 1u;
         B->box[i].f.G = 0;
       }
-      launch_batch_ghosts(B->dF.p, W + o_rs, (uint32_t)n_rs, W + o_rc, (uint32_t)n_rc, W + o_list, (uint32_t)reb.size(), par, B->st);
+      launch_batch_ghosts(B->dF.p, W + o_rs, (uint32_t)n_rs, W + o_rg, (uint32_t)n_rg, W + o_list, (uint32_t)reb.size(), par, B->st);
       hipError_t he = launch_batch_nbh(B->dF.p, W + o_rb, (uint32_t)n_rb, Treb, ldsreb, par, B->st);
       if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched neighbour build: ") + hipGetErrorString(he));
       BHIP(B, hipMemcpyAsync(B->hwords, B->dwords.p, 4 * kMaxBatchBoxes * sizeof(uint32_t), hipMemcpyDeviceToHost, B->st));

@@ -44,6 +44,41 @@ __host__ __device__ inline bool is_ghost(const Geom& g, int i, int j, int k) {
 
 __host__ __device__ inline int pmod(int i, int n) { return ((i % n) + n) % n; }
 
+// Ghost cells (the gl-thick shell of the grid) enumerated as: the two k slabs (all i, j),
+// then the two j slabs of the inner k range (all i), then the two i slabs of the inner j and
+// k ranges.
+__host__ __device__ inline uint32_t ghost_cell_count(const Geom& g) {
+  const uint32_t gl = (uint32_t)g.gl, di = (uint32_t)g.di, dj = (uint32_t)g.dj, dk = (uint32_t)g.dk;
+  const uint32_t km = dk - 2 * gl, jm = dj - 2 * gl;
+  return 2 * gl * di * dj + 2 * gl * di * km + 2 * gl * jm * km;
+}
+__host__ __device__ inline uint32_t ghost_cell_at(const Geom& g, uint32_t q) {
+  const uint32_t gl = (uint32_t)g.gl, di = (uint32_t)g.di, dj = (uint32_t)g.dj, dk = (uint32_t)g.dk;
+  const uint32_t km = dk - 2 * gl, jm = dj - 2 * gl;
+  uint32_t i, j, k;
+  const uint32_t A = gl * di * dj, B = gl * di * km, C = gl * jm * km;
+  if (q < 2 * A) {
+    const uint32_t hi = q >= A ? 1u : 0u, r = q - hi * A;
+    i = r % di;
+    j = (r / di) % dj;
+    k = r / (di * dj) + hi * (dk - gl);
+  } else if (q < 2 * A + 2 * B) {
+    q -= 2 * A;
+    const uint32_t hi = q >= B ? 1u : 0u, r = q - hi * B;
+    i = r % di;
+    j = (r / di) % gl + hi * (dj - gl);
+    k = gl + r / (di * gl);
+  } else {
+    q -= 2 * A + 2 * B;
+    const uint32_t hi = q >= C ? 1u : 0u, r = q - hi * C;
+    i = r % gl + hi * (di - gl);
+    j = gl + (r / gl) % jm;
+    k = gl + r / (gl * jm);
+  }
+  return cell_index(g, (int)i, (int)j, (int)k);
+}
+
+
 // domain_periodic_location (src/core/include/exanb/core/domain.h:160-191).
 // Returns the domain cell location and wraps x on periodic axes.
 __host__ __device__ inline void periodic_location(const Geom& g, double& x, double& y, double& z,

@@ -89,7 +89,7 @@ void launch_batch_zero_cells(const BoxFrame* F, const uint32_t* work, uint32_t n
 void launch_batch_zero_words(uint32_t* words, const uint32_t* boxes, uint32_t n, hipStream_t st);
 void launch_batch_move(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wcells, uint32_t ncells,
                        const uint32_t* boxes, uint32_t nboxes, uint32_t par, hipStream_t st);
-void launch_batch_ghosts(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wcells, uint32_t ncells,
+void launch_batch_ghosts(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wghost, uint32_t nghost,
                          const uint32_t* boxes, uint32_t nboxes, uint32_t par, hipStream_t st);
 void launch_batch_ghost_update_r(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t par, hipStream_t st);
 void launch_batch_first_half(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t par, uint32_t gstep, hipStream_t st);

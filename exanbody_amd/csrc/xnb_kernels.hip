@@ -317,37 +317,48 @@ __device__ inline bool ghost_keep(const Geom& g, double4 p, int si, int sj, int 
   return q.x >= bx0 && q.x <= bx1 && q.y >= by0 && q.y <= by1 && q.z >= bz0 && q.z <= bz1;
 }
 
-__device__ inline void ghost_count_item(Geom g, const double4* __restrict__ r, const uint32_t* __restrict__ cell_off,
-                              const uint32_t* __restrict__ cell_n, uint32_t* __restrict__ gcnt, uint32_t c) {
+// Ghost cells are handled one wave per cell (only the ghost cells are visited; gcnt of inner
+// cells stays 0 from allocation): lanes take the source cell's particles 64 at a time and the
+// kept ones are compacted in source order by a ballot prefix.
+__device__ inline void ghost_count_wave(const Geom& g, const double4* __restrict__ r, const uint32_t* __restrict__ cell_off,
+                                        const uint32_t* __restrict__ cell_n, uint32_t* __restrict__ gcnt, uint32_t c) {
+  const uint32_t lane = threadIdx.x & 63u;
   uint32_t src;
   int si, sj, sk;
   uint32_t cnt = 0;
   if (ghost_source(g, c, src, si, sj, sk)) {
     const uint32_t off = cell_off[src], n = cell_n[src];
-    for (uint32_t q = 0; q < n; q++) {
+    for (uint32_t q0 = 0; q0 < n; q0 += 64) {
       double4 gp;
-      if (ghost_keep(g, r[off + q], si, sj, sk, gp)) ++cnt;
+      const bool keep = q0 + lane < n && ghost_keep(g, r[off + q0 + lane], si, sj, sk, gp);
+      cnt += (uint32_t)__popcll(__ballot(keep));
     }
   }
-  gcnt[c] = cnt;
+  if (lane == 0) gcnt[c] = cnt;
 }
 
-__global__ void k_ghost_count(Geom g, const double4* __restrict__ r, const uint32_t* __restrict__ cell_off,
-                              const uint32_t* __restrict__ cell_n, uint32_t* __restrict__ gcnt) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.ncells) return;
-  ghost_count_item(g, r, cell_off, cell_n, gcnt, c);
+// wave w of the launch -> ghost cell w (wave-uniform; blockDim.x is a multiple of 64)
+__device__ inline bool ghost_wave_cell(const Geom& g, uint32_t thread, uint32_t& c) {
+  const uint32_t w = thread >> 6;
+  if (w >= ghost_cell_count(g)) return false;
+  c = ghost_cell_at(g, w);
+  return true;
 }
 
-// Ghost cell c of one box.  Slots past cap are not written: the cell count is still stored
-// and *status gets bit 2, so the caller sizes the buffers and runs the fill again.
-__device__ inline void ghost_fill_item(const Geom& g, uint32_t N, const PState& B, const uint32_t* __restrict__ goff,
+__global__ void __launch_bounds__(256) k_ghost_count(Geom g, const double4* __restrict__ r, const uint32_t* __restrict__ cell_off,
+                                                     const uint32_t* __restrict__ cell_n, uint32_t* __restrict__ gcnt) {
+  uint32_t c;
+  if (!ghost_wave_cell(g, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
+  ghost_count_wave(g, r, cell_off, cell_n, gcnt, c);
+}
+
+// Ghost cell c of one box (one wave).  Slots past cap are not written: the cell count is still
+// stored and *status gets bit 2, so the caller sizes the buffers and runs the fill again.
+__device__ inline void ghost_fill_wave(const Geom& g, uint32_t N, const PState& B, const uint32_t* __restrict__ goff,
                                        uint32_t* __restrict__ cell_off, uint32_t* __restrict__ cell_n, uint32_t* __restrict__ slot_cell,
                                        uint32_t* __restrict__ ghost_src, uint32_t* __restrict__ ghost_shift, uint32_t cap,
                                        uint32_t* __restrict__ status, uint32_t c) {
-  int i, j, k;
-  cell_ijk(g, c, i, j, k);
-  if (!is_ghost(g, i, j, k)) return;
+  const uint32_t lane = threadIdx.x & 63u;
   uint32_t src;
   int si, sj, sk;
   const uint32_t base = N + goff[c];
@@ -355,35 +366,41 @@ __device__ inline void ghost_fill_item(const Geom& g, uint32_t N, const PState& 
   if (ghost_source(g, c, src, si, sj, sk)) {
     const uint32_t off = cell_off[src], n = cell_n[src];
     const uint32_t code = pack_shift(si, sj, sk);
-    for (uint32_t q = 0; q < n; q++) {
+    for (uint32_t q0 = 0; q0 < n; q0 += 64) {
+      const uint32_t s = off + q0 + lane;
       double4 gp;
-      if (ghost_keep(g, B.r[off + q], si, sj, sk, gp)) {
-        const uint32_t t = base + cnt;
-        ++cnt;
+      const bool keep = q0 + lane < n && ghost_keep(g, B.r[s], si, sj, sk, gp);
+      const unsigned long long m = __ballot(keep);
+      if (keep) {
+        const uint32_t t = base + cnt + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         if (t >= cap) {
           atomicOr(status, 4u);
-          continue;
+        } else {
+          B.r[t] = gp;
+          B.vx[t] = B.vx[s]; B.vy[t] = B.vy[s]; B.vz[t] = B.vz[s];
+          B.fx[t] = 0.; B.fy[t] = 0.; B.fz[t] = 0.; B.ep[t] = 0.;
+          B.id[t] = B.id[s];
+          slot_cell[t] = c;
+          ghost_src[t - N] = s;
+          ghost_shift[t - N] = code;
         }
-        B.r[t] = gp;
-        B.vx[t] = B.vx[off + q]; B.vy[t] = B.vy[off + q]; B.vz[t] = B.vz[off + q];
-        B.fx[t] = 0.; B.fy[t] = 0.; B.fz[t] = 0.; B.ep[t] = 0.;
-        B.id[t] = B.id[off + q];
-        slot_cell[t] = c;
-        ghost_src[t - N] = off + q;
-        ghost_shift[t - N] = code;
       }
+      cnt += (uint32_t)__popcll(m);
     }
   }
-  cell_off[c] = base;
-  cell_n[c] = cnt;
+  if (lane == 0) {
+    cell_off[c] = base;
+    cell_n[c] = cnt;
+  }
 }
 
-__global__ void k_ghost_fill(Geom g, uint32_t N, PState B, const uint32_t* __restrict__ goff, uint32_t* __restrict__ cell_off,
-                             uint32_t* __restrict__ cell_n, uint32_t* __restrict__ slot_cell, uint32_t* __restrict__ ghost_src,
-                             uint32_t* __restrict__ ghost_shift, uint32_t cap, uint32_t* __restrict__ status) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.ncells) return;
-  ghost_fill_item(g, N, B, goff, cell_off, cell_n, slot_cell, ghost_src, ghost_shift, cap, status, c);
+__global__ void __launch_bounds__(256) k_ghost_fill(Geom g, uint32_t N, PState B, const uint32_t* __restrict__ goff,
+                                                    uint32_t* __restrict__ cell_off, uint32_t* __restrict__ cell_n,
+                                                    uint32_t* __restrict__ slot_cell, uint32_t* __restrict__ ghost_src,
+                                                    uint32_t* __restrict__ ghost_shift, uint32_t cap, uint32_t* __restrict__ status) {
+  uint32_t c;
+  if (!ghost_wave_cell(g, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
+  ghost_fill_wave(g, N, B, goff, cell_off, cell_n, slot_cell, ghost_src, ghost_shift, cap, status, c);
 }
 
 // ghost_update_r (update_ghosts.cu:58): re-copy shifted positions along the stored scheme
@@ -908,11 +925,12 @@ void launch_set_inner_cells(const Geom& g, const uint32_t* new_n, const uint32_t
 }
 void launch_ghost_count(const Geom& g, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, uint32_t* gcnt,
                         hipStream_t st) {
-  k_ghost_count<<<nblk(g.ncells, 128), 128, 0, st>>>(g, r, cell_off, cell_n, gcnt);
+  k_ghost_count<<<nblk(ghost_cell_count(g) * 64u, 256), 256, 0, st>>>(g, r, cell_off, cell_n, gcnt);
 }
 void launch_ghost_fill(const Geom& g, uint32_t N, const PState& B, const uint32_t* goff, uint32_t* cell_off, uint32_t* cell_n,
                        uint32_t* slot_cell, uint32_t* ghost_src, uint32_t* ghost_shift, uint32_t cap, uint32_t* status, hipStream_t st) {
-  k_ghost_fill<<<nblk(g.ncells, 128), 128, 0, st>>>(g, N, B, goff, cell_off, cell_n, slot_cell, ghost_src, ghost_shift, cap, status);
+  k_ghost_fill<<<nblk(ghost_cell_count(g) * 64u, 256), 256, 0, st>>>(g, N, B, goff, cell_off, cell_n, slot_cell, ghost_src,
+                                                                       ghost_shift, cap, status);
 }
 void launch_ghost_update_r(const Geom& g, uint32_t N, uint32_t G, double4* r, const uint32_t* ghost_src,
                            const uint32_t* ghost_shift, hipStream_t st) {
@@ -1155,24 +1173,25 @@ __global__ void __launch_bounds__(kBatchSlotBlock) kb_backup_r(const BoxFrame* _
   backup_r_item(f.g, f.N, X.A.r, X.scA, f.backup, s);
 }
 
+// work: blocks of kBatchSlotBlock threads over ghost_cell_count * 64 threads per box
 __global__ void __launch_bounds__(kBatchSlotBlock) kb_ghost_count(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
                                                                    uint32_t par) {
-  uint32_t box, c;
-  if (!batch_item(work, box, c)) return;
+  uint32_t box, idx, c;
+  if (!batch_item(work, box, idx)) return;
   const BoxFrame& f = F[box];
-  if (c >= f.g.ncells) return;
+  if (!ghost_wave_cell(f.g, idx, c)) return;
   const FrameState X = frame_state(f, par);
-  ghost_count_item(f.g, X.A.r, f.cell_off, f.cell_n, f.gcnt, c);
+  ghost_count_wave(f.g, X.A.r, f.cell_off, f.cell_n, f.gcnt, c);
 }
 
 __global__ void __launch_bounds__(kBatchSlotBlock) kb_ghost_fill(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
                                                                   uint32_t par) {
-  uint32_t box, c;
-  if (!batch_item(work, box, c)) return;
+  uint32_t box, idx, c;
+  if (!batch_item(work, box, idx)) return;
   const BoxFrame& f = F[box];
-  if (c >= f.g.ncells) return;
+  if (!ghost_wave_cell(f.g, idx, c)) return;
   const FrameState X = frame_state(f, par);
-  ghost_fill_item(f.g, f.N, X.A, f.goff, f.cell_off, f.cell_n, X.scA, f.ghost_src, f.ghost_shift, f.cap, f.words + 0, c);
+  ghost_fill_wave(f.g, f.N, X.A, f.goff, f.cell_off, f.cell_n, X.scA, f.ghost_src, f.ghost_shift, f.cap, f.words + 0, c);
 }
 
 __global__ void __launch_bounds__(kBatchSlotBlock) kb_ghost_update_r(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
@@ -1224,13 +1243,13 @@ void launch_batch_move(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, con
   kb_flip<<<(nboxes + 255) / 256, 256, 0, st>>>(F, boxes, nboxes);
 }
 
-void launch_batch_ghosts(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wcells, uint32_t ncells,
+void launch_batch_ghosts(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wghost, uint32_t nghost,
                          const uint32_t* boxes, uint32_t nboxes, uint32_t par, hipStream_t st) {
   if (!nboxes) return;
   if (nslots) kb_backup_r<<<nslots, kBatchSlotBlock, 0, st>>>(F, wslots, par);
-  if (ncells) kb_ghost_count<<<ncells, kBatchSlotBlock, 0, st>>>(F, wcells, par);
+  if (nghost) kb_ghost_count<<<nghost, kBatchSlotBlock, 0, st>>>(F, wghost, par);
   kb_scan_cells<<<nboxes, 1024, 0, st>>>(F, boxes, 1);
-  if (ncells) kb_ghost_fill<<<ncells, kBatchSlotBlock, 0, st>>>(F, wcells, par);
+  if (nghost) kb_ghost_fill<<<nghost, kBatchSlotBlock, 0, st>>>(F, wghost, par);
 }
 
 void launch_batch_ghost_update_r(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t par, hipStream_t st) {
