@@ -82,6 +82,25 @@ def _with_bricks(monkeypatch, value):
         monkeypatch.setenv("XNB_BRICKS", value)
 
 
+@pytest.mark.parametrize("periodic,min_cell,steps", [((1, 1, 0), RCUT + SKIN, 0), ((1, 1, 1), 0.5 * (RCUT + SKIN) + 0.05, 20)])
+def test_ghosts_open_axis_and_two_layers(periodic, min_cell, steps):
+    """Ghost cells along a non-periodic axis stay empty; with cells thinner than the ghost
+    distance two ghost layers are filled.  Layout with ghost images and FP64 forces equal the
+    oracle's, also after a run with rebuilds (two-layer case)."""
+    r, v, ids, L = fcc_inputs(7, temperature=0.72, seed=31)
+    o, b, dom = make_pair(r, v, ids, L, min_cell=min_cell, periodic=periodic)
+    o.init(1.0, 1.0, RCUT, SKIN)
+    b.init(1.0, 1.0, RCUT, SKIN, PRECISION_FP64)
+    assert_same_layout(o.export(ghosts=True), b.export(ghosts=True), "inner+ghost")
+    np.testing.assert_array_equal(o.export()["f"].view(np.uint64), b.export()["f"].view(np.uint64))
+    if steps:
+        ro = o.run(steps, 0.005, 1.0, 1.0, RCUT, SKIN, rebuild_every=5)
+        rg = b.run(steps, 0.005, 1.0, 1.0, RCUT, SKIN, precision=PRECISION_FP64, rebuild_every=5)
+        assert ro == rg
+        assert_same_layout(o.export(ghosts=True), b.export(ghosts=True), "inner+ghost after run")
+        np.testing.assert_array_equal(o.export()["f"].view(np.uint64), b.export()["f"].view(np.uint64))
+
+
 @pytest.mark.parametrize("bricks,fuse", [("0", "1"), ("1,1,1", "1"), ("2,2,1", "1"), ("4,2,2", "1"), ("3,2,2", "1"),
                                          ("4,4,2", "1"), ("4,4,2", "0")])
 def test_list_paths_bitwise(monkeypatch, bricks, fuse):
