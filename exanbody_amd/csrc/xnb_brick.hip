@@ -427,19 +427,35 @@ __device__ __forceinline__ void lj_brick_body(const Geom& g, const BrickParams& 
       j[4 * w + 2] = wc[w].y & 0xffffu;
       j[4 * w + 3] = wc[w].y >> 16;
     }
-    double4 rb[U];  // entries past a lane's count are the sentinel (rejected by the cutoff)
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      rb[u].x = L.px[3 * j[u]];
-      rb[u].y = L.py[3 * j[u]];
-      rb[u].z = L.pz[3 * j[u]];
-      rb[u].w = 0.;
-    }
+    // entries past a lane's count are the sentinel (rejected by the cutoff)
     if (MIXED) {
+      double4 rb[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        rb[u].x = L.px[3 * j[u]];
+        rb[u].y = L.py[3 * j[u]];
+        rb[u].z = L.pz[3 * j[u]];
+        rb[u].w = 0.;
+      }
 #pragma unroll
       for (int u = 0; u < U; u++) lj_pair_mixed<EPI != EPI_NEXT_LEAN>(lj, ra, rb[u], tx, ty, tz, te);
     } else {
-      lj_batch_fp64<U, EPI != EPI_NEXT_LEAN>(lj, ra, rb, tx, ty, tz, te);
+      // FP64: pairs by two, each pair's positions read just before its arithmetic, and the
+      // scheduler kept from hoisting the next pair's reads (with k_lj_brick's register cap:
+      // C2 FP64 force 1.91 -> 1.78 ms against eight pairs read up front)
+#pragma unroll
+      for (int h = 0; h < U; h += 2) {
+        double4 rb[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          rb[u].x = L.px[3 * j[h + u]];
+          rb[u].y = L.py[3 * j[h + u]];
+          rb[u].z = L.pz[3 * j[h + u]];
+          rb[u].w = 0.;
+        }
+        lj_batch_fp64<2, EPI != EPI_NEXT_LEAN>(lj, ra, rb, tx, ty, tz, te);
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
   }
   constexpr bool NEXT = EPI == EPI_NEXT || EPI == EPI_NEXT_LEAN;
@@ -478,7 +494,9 @@ __device__ __forceinline__ void lj_brick_body(const Geom& g, const BrickParams& 
 }
 
 template <bool MIXED, bool ACCUM, int EPI, int U>
-__global__ void __launch_bounds__(1024) k_lj_brick(Geom g, BrickParams bp, const double4* __restrict__ r,
+// waves_per_eu(5) caps registers at 96 (a few loop-invariant values spill): measured faster for
+// the FP64 instantiations, neutral for the mixed ones
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) k_lj_brick(Geom g, BrickParams bp, const double4* __restrict__ r,
                                                    const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
                                                    const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt,
                                                    LJConst lj, PState A, double dth, NextStep nx) {
