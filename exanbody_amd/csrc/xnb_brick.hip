@@ -263,6 +263,10 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
     // are masked off after the chunk.  A lane whose list would pass cap stores nothing more
     // (its count still grows, so the caller rebuilds with a larger cap).
     const uint32_t R = L.rpref[bp.nrc];
+    // byte offset of entry cnt in the slab, 2 * lpos(cnt, t, T), advanced per stored entry
+    char* const ob = reinterpret_cast<char*>(out);
+    uint32_t wo = 8u * t;
+    const uint32_t wstep = 8u * T - 6u;
     for (int dk = -1; dk <= 1; dk++)
       for (int dj = -1; dj <= 1; dj++) {
         const int rc0 = (int)A.rc + (dk * bp.RY + dj) * bp.RX - 1;
@@ -295,8 +299,9 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
           while (m) {
             const uint32_t k = (uint32_t)__builtin_clz(m);
             m ^= 0x80000000u >> k;
-            out[lpos(cnt, t, T)] = (uint16_t)(lb + k);
+            *reinterpret_cast<uint16_t*>(ob + wo) = (uint16_t)(lb + k);
             ++cnt;
+            wo += (cnt & 3u) ? 2u : wstep;
           }
         }
       }
