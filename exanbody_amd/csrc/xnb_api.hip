@@ -199,10 +199,10 @@ void swap_buffers(xnb_ctx* c) {
   bind_states(c);
 }
 
-int read_u32(xnb_ctx* c, const uint32_t* dptr, uint32_t* out) {
-  HIPCHK(c, hipMemcpyAsync(c->h_u32, dptr, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+int read_u32(xnb_ctx* c, const uint32_t* dptr, uint32_t* out, int n = 1) {
+  HIPCHK(c, hipMemcpyAsync(c->h_u32, dptr, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
-  *out = c->h_u32[0];
+  for (int q = 0; q < n; q++) out[q] = c->h_u32[q];
   return XNB_OK;
 }
 
@@ -216,7 +216,9 @@ int scan(xnb_ctx* c, const uint32_t* in, uint32_t n, uint32_t add, uint32_t* out
 
 // ---- operators
 
-int op_move(xnb_ctx* c, bool forces = true) {
+// check false: the out-of-domain flag (u32s[0]) is left for the caller to read (op_ghost_all
+// reads it with the ghost count in one transfer)
+int op_move(xnb_ctx* c, bool forces = true, bool check = true) {
   const Geom& g = c->g;
   c->lists_valid = false;
   c->ghosts_valid = false;
@@ -239,7 +241,7 @@ int op_move(xnb_ctx* c, bool forces = true) {
   tstop(c);
   HIPCHK(c, hipGetLastError());
   uint32_t errflag = 0;
-  if ((e = read_u32(c, c->u32s.p, &errflag))) return e;
+  if (check && (e = read_u32(c, c->u32s.p, &errflag))) return e;
   swap_buffers(c);
   // ghost cells are empty until ghost_update_all rebuilds them
   c->G = 0;
@@ -247,15 +249,18 @@ int op_move(xnb_ctx* c, bool forces = true) {
   return XNB_OK;
 }
 
-int op_ghost_all(xnb_ctx* c) {
+// move_check: also report the out-of-domain flag of a preceding op_move(c, ..., false)
+int op_ghost_all(xnb_ctx* c, bool move_check) {
   const Geom& g = c->g;
   int e;
   tstart(c, XNB_TIMER_GHOST);
   launch_ghost_count(g, c->A.r, c->cell_off.p, c->cell_n.p, c->gcnt.p, c->st);
   if ((e = scan(c, c->gcnt.p, g.ncells, 0, c->goff.p))) return e;
   tstop(c);
-  uint32_t G = 0;
-  if ((e = read_u32(c, c->u32s.p + 1, &G))) return e;
+  uint32_t w[2] = {0, 0};  // u32s[0]: out-of-domain flag, u32s[1]: scan total = ghost count
+  if ((e = read_u32(c, c->u32s.p, w, 2))) return e;
+  if (move_check && w[0]) return fail(c, XNB_E_OUTSIDE, "a particle left the domain along a non-periodic axis");
+  const uint32_t G = w[1];
   if ((e = ensure_capacity(c, (size_t)c->N + G, c->N))) return e;
   tstart(c, XNB_TIMER_GHOST);
   launch_ghost_fill(g, c->N, c->A, c->goff.p, c->cell_off.p, c->cell_n.p, c->slot_cell_A.p, c->ghost_src.p, c->ghost_shift.p,
@@ -340,9 +345,9 @@ int op_nbh_brick(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& 
                                      c->blist.p, c->bcnt.p, c->u32s.p + 2, c->st);
     tstop(c);
     if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick neighbour build: ") + hipGetErrorString(he));
-    uint32_t mx = 0, bad = 0;
-    if ((e = read_u32(c, c->u32s.p + 2, &mx))) return e;
-    if ((e = read_u32(c, c->u32s.p + 6, &bad))) return e;
+    uint32_t w[5];  // u32s[2]: longest list, u32s[6]: brick status
+    if ((e = read_u32(c, c->u32s.p + 2, w, 5))) return e;
+    const uint32_t mx = w[0], bad = w[4];
     if (bad) return fail(c, XNB_E_STATE, "brick sizes changed between statistics and build");
     c->max_neighbors = mx;
     if (mx <= bp.cap) {
@@ -453,9 +458,9 @@ int launch_force_multimat(xnb_ctx* c, int precision, const LJMulti& mm, int ntyp
 
 int full_update(xnb_ctx* c, bool forces) {
   int e;
-  if ((e = op_move(c, forces))) return e;
+  if ((e = op_move(c, forces, false))) return e;
   if ((e = op_backup(c))) return e;
-  if ((e = op_ghost_all(c))) return e;
+  if ((e = op_ghost_all(c, true))) return e;
   xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
   return op_nbh(c, c->nbh_dist_lab, cfg, 0);
 }
