@@ -25,7 +25,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "LJ atom-steps/s (rc=2.5σ) mixed vs FP64 on 1 GPU; batched p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the headline force kernel k_lj_brick<MIXED, false, 3, 8> on this
+# HBM bytes per launch of the headline force kernel k_lj_brick<MIXED, false, 2, 8> on this
 # workload: 2 x FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc, calibrated on the copy probe
 # (profiles/lj_c2_force_pmc.md)
 FORCE_TRAFFIC_BYTES = {"mixed": 3.106e9, "fp64": 3.215e9}
@@ -300,7 +300,7 @@ def main():
     roofline = dict(bound="mfma", achieved=achieved_tflops, peak=VALU_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=(achieved_tflops / VALU_PEAK_TFLOPS) if achieved_tflops else None, traffic=traffic,
                     traffic_gbs=traffic / (fk["avg_ms"] * 1e-3) / 1e9 if (traffic and fk["avg_ms"]) else None,
-                    kernel=f"k_lj_brick<{'true' if head_prec == 'mixed' else 'false'}, false, 3, 8> (LJ force + push_f_v + "
+                    kernel=f"k_lj_brick<{'true' if head_prec == 'mixed' else 'false'}, false, 2, 8> (LJ force + push_f_v + "
                            "next step's push_f_v_r/push_f_v and displacement count, f and ep not stored)",
                     algorithmic_flops_per_atom=flops_per_atom, avg_launch_ms=fk["avg_ms"],
                     note="VALU pair kernel (no MFMA): flops = 8/list entry + 24/in-cutoff pair; peak = FP64 vector "

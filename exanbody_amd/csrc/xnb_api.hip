@@ -421,9 +421,9 @@ int ensure_slot_lists(xnb_ctx* c) {
 // With next (brick lists only, no accum), the kernel also advances the next step's first
 // half into buffer B's position array, which the caller then swaps in.
 int launch_force(xnb_ctx* c, int precision, bool accum, bool push, bool ghost, double eps, double sigma, double rcut, double dth,
-                 const NextStep* next = nullptr, bool lean = false) {
+                 const NextStep* next = nullptr) {
   if (c->list_kind == 1 && !ghost) {
-    const int epi = next ? (lean ? EPI_NEXT_LEAN : EPI_NEXT) : (push ? EPI_KICK : EPI_NONE);
+    const int epi = next ? EPI_NEXT : (push ? EPI_KICK : EPI_NONE);
     hipError_t he = launch_lj_brick(precision, accum, epi, c->g, c->bp, c->A, c->cell_off.p, c->cell_n.p, c->blist.p, c->bcnt.p, eps,
                                     sigma, rcut, dth, next, c->st);
     if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick LJ kernel: ") + hipGetErrorString(he));
@@ -439,9 +439,9 @@ int launch_force(xnb_ctx* c, int precision, bool accum, bool push, bool ghost, d
 
 // Multimat force of a run-loop step: zero + compute_pair_multimat (+ the epilogue) in one brick
 // kernel, or, on generic lists, the separate kernels (no fusion).
-int launch_force_multimat(xnb_ctx* c, int precision, const LJMulti& mm, int ntypes, double dth, const NextStep* next, bool lean) {
+int launch_force_multimat(xnb_ctx* c, int precision, const LJMulti& mm, int ntypes, double dth, const NextStep* next) {
   if (c->list_kind == 1) {
-    const int epi = next ? (lean ? EPI_NEXT_LEAN : EPI_NEXT) : EPI_KICK;
+    const int epi = next ? EPI_NEXT : EPI_KICK;
     hipError_t he = launch_ljmm_brick(precision, ntypes, epi, c->g, c->bp, c->A, c->cell_off.p, c->cell_n.p, c->blist.p, c->bcnt.p, mm,
                                       dth, next, c->st);
     if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick multimat kernel: ") + hipGetErrorString(he));
@@ -1145,16 +1145,14 @@ int run_loop(xnb_ctx* c, xnb_run_params* p, const LJMulti* mm, int ntypes) {
       tstop(c);
     }
     const bool fuse = c->fuse_push && step < p->steps && c->list_kind == 1;
-    // A fused step need not store forces and energies: the next step overwrites them before
+    // A fused step (EPI_NEXT) stores no forces or energies: the next step overwrites them before
     // anything reads them (its force kernel sets f = 0 + sum for every inner atom; a rebuild in
-    // between only carries the stale values along, and the first half-step of the next step is
-    // done by this kernel from the force in registers).  The last step of a run is not fused
-    // and stores both.
-    const bool lean = fuse;
+    // between does not carry them, and the first half-step of the next step is done by this
+    // kernel from the force in registers).  The last step of a run is not fused and stores both.
     NextStep nx{c->rB.p, c->backup.p, dt, dt2, thr2, cnt + 1};
     tstart(c, XNB_TIMER_FORCE);
-    if (mm) e = launch_force_multimat(c, p->precision, *mm, ntypes, dth, fuse ? &nx : nullptr, lean);
-    else e = launch_force(c, p->precision, false, true, false, p->lj.epsilon, p->lj.sigma, p->rcut, dth, fuse ? &nx : nullptr, lean);
+    if (mm) e = launch_force_multimat(c, p->precision, *mm, ntypes, dth, fuse ? &nx : nullptr);
+    else e = launch_force(c, p->precision, false, true, false, p->lj.epsilon, p->lj.sigma, p->rcut, dth, fuse ? &nx : nullptr);
     tstop(c);
     if (e) return e;
     if (fuse) {

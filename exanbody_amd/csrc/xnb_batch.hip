@@ -449,7 +449,7 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
       const bool fuse = k < bb.steps;
       frc.push_back(i);
       nbr.push_back(bb.f.bp.nbricks);
-      tag.push_back(fuse ? (uint32_t)EPI_NEXT_LEAN : (uint32_t)EPI_KICK);
+      tag.push_back(fuse ? (uint32_t)EPI_NEXT : (uint32_t)EPI_KICK);
       Tf = std::max(Tf, bb.f.bp.T);
       ldsf = std::max(ldsf, brick_lds_bytes(bb.f.bp));
     }

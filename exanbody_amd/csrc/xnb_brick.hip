@@ -376,13 +376,12 @@ __global__ void __launch_bounds__(1024) k_nbh_brick_mask(Geom g, BrickParams bp,
 // EPI selects the epilogue after the force store (f = 0 + sum, or f += sum with ACCUM):
 //   EPI_NONE       nothing more;
 //   EPI_KICK       v += f dth, the closing push_f_v of the step (numerical-scheme.msp);
-//   EPI_NEXT       EPI_KICK, then the next step's push_f_v_r / push_f_v
+//   EPI_NEXT       the closing push_f_v, then the next step's push_f_v_r / push_f_v
 //                  (x += v dt + f dt2; v += f dth) into NextStep::r_next, and the
-//                  particle_displ_over count of the new positions against the backup;
-//   EPI_NEXT_LEAN  EPI_NEXT without computing or storing f and ep: for a step whose
-//                  forces and energies nothing reads before the next force kernel
-//                  overwrites them (the run loop uses it when the next step does not
-//                  rebuild; the last step of a run always stores them).
+//                  particle_displ_over count of the new positions against the backup,
+//                  all from the force in registers; f and ep are neither computed nor
+//                  stored (the next force kernel overwrites both before anything reads
+//                  them; the last step of a run uses EPI_KICK and stores them).
 // Positions are read from r (staged in LDS) and, with EPI_NEXT, written to a second buffer,
 // so no workgroup reads a position another one has already advanced.
 template <bool MIXED, bool ACCUM, int EPI, int U>
@@ -443,7 +442,7 @@ __device__ __forceinline__ void lj_brick_body(const Geom& g, const BrickParams& 
         rb[u].w = 0.;
       }
 #pragma unroll
-      for (int u = 0; u < U; u++) lj_pair_mixed<EPI != EPI_NEXT_LEAN>(lj, ra, rb[u], tx, ty, tz, te);
+      for (int u = 0; u < U; u++) lj_pair_mixed<EPI != EPI_NEXT>(lj, ra, rb[u], tx, ty, tz, te);
     } else {
       // FP64: pairs by two, each pair's positions read just before its arithmetic, and the
       // scheduler kept from hoisting the next pair's reads (with k_lj_brick's register cap:
@@ -458,21 +457,21 @@ __device__ __forceinline__ void lj_brick_body(const Geom& g, const BrickParams& 
           rb[u].z = L.pz[3 * j[h + u]];
           rb[u].w = 0.;
         }
-        lj_batch_fp64<2, EPI != EPI_NEXT_LEAN>(lj, ra, rb, tx, ty, tz, te);
+        lj_batch_fp64<2, EPI != EPI_NEXT>(lj, ra, rb, tx, ty, tz, te);
         __builtin_amdgcn_sched_barrier(0);
       }
     }
   }
-  constexpr bool NEXT = EPI == EPI_NEXT || EPI == EPI_NEXT_LEAN;
+  constexpr bool NEXT = EPI == EPI_NEXT;
   bool over = false;
   if (At.live) {
     const uint32_t s = At.slot;
-    if (EPI != EPI_NEXT_LEAN) store_force<ACCUM>(s, tx, ty, tz, te, A.fx, A.fy, A.fz, A.ep);
+    if (EPI != EPI_NEXT) store_force<ACCUM>(s, tx, ty, tz, te, A.fx, A.fy, A.fz, A.ep);
     if (EPI != EPI_NONE) {
-      // the stored force (0 + sum with ACCUM false, which EPI_NEXT* requires)
-      const double ax = EPI == EPI_NEXT_LEAN ? 0.0 + tx : A.fx[s];
-      const double ay = EPI == EPI_NEXT_LEAN ? 0.0 + ty : A.fy[s];
-      const double az = EPI == EPI_NEXT_LEAN ? 0.0 + tz : A.fz[s];
+      // the force as stored (0 + sum with ACCUM false, which EPI_NEXT requires)
+      const double ax = EPI == EPI_NEXT ? 0.0 + tx : A.fx[s];
+      const double ay = EPI == EPI_NEXT ? 0.0 + ty : A.fy[s];
+      const double az = EPI == EPI_NEXT ? 0.0 + tz : A.fz[s];
       double ux = A.vx[s] + ax * dth;
       double uy = A.vy[s] + ay * dth;
       double uz = A.vz[s] + az * dth;
@@ -548,7 +547,7 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
     e4f[q] = mm.eps4f[pid];
     m24f[q] = mm.m24epsf[pid];
   }
-  constexpr bool ENERGY = EPI != EPI_NEXT_LEAN;
+  constexpr bool ENERGY = EPI != EPI_NEXT;
   constexpr int H = MIXED ? U : 2;
   double gx[NT], gy[NT], gz[NT], ge[NT];
   bool any[NT];
@@ -663,7 +662,7 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
       }
     }
   }
-  constexpr bool NEXT = EPI == EPI_NEXT || EPI == EPI_NEXT_LEAN;
+  constexpr bool NEXT = EPI == EPI_NEXT;
   bool over = false;
   if (At.live) {
     const uint32_t s = At.slot;
@@ -676,7 +675,7 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
         fz += gz[q];
         fe += ge[q];
       }
-    if (EPI != EPI_NEXT_LEAN) {
+    if (EPI != EPI_NEXT) {
       A.fx[s] = fx;
       A.fy[s] = fy;
       A.fz[s] = fz;
@@ -834,17 +833,15 @@ hipError_t launch_lj_brick(int precision, bool accum, int epi, const Geom& g, Br
   const LJConst lj = make_lj_const(eps, sigma, rcut);
   const uint32_t lds = brick_layout(bp);
   const NextStep nx = next ? *next : NextStep{};
-  const bool next_epi = epi == EPI_NEXT || epi == EPI_NEXT_LEAN;
+  const bool next_epi = epi == EPI_NEXT;
   if (next_epi && (accum || !next)) return hipErrorInvalidValue;
 #define XNB_LJB_ARGS g, bp, lds, A.r, cell_off, cell_n, blist, bcnt, lj, A, dth, nx, st
   if (precision) {
     if (epi == EPI_NEXT) return ljb_launch<true, false, EPI_NEXT>(XNB_LJB_ARGS);
-    if (epi == EPI_NEXT_LEAN) return ljb_launch<true, false, EPI_NEXT_LEAN>(XNB_LJB_ARGS);
     if (accum) return epi ? ljb_launch<true, true, EPI_KICK>(XNB_LJB_ARGS) : ljb_launch<true, true, EPI_NONE>(XNB_LJB_ARGS);
     return epi ? ljb_launch<true, false, EPI_KICK>(XNB_LJB_ARGS) : ljb_launch<true, false, EPI_NONE>(XNB_LJB_ARGS);
   }
   if (epi == EPI_NEXT) return ljb_launch<false, false, EPI_NEXT>(XNB_LJB_ARGS);
-  if (epi == EPI_NEXT_LEAN) return ljb_launch<false, false, EPI_NEXT_LEAN>(XNB_LJB_ARGS);
   if (accum) return epi ? ljb_launch<false, true, EPI_KICK>(XNB_LJB_ARGS) : ljb_launch<false, true, EPI_NONE>(XNB_LJB_ARGS);
   return epi ? ljb_launch<false, false, EPI_KICK>(XNB_LJB_ARGS) : ljb_launch<false, false, EPI_NONE>(XNB_LJB_ARGS);
 #undef XNB_LJB_ARGS
@@ -867,8 +864,7 @@ static hipError_t ljmm_dispatch(int epi, const Geom& g, const BrickParams& bp, u
   switch (epi) {
     case EPI_NONE: return ljmm_launch<MIXED, NT, true, EPI_NONE>(g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
     case EPI_KICK: return ljmm_launch<MIXED, NT, false, EPI_KICK>(g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
-    case EPI_NEXT: return ljmm_launch<MIXED, NT, false, EPI_NEXT>(g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
-    default: return ljmm_launch<MIXED, NT, false, EPI_NEXT_LEAN>(g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
+    default: return ljmm_launch<MIXED, NT, false, EPI_NEXT>(g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
   }
 }
 
@@ -878,7 +874,7 @@ hipError_t launch_ljmm_brick(int precision, int ntypes, int epi, const Geom& g, 
                              const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, const LJMulti& mm, double dth,
                              const NextStep* next, hipStream_t st) {
   if (ntypes < 1 || ntypes > 4) return hipErrorInvalidValue;
-  if ((epi == EPI_NEXT || epi == EPI_NEXT_LEAN) && !next) return hipErrorInvalidValue;
+  if (epi == EPI_NEXT && !next) return hipErrorInvalidValue;
   const uint32_t lds = brick_layout(bp);
   const NextStep nx = next ? *next : NextStep{};
   if (precision) {
@@ -929,7 +925,7 @@ __device__ __forceinline__ void lj_batch_case(const BoxFrame& f, uint32_t b, uin
 }
 
 // Force of every brick of every box; precision and epilogue are per box (uniform per
-// workgroup).  EPI_KICK closes a box's run; EPI_NEXT / EPI_NEXT_LEAN fuse its next step.
+// workgroup).  EPI_KICK closes a box's run; EPI_NEXT fuses its next step.
 __global__ void __launch_bounds__(1024) kb_lj_brick(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work, uint32_t par,
                                                     uint32_t gstep) {
   uint32_t box, b;
@@ -941,10 +937,8 @@ __global__ void __launch_bounds__(1024) kb_lj_brick(const BoxFrame* __restrict__
   switch (key) {
     case EPI_KICK: lj_batch_case<false, EPI_KICK>(f, b, par, gstep); break;
     case EPI_NEXT: lj_batch_case<false, EPI_NEXT>(f, b, par, gstep); break;
-    case EPI_NEXT_LEAN: lj_batch_case<false, EPI_NEXT_LEAN>(f, b, par, gstep); break;
     case 4u | EPI_KICK: lj_batch_case<true, EPI_KICK>(f, b, par, gstep); break;
     case 4u | EPI_NEXT: lj_batch_case<true, EPI_NEXT>(f, b, par, gstep); break;
-    case 4u | EPI_NEXT_LEAN: lj_batch_case<true, EPI_NEXT_LEAN>(f, b, par, gstep); break;
     default: break;
   }
 }

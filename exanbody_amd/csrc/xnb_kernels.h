@@ -34,7 +34,7 @@ struct BrickParams {
 
 // Particle arrays of one storage buffer (slots [0, N) inner, [N, N+G) ghosts).
 // Force-kernel epilogues (see k_lj_brick)
-enum { EPI_NONE = 0, EPI_KICK = 1, EPI_NEXT = 2, EPI_NEXT_LEAN = 3 };
+enum { EPI_NONE = 0, EPI_KICK = 1, EPI_NEXT = 2 };
 
 // Next-step first half fused into the force kernel: positions go to r_next, the count of
 // atoms displaced by >= sqrt(thr2) from their backup is added to *count.
