@@ -44,7 +44,9 @@ struct BatchBox {
   std::string err;
 };
 
-constexpr double kBrickMargin = 1.25;  // brick T / Rmax head room of attached boxes
+// brick T / Rmax head room of attached boxes.  512-box batch, 500 steps: 1.25 -> 1.12 gives
+// +4 % throughput for 9 recoveries instead of 1; 1.0 recovers thousands of times.
+constexpr double kBrickMargin = 1.12;
 constexpr double kSlotMargin = 1.3;    // slot capacity head room for ghosts
 
 }  // namespace

@@ -23,11 +23,12 @@ def _alone(j):
     return out, res
 
 
-@pytest.mark.parametrize("max_active", [0, 1])
-def test_serve_results_equal_alone(max_active):
+@pytest.mark.parametrize("max_active,steps_per_iter", [(0, 1), (1, 1), (0, 7)])
+def test_serve_results_equal_alone(max_active, steps_per_iter):
+    """Several steps per loop iteration (jobs finish inside an iteration) change nothing either."""
     jobs = job_mix(8, rate=200.0, atoms=(300, 3000), steps=(15, 45), seed=3)
     ref = [_alone(j) for j in jobs]
-    results, stats = serve(jobs, max_active=max_active, outputs=True)
+    results, stats = serve(jobs, max_active=max_active, steps_per_iter=steps_per_iter, outputs=True)
     assert stats["completed"] == len(jobs) and stats["failed"] == 0
     if max_active == 1:
         assert stats["peak_active"] == 1
