@@ -38,17 +38,17 @@ hipEvent_t get_event(xnb_ctx* c) {
     c->pool.pop_back();
     return e;
   }
-  hipEvent_t e;
-  hipEventCreate(&e);
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);  // a failed timer event only loses that timing
   return e;
 }
 
 void flush_timers(xnb_ctx* c) {
   if (c->pending.empty()) return;
-  hipStreamSynchronize(c->st);
+  (void)hipStreamSynchronize(c->st);  // errors surface on the next checked call
   for (auto& t : c->pending) {
     float ms = 0.f;
-    hipEventElapsedTime(&ms, t.a, t.b);
+    (void)hipEventElapsedTime(&ms, t.a, t.b);
     c->t_ms[t.id] += ms;
     c->t_n[t.id] += 1;
     c->pool.push_back(t.a);
@@ -61,13 +61,13 @@ void tstart(xnb_ctx* c, int id) {
   if (!c->timing) return;
   c->t_cur = id;
   c->t_a = get_event(c);
-  hipEventRecord(c->t_a, c->st);
+  (void)hipEventRecord(c->t_a, c->st);
 }
 
 void tstop(xnb_ctx* c) {
   if (!c->timing || c->t_cur < 0) return;
   hipEvent_t b = get_event(c);
-  hipEventRecord(b, c->st);
+  (void)hipEventRecord(b, c->st);
   c->pending.push_back({c->t_cur, c->t_a, b});
   c->t_cur = -1;
   if (c->pending.size() >= 2048) flush_timers(c);
@@ -126,11 +126,13 @@ int ensure_capacity(xnb_ctx* c, size_t ncap, size_t keep) {
     DArr<T> na;
     int e = dalloc(c, na, elems);
     if (e) return e;
-    if (arr.p && keep) {
-      hipError_t he = hipMemcpyAsync(na.p, arr.p, keep * sizeof(T), hipMemcpyDeviceToDevice, c->st);
-      if (he != hipSuccess) return fail(c, XNB_E_HIP, "capacity copy failed");
+    hipError_t he = hipSuccess;
+    if (arr.p && keep) he = hipMemcpyAsync(na.p, arr.p, keep * sizeof(T), hipMemcpyDeviceToDevice, c->st);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->st);
+    if (he != hipSuccess) {
+      dfree(na);
+      return fail(c, XNB_E_HIP, std::string("capacity copy failed: ") + hipGetErrorString(he));
     }
-    hipStreamSynchronize(c->st);
     dfree(arr);
     arr = na;
     return XNB_OK;
@@ -555,13 +557,14 @@ int xnb_create(xnb_ctx** out, const xnb_domain* d, int device) {
 
 int xnb_destroy(xnb_ctx* c) {
   if (!c) return XNB_OK;
-  hipSetDevice(c->device);
-  if (c->st) hipStreamSynchronize(c->st);
+  // teardown: failures here leave nothing to recover, so results are not checked
+  (void)hipSetDevice(c->device);
+  if (c->st) (void)hipStreamSynchronize(c->st);
   for (auto& t : c->pending) {
-    hipEventDestroy(t.a);
-    hipEventDestroy(t.b);
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
   }
-  for (auto ev : c->pool) hipEventDestroy(ev);
+  for (auto ev : c->pool) (void)hipEventDestroy(ev);
   dfree(c->slot_cell_A); dfree(c->slot_cell_B); dfree(c->rA); dfree(c->rB);
   for (int a = 0; a < 3; a++) { dfree(c->vA[a]); dfree(c->vB[a]); dfree(c->fA[a]); dfree(c->fB[a]); }
   dfree(c->eA); dfree(c->eB); dfree(c->idA); dfree(c->idB);
@@ -571,9 +574,9 @@ int xnb_destroy(xnb_ctx* c) {
     dfree(*a);
   dfree(c->blist); dfree(c->bcnt);
   dfree(c->stream_buf); dfree(c->u64s); dfree(c->step_counts); dfree(c->red_part); dfree(c->red_out);
-  if (c->h_u32) hipHostFree(c->h_u32);
-  if (c->h_u64) hipHostFree(c->h_u64);
-  if (c->st) hipStreamDestroy(c->st);
+  if (c->h_u32) (void)hipHostFree(c->h_u32);
+  if (c->h_u64) (void)hipHostFree(c->h_u64);
+  if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return XNB_OK;
 }
@@ -1231,9 +1234,21 @@ int xnb_autotune(xnb_ctx* c, const xnb_lj_params* p, double rcut, int precision,
   xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
   if (!cached) {
     static const int cand[][3] = {{4, 4, 2}, {4, 2, 2}, {2, 4, 2}, {4, 4, 1}, {2, 2, 2}, {4, 2, 1}};
-    hipEvent_t t0, t1;
-    HIPCHK(c, hipEventCreate(&t0));
-    HIPCHK(c, hipEventCreate(&t1));
+    struct Events {  // destroyed on every return path
+      hipEvent_t a = nullptr, b = nullptr;
+      ~Events() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+      }
+    } ev;
+    struct Dims {  // the caller's shape is back on every exit from the trials (the winner is set below)
+      xnb_ctx* c;
+      int d[3];
+      ~Dims() { std::copy(d, d + 3, c->brick_dims); }
+    } restore{c, {saved[0], saved[1], saved[2]}};
+    HIPCHK(c, hipEventCreate(&ev.a));
+    HIPCHK(c, hipEventCreate(&ev.b));
+    hipEvent_t t0 = ev.a, t1 = ev.b;
     float best_ms = 1e30f;
     for (const auto& d : cand) {
       c->brick_dims[0] = d[0];
@@ -1255,8 +1270,6 @@ int xnb_autotune(xnb_ctx* c, const xnb_lj_params* p, double rcut, int precision,
         best = {d[0], d[1], d[2]};
       }
     }
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
     if (e) return e;
     if (best[0] > 0) {
       std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1328,27 +1341,28 @@ int xnb_copy_probe(xnb_ctx* c, size_t bytes, int iters, double* gbps) {
   if (bytes < 16 || iters <= 0 || !gbps) return fail(c, XNB_E_ARG, "bad copy probe arguments");
   bytes &= ~(size_t)15;
   void *a = nullptr, *b = nullptr;
-  HIPCHK(c, hipMalloc(&a, bytes));
-  if (hipMalloc(&b, bytes) != hipSuccess) {
-    hipFree(a);
-    return fail(c, XNB_E_NOMEM, "copy probe allocation");
-  }
-  hipMemsetAsync(a, 1, bytes, c->st);
-  hipEvent_t t0, t1;
-  hipEventCreate(&t0);
-  hipEventCreate(&t1);
-  launch_copy_probe(a, b, bytes, c->st);  // warm-up
-  hipEventRecord(t0, c->st);
-  for (int q = 0; q < iters; q++) launch_copy_probe(a, b, bytes, c->st);
-  hipEventRecord(t1, c->st);
-  hipError_t he = hipEventSynchronize(t1);
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  // every step's status folds into he; the buffers and events are released on all paths
+  hipError_t he = hipMalloc(&a, bytes);
+  if (he == hipSuccess) he = hipMalloc(&b, bytes);
+  if (he == hipSuccess) he = hipMemsetAsync(a, 1, bytes, c->st);
+  if (he == hipSuccess) he = hipEventCreate(&t0);
+  if (he == hipSuccess) he = hipEventCreate(&t1);
   float ms = 0.f;
-  hipEventElapsedTime(&ms, t0, t1);
-  hipEventDestroy(t0);
-  hipEventDestroy(t1);
-  hipFree(a);
-  hipFree(b);
-  if (he != hipSuccess) return fail(c, XNB_E_HIP, "copy probe failed");
+  if (he == hipSuccess) {
+    launch_copy_probe(a, b, bytes, c->st);  // warm-up
+    he = hipEventRecord(t0, c->st);
+    for (int q = 0; q < iters && he == hipSuccess; q++) launch_copy_probe(a, b, bytes, c->st);
+    if (he == hipSuccess) he = hipGetLastError();
+    if (he == hipSuccess) he = hipEventRecord(t1, c->st);
+    if (he == hipSuccess) he = hipEventSynchronize(t1);
+    if (he == hipSuccess) he = hipEventElapsedTime(&ms, t0, t1);
+  }
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("copy probe: ") + hipGetErrorString(he));
   *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
   return XNB_OK;
 }
@@ -1368,7 +1382,7 @@ int xnb_fp64_core(xnb_ctx* c, const double* x, size_t n, double* sqrt_out, doubl
   if (he == hipSuccess) he = hipMemcpyAsync(sqrt_out, d + n, n * sizeof(double), hipMemcpyDeviceToHost, c->st);
   if (he == hipSuccess) he = hipMemcpyAsync(recip_out, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost, c->st);
   if (he == hipSuccess) he = hipStreamSynchronize(c->st);
-  hipFree(d);
+  if (d) (void)hipFree(d);
   if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("fp64 core: ") + hipGetErrorString(he));
   return XNB_OK;
 }

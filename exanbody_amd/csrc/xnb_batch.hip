@@ -200,12 +200,12 @@ int upload_tables(xnb_batch* B, const Tables& T) {
   if (n > B->hwork_cap) {
     BHIP(B, hipStreamSynchronize(B->st));
     for (int k = 0; k < 2; k++) {
-      if (B->hwork[k]) hipHostFree(B->hwork[k]);
+      if (B->hwork[k]) (void)hipHostFree(B->hwork[k]);  // after the synchronisation above
       B->hwork[k] = nullptr;
     }
     const size_t cap = n + n / 2 + 1024;
     for (int k = 0; k < 2; k++) BHIP(B, hipHostMalloc((void**)&B->hwork[k], cap * sizeof(uint32_t), hipHostMallocDefault));
-    if (B->dwork.p) hipFree(B->dwork.p);
+    if (B->dwork.p) (void)hipFree(B->dwork.p);
     B->dwork.p = nullptr;
     BHIP(B, hipMalloc((void**)&B->dwork.p, cap * sizeof(uint32_t)));
     B->dwork.n = cap;
@@ -256,8 +256,8 @@ int xnb_batch_create(xnb_batch** out, int device) {
   B->device = device;
   B->box.resize(kMaxBatchBoxes);
   int lds = 0;
-  hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
-  if (lds > 0) B->lds_limit = (uint32_t)lds;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+    B->lds_limit = (uint32_t)lds;
   bool ok = hipStreamCreate(&B->st) == hipSuccess;
   ok = ok && hipMalloc((void**)&B->dF.p, kMaxBatchBoxes * sizeof(BoxFrame)) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->hF, kMaxBatchBoxes * sizeof(BoxFrame), hipHostMallocDefault) == hipSuccess;
@@ -268,7 +268,7 @@ int xnb_batch_create(xnb_batch** out, int device) {
   // both staging events start recorded, so the first waits on them return at once
   if (ok) ok = hipEventRecord(B->hev[0], B->st) == hipSuccess && hipEventRecord(B->hev[1], B->st) == hipSuccess;
   if (!ok) {
-    delete B;
+    xnb_batch_destroy(B);  // frees whatever was created
     return XNB_E_HIP;
   }
   *out = B;
@@ -277,20 +277,21 @@ int xnb_batch_create(xnb_batch** out, int device) {
 
 int xnb_batch_destroy(xnb_batch* B) {
   if (!B) return XNB_E_ARG;
-  hipSetDevice(B->device);
-  hipStreamSynchronize(B->st);
+  // teardown: failures here leave nothing to recover, so results are not checked
+  (void)hipSetDevice(B->device);
+  if (B->st) (void)hipStreamSynchronize(B->st);
   for (auto& bb : B->box)
-    if (bb.counts.p) hipFree(bb.counts.p);
-  if (B->dF.p) hipFree(B->dF.p);
-  if (B->hF) hipHostFree(B->hF);
-  if (B->dwords.p) hipFree(B->dwords.p);
-  if (B->hwords) hipHostFree(B->hwords);
-  if (B->dwork.p) hipFree(B->dwork.p);
+    if (bb.counts.p) (void)hipFree(bb.counts.p);
+  if (B->dF.p) (void)hipFree(B->dF.p);
+  if (B->hF) (void)hipHostFree(B->hF);
+  if (B->dwords.p) (void)hipFree(B->dwords.p);
+  if (B->hwords) (void)hipHostFree(B->hwords);
+  if (B->dwork.p) (void)hipFree(B->dwork.p);
   for (int k = 0; k < 2; k++) {
-    if (B->hwork[k]) hipHostFree(B->hwork[k]);
-    if (B->hev[k]) hipEventDestroy(B->hev[k]);
+    if (B->hwork[k]) (void)hipHostFree(B->hwork[k]);
+    if (B->hev[k]) (void)hipEventDestroy(B->hev[k]);
   }
-  if (B->st) hipStreamDestroy(B->st);
+  if (B->st) (void)hipStreamDestroy(B->st);
   delete B;
   return XNB_OK;
 }
@@ -526,7 +527,7 @@ int xnb_batch_detach(xnb_batch* B, int slot, xnb_run_params* p) {
   }
   const int rc = bb.failed ? XNB_E_STATE : XNB_OK;
   if (bb.failed) B->err = bb.err;
-  hipFree(bb.counts.p);
+  (void)hipFree(bb.counts.p);  // the batch stream was synchronised above
   bb = BatchBox{};
   return rc;
 }

@@ -108,7 +108,7 @@ int fail(xnb_ctx* c, int code, const std::string& msg);
 template <class T>
 int dalloc(xnb_ctx* c, DArr<T>& a, size_t n) {
   if (a.n >= n && a.p) return XNB_OK;
-  if (a.p) hipFree(a.p);
+  if (a.p) (void)hipFree(a.p);  // hipFree waits for work still using the buffer
   a.p = nullptr;
   a.n = 0;
   size_t nn = n ? n : 1;
@@ -123,7 +123,7 @@ int dalloc(xnb_ctx* c, DArr<T>& a, size_t n) {
 
 template <class T>
 void dfree(DArr<T>& a) {
-  if (a.p) hipFree(a.p);
+  if (a.p) (void)hipFree(a.p);
   a.p = nullptr;
   a.n = 0;
 }
