@@ -21,7 +21,8 @@ class Batch:
             raise XnbError(rc, f"{what}: {msg.decode() if msg else ''}")
 
     def attach(self, box, steps, dt, epsilon, sigma, rcut, rcut_inc, precision=_lib.PRECISION_FP64, rebuild_every=10):
-        """Attach an initialised Box for `steps` steps; returns its slot."""
+        """Attach an initialised Box for `steps` steps; returns its slot.  rebuild_every 0
+        rebuilds on the displacement trigger (particle_displ_over), like Box.run."""
         p = _lib.RunParams(dt, _lib.LJParams(epsilon, sigma), rcut, rcut_inc, int(precision), int(rebuild_every), int(steps),
                            0, 0)
         slot = C.c_int()

@@ -34,7 +34,7 @@ struct BatchBox {
   bool used = false, finished = false, failed = false;
   BoxFrame f{};  // host mirror of the device frame
   int64_t steps = 0, done = 0;
-  int every = 10;
+  int every = 10;         // rebuild every this many steps; 0: when an atom passed max_displ
   bool fresh = true;      // the next step's first half still has to be launched
   uint32_t last_par = 0;  // parity of the box's last step
   int64_t n_rebuilds = 0, n_violations = 0, n_recover = 0;
@@ -59,6 +59,8 @@ struct xnb_batch {
   BoxFrame* hF = nullptr;  // pinned staging of frame uploads, one entry per slot
   DArr<uint32_t> dwords;   // 4 words per slot
   uint32_t* hwords = nullptr;
+  DArr<uint32_t> dtrig;    // displacement trigger per slot for the step about to run
+  uint32_t* htrig = nullptr;
   DArr<uint32_t> dwork;
   uint32_t* hwork[2] = {nullptr, nullptr};
   size_t hwork_cap = 0;
@@ -263,8 +265,11 @@ int xnb_batch_create(xnb_batch** out, int device) {
   ok = ok && hipHostMalloc((void**)&B->hF, kMaxBatchBoxes * sizeof(BoxFrame), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipMalloc((void**)&B->dwords.p, 4 * kMaxBatchBoxes * sizeof(uint32_t)) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->hwords, 4 * kMaxBatchBoxes * sizeof(uint32_t), hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipMalloc((void**)&B->dtrig.p, kMaxBatchBoxes * sizeof(uint32_t)) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&B->htrig, kMaxBatchBoxes * sizeof(uint32_t), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipEventCreate(&B->hev[0]) == hipSuccess && hipEventCreate(&B->hev[1]) == hipSuccess;
   if (ok) ok = hipMemset(B->dwords.p, 0, 4 * kMaxBatchBoxes * sizeof(uint32_t)) == hipSuccess;
+  if (ok) ok = hipMemset(B->dtrig.p, 0, kMaxBatchBoxes * sizeof(uint32_t)) == hipSuccess;
   // both staging events start recorded, so the first waits on them return at once
   if (ok) ok = hipEventRecord(B->hev[0], B->st) == hipSuccess && hipEventRecord(B->hev[1], B->st) == hipSuccess;
   if (!ok) {
@@ -286,6 +291,8 @@ int xnb_batch_destroy(xnb_batch* B) {
   if (B->hF) (void)hipHostFree(B->hF);
   if (B->dwords.p) (void)hipFree(B->dwords.p);
   if (B->hwords) (void)hipHostFree(B->hwords);
+  if (B->dtrig.p) (void)hipFree(B->dtrig.p);
+  if (B->htrig) (void)hipHostFree(B->htrig);
   if (B->dwork.p) (void)hipFree(B->dwork.p);
   for (int k = 0; k < 2; k++) {
     if (B->hwork[k]) (void)hipHostFree(B->hwork[k]);
@@ -299,7 +306,7 @@ int xnb_batch_destroy(xnb_batch* B) {
 const char* xnb_batch_error(const xnb_batch* B) { return B ? B->err.c_str() : "null batch"; }
 
 int xnb_batch_attach(xnb_batch* B, xnb_ctx* c, const xnb_run_params* p, int* slot) {
-  if (!B || !c || !p || !slot || p->steps < 1 || p->rebuild_every < 1) return bfail(B, XNB_E_ARG, "bad attach arguments");
+  if (!B || !c || !p || !slot || p->steps < 1) return bfail(B, XNB_E_ARG, "bad attach arguments");
   int e;
   if ((e = check_ctx(c))) return e;
   if (c->device != B->device) return bfail(B, XNB_E_ARG, "box on another device");
@@ -324,7 +331,7 @@ int xnb_batch_attach(xnb_batch* B, xnb_ctx* c, const xnb_run_params* p, int* slo
   bb.c = c;
   bb.used = true;
   bb.steps = p->steps;
-  bb.every = p->rebuild_every;
+  bb.every = std::max(p->rebuild_every, 0);  // <= 0: displacement trigger, as in xnb_run
   bb.rebuilt.assign((size_t)p->steps, 0);
   if (hipMalloc((void**)&bb.counts.p, (size_t)p->steps * sizeof(unsigned long long)) != hipSuccess) {
     bb.used = false;
@@ -352,20 +359,40 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
       if (active(B->box[i])) act.push_back(i);
     if (act.empty()) break;
     const uint32_t par = B->par, gstep = B->gstep;
-    // 1. first half of boxes whose previous force kernel did not do it
-    std::vector<int> fresh, reb, quiet;
+    // 1. first half of boxes whose previous force kernel did not do it; then the trigger of
+    // boxes rebuilt on displacement: this step's particle_displ_over count, written by the
+    // previous force kernel or by that first half (one host read per step while such boxes run)
+    std::vector<int> fresh, disp, reb, quiet;
     for (int i : act) {
-      BatchBox& bb = B->box[i];
-      if (bb.fresh) fresh.push_back(i);
+      if (B->box[i].fresh) fresh.push_back(i);
+      if (B->box[i].every == 0) disp.push_back(i);
+    }
+    {
+      Tables T0;
+      std::vector<uint32_t> nF;
+      for (int i : fresh) nF.push_back(B->box[i].f.N);
+      const size_t o_fresh = T0.add(blocks_table(fresh, nF));
+      const size_t n_fresh = T0.v.size() - o_fresh;
+      const size_t o_disp = T0.add(std::vector<uint32_t>(disp.begin(), disp.end()));
+      if ((e = upload_tables(B, T0))) return e;
+      const uint32_t* W0 = B->dwork.p;
+      if (n_fresh) launch_batch_first_half(B->dF.p, W0 + o_fresh, (uint32_t)n_fresh, par, gstep, B->st);
+      for (int i : fresh) B->box[i].fresh = false;
+      if (!disp.empty()) {
+        launch_batch_trigger(B->dF.p, W0 + o_disp, (uint32_t)disp.size(), gstep, B->dtrig.p, B->st);
+        BHIP(B, hipMemcpyAsync(B->htrig, B->dtrig.p, kMaxBatchBoxes * sizeof(uint32_t), hipMemcpyDeviceToHost, B->st));
+        BHIP(B, hipStreamSynchronize(B->st));
+      }
+    }
+    for (int i : act) {
+      const BatchBox& bb = B->box[i];
       const int64_t k = bb.done + 1;
-      if (k % bb.every == 0) reb.push_back(i);
+      const bool trigger = bb.every > 0 ? k % bb.every == 0 : B->htrig[i] != 0;
+      if (trigger) reb.push_back(i);
       else quiet.push_back(i);
     }
     Tables T1;
     std::vector<uint32_t> nN, nC, nGh, nBr;
-    for (int i : fresh) nN.push_back(B->box[i].f.N);
-    const size_t o_fresh = T1.add(blocks_table(fresh, nN));
-    const size_t n_fresh = T1.v.size() - o_fresh;
     size_t o_rs = 0, n_rs = 0, o_rc = 0, n_rc = 0, o_rg = 0, n_rg = 0, o_rb = 0, n_rb = 0, o_list = 0;
     uint32_t Treb = 64, ldsreb = 0;
     if (!reb.empty()) {
@@ -392,8 +419,6 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
     }
     if ((e = upload_tables(B, T1))) return e;
     const uint32_t* W = B->dwork.p;
-    if (n_fresh) launch_batch_first_half(B->dF.p, W + o_fresh, (uint32_t)n_fresh, par, gstep, B->st);
-    for (int i : fresh) B->box[i].fresh = false;
     // 2. rebuilds (move_particles, backup_r, ghosts, chunk_neighbors), one launch each
     if (!reb.empty()) {
       launch_batch_zero_words(B->dwords.p, W + o_list, (uint32_t)reb.size(), B->st);

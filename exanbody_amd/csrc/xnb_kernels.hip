@@ -1256,6 +1256,21 @@ void launch_batch_ghost_update_r(const BoxFrame* F, const uint32_t* work, uint32
   if (nwork) kb_ghost_update_r<<<nwork, kBatchSlotBlock, 0, st>>>(F, work, par);
 }
 
+// displacement trigger of each listed box for the step at gstep: its particle_displ_over
+// count (local step k = gstep - gstep0 + 1 keeps it at counts[k - 1]) is positive
+__global__ void kb_trigger(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ boxes, uint32_t n, uint32_t gstep,
+                           uint32_t* __restrict__ out) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const uint32_t box = boxes[q];
+  const BoxFrame& f = F[box];
+  out[box] = f.counts[gstep - f.gstep0] > 0 ? 1u : 0u;
+}
+
+void launch_batch_trigger(const BoxFrame* F, const uint32_t* boxes, uint32_t n, uint32_t gstep, uint32_t* out, hipStream_t st) {
+  if (n) kb_trigger<<<(n + 255) / 256, 256, 0, st>>>(F, boxes, n, gstep, out);
+}
+
 void launch_batch_first_half(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t par, uint32_t gstep, hipStream_t st) {
   if (nwork) kb_first_half<<<nwork, kBatchSlotBlock, 0, st>>>(F, work, par, gstep);
 }

@@ -213,9 +213,11 @@ int xnb_snapshot_load(xnb_ctx* ctx, const void* buf, size_t bytes);
 /* No counterpart in the reference, which advances one domain per process: many independent
  * periodic boxes advanced together on one GPU, each kernel of a step being one launch over
  * all boxes.  A box is an xnb_ctx after xnb_init (brick-path geometry: cell edge >=
- * nbh_dist); attached with the parameters xnb_run would take (fixed-interval rebuild,
- * rebuild_every >= 1), it follows xnb_run's operator sequence and ends bitwise equal to the
- * same box advanced by xnb_run alone.  Boxes may be attached and detached between steps. */
+ * nbh_dist); attached with the parameters xnb_run would take (fixed-interval or
+ * displacement-triggered rebuilds), it follows xnb_run's operator sequence and ends bitwise
+ * equal to the same box advanced by xnb_run alone.  Boxes may be attached and detached
+ * between steps.  While a displacement-triggered box runs, each step reads the boxes'
+ * trigger flags back to the host once. */
 typedef struct xnb_batch xnb_batch;
 int xnb_batch_create(xnb_batch** out, int device);
 int xnb_batch_destroy(xnb_batch* batch);

@@ -1,6 +1,7 @@
 """Batched boxes (xnb_batch): every box advanced in a batch, alone or in the mix, attached and
 detached at different steps, ends bitwise equal to the same box advanced by xnb_run alone --
-FP64 and mixed precision alike (same per-atom operations in the same order)."""
+FP64 and mixed precision alike (same per-atom operations in the same order), rebuilt at a
+fixed interval or on the displacement trigger."""
 import numpy as np
 import pytest
 
@@ -18,6 +19,9 @@ JOBS = [
     (7, 0.80, 1.00, 14, 40, PRECISION_MIXED, 10),
     (5, 0.70, 1.50, 15, 12, PRECISION_MIXED, 4),
     (8, 0.85, 0.80, 16, 26, PRECISION_FP64, 10),
+    # rebuilt on the displacement trigger (hot, so atoms pass max_displ every few steps)
+    (5, 0.80, 2.50, 17, 40, PRECISION_FP64, 0),
+    (6, 0.75, 3.00, 18, 35, PRECISION_MIXED, 0),
 ]
 
 
@@ -65,6 +69,8 @@ def test_batch_equals_alone_all_at_once():
         assert batch.status(s) == (j[4], 1)
         res = batch.detach(s)
         assert res == ref[q][1], f"job {q}: rebuild / violation counts"
+        if j[6] == 0:
+            assert res["rebuilds"] >= 2 and res["violations"] == 0, f"job {q}: displacement-triggered rebuilds {res}"
         _same(b.export(), ref[q][0], f"job {q}")
         b.close()
     batch.close()
