@@ -934,6 +934,9 @@ __global__ void __launch_bounds__(1024) kb_nbh_brick(const BoxFrame* __restrict_
   uint32_t box, b;
   if (!batch_brick(work, box, b)) return;  // whole workgroup
   const BoxFrame& f = F[box];
+  // ghost slots overflowed in this rebuild (bit 2, set by the ghost fill): the cell ranges
+  // reach past the slot arrays, so the box is left to the one-box recovery
+  if (f.words[0] & 4u) return;
   const double4* r = f.rbuf[(f.rcur ^ par) & 1u];
   nbh_brick_body<false>(f.g, f.bp, b, r, f.cell_off, f.cell_n, f.max2, 0, 0, f.blist, f.bcnt, f.words + 1);
 }
