@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the headline force kernel k_lj_brick<MIXED, false, 2, 8> on this
 # workload: 2 x FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc, calibrated on the copy probe
 # (profiles/lj_c2_force_pmc.md)
-FORCE_TRAFFIC_BYTES = {"mixed": 2.873e9, "fp64": 3.215e9}
+FORCE_TRAFFIC_BYTES = {"mixed": 2.882e9, "fp64": 3.215e9}
 VALU_PEAK_TFLOPS = 78.6  # FP64 vector; scalar FP32 issues at the same rate (packed FP32 disabled)
 
 WORKLOADS = {
