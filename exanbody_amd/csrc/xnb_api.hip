@@ -225,12 +225,13 @@ int op_move(xnb_ctx* c, bool forces = true, bool check = true) {
   c->lists_valid = false;
   c->ghosts_valid = false;
   c->backup_valid = false;
+  // the out-of-domain flag describes this move, also when there is nothing to move
+  HIPCHK(c, hipMemsetAsync(c->u32s.p, 0, sizeof(uint32_t), c->st));
   if (c->N == 0) return XNB_OK;
   tstart(c, XNB_TIMER_REBIN);
   HIPCHK(c, hipMemsetAsync(c->cnt_out.p, 0, g.ncells * sizeof(uint32_t), c->st));
   HIPCHK(c, hipMemsetAsync(c->cnt_in.p, 0, g.ncells * sizeof(uint32_t), c->st));
   HIPCHK(c, hipMemsetAsync(c->in_cur.p, 0, g.ncells * sizeof(uint32_t), c->st));
-  HIPCHK(c, hipMemsetAsync(c->u32s.p, 0, sizeof(uint32_t), c->st));
   launch_move_flag(g, c->N, c->A.r, c->slot_cell_A.p, c->mv_dst.p, c->cnt_out.p, c->cnt_in.p, c->u32s.p, c->st);
   launch_move_pack(g, c->cell_off.p, c->cell_n.p, c->cnt_out.p, c->cnt_in.p, c->mv_dst.p, c->pk.p, c->newloc.p, c->new_n.p, c->st);
   int e;
@@ -251,8 +252,9 @@ int op_move(xnb_ctx* c, bool forces = true, bool check = true) {
   return XNB_OK;
 }
 
-// move_check: also report the out-of-domain flag of a preceding op_move(c, ..., false)
-int op_ghost_all(xnb_ctx* c, bool move_check) {
+// move_check: also report the out-of-domain flag of a preceding op_move(c, ..., false).
+// forces: images copy f and ep of their sources (false: f and ep of the images are 0)
+int op_ghost_all(xnb_ctx* c, bool move_check, bool forces) {
   const Geom& g = c->g;
   int e;
   tstart(c, XNB_TIMER_GHOST);
@@ -266,7 +268,7 @@ int op_ghost_all(xnb_ctx* c, bool move_check) {
   if ((e = ensure_capacity(c, (size_t)c->N + G, c->N))) return e;
   tstart(c, XNB_TIMER_GHOST);
   launch_ghost_fill(g, c->N, c->A, c->goff.p, c->cell_off.p, c->cell_n.p, c->slot_cell_A.p, c->ghost_src.p, c->ghost_shift.p,
-                    (uint32_t)std::min<size_t>(c->cap, 0xffffffffu), c->u32s.p + 7, c->st);
+                    (uint32_t)std::min<size_t>(c->cap, 0xffffffffu), c->u32s.p + 7, forces, c->st);
   tstop(c);
   HIPCHK(c, hipGetLastError());
   c->G = G;
@@ -462,7 +464,7 @@ int full_update(xnb_ctx* c, bool forces) {
   int e;
   if ((e = op_move(c, forces, false))) return e;
   if ((e = op_backup(c))) return e;
-  if ((e = op_ghost_all(c, true))) return e;
+  if ((e = op_ghost_all(c, true, forces))) return e;
   xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
   return op_nbh(c, c->nbh_dist_lab, cfg, 0);
 }
@@ -535,6 +537,12 @@ int xnb_create(xnb_ctx** out, const xnb_domain* d, int device) {
   if ((e = set_geometry(c, 0.0)) || (e = dalloc(c, c->u32s, 8)) || (e = dalloc(c, c->u64s, 8)) || (e = dalloc(c, c->red_out, 2))) {
     xnb_destroy(c);
     return e;
+  }
+  // flags and counters start at 0 (hipMalloc does not clear)
+  if (hipMemsetAsync(c->u32s.p, 0, c->u32s.n * sizeof(uint32_t), c->st) != hipSuccess ||
+      hipMemsetAsync(c->u64s.p, 0, c->u64s.n * sizeof(unsigned long long), c->st) != hipSuccess) {
+    xnb_destroy(c);
+    return fail(nullptr, XNB_E_HIP, "scalar buffer clear failed");
   }
   if (hipHostMalloc((void**)&c->h_u32, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->h_u64, 64 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {

@@ -235,7 +235,7 @@ int recover_box(xnb_batch* B, int i, const xnb_run_params& p) {
   int e;
   c->brick_margin = kBrickMargin;
   if ((e = ensure_capacity(c, (size_t)std::ceil((c->N + c->G) * kSlotMargin) + 256, c->N))) return e;
-  if ((e = op_ghost_all(c))) return e;
+  if ((e = op_ghost_all(c, false, false))) return e;  // f and ep of the run's boxes are not carried
   xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
   if ((e = op_nbh(c, c->nbh_dist_lab, cfg, 0))) return e;
   if (c->list_kind != 1) return fail(c, XNB_E_STATE, "box no longer fits the brick path");

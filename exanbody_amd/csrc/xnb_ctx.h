@@ -133,10 +133,10 @@ void bind_states(xnb_ctx* c);
 void swap_buffers(xnb_ctx* c);
 int ensure_capacity(xnb_ctx* c, size_t ncap, size_t keep);
 int check_ctx(xnb_ctx* c);
-int op_ghost_all(xnb_ctx* c, bool move_check = false);
+int op_ghost_all(xnb_ctx* c, bool move_check = false, bool forces = true);
 int op_backup(xnb_ctx* c);
 int op_nbh(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& cfg, int all_cells);
-int full_update(xnb_ctx* c, bool forces = true);  // forces false: f and ep not carried by the move
+int full_update(xnb_ctx* c, bool forces = true);  // forces false: f and ep neither carried by the move nor copied to ghosts
 void tstart(xnb_ctx* c, int id);
 void tstop(xnb_ctx* c);
 

@@ -354,10 +354,13 @@ __global__ void __launch_bounds__(256) k_ghost_count(Geom g, const double4* __re
 
 // Ghost cell c of one box (one wave).  Slots past cap are not written: the cell count is still
 // stored and *status gets bit 2, so the caller sizes the buffers and runs the fill again.
+// Each image copies every field of its source, forces and energy included, like
+// ghost_update_all (update_ghosts.cu:45, AddDefaultFields of the grid's fields); forces false
+// (run loops, whose re-binning does not carry f and ep) writes 0 to f and ep instead.
 __device__ inline void ghost_fill_wave(const Geom& g, uint32_t N, const PState& B, const uint32_t* __restrict__ goff,
                                        uint32_t* __restrict__ cell_off, uint32_t* __restrict__ cell_n, uint32_t* __restrict__ slot_cell,
                                        uint32_t* __restrict__ ghost_src, uint32_t* __restrict__ ghost_shift, uint32_t cap,
-                                       uint32_t* __restrict__ status, uint32_t c) {
+                                       uint32_t* __restrict__ status, uint32_t c, bool forces) {
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t src;
   int si, sj, sk;
@@ -378,7 +381,11 @@ __device__ inline void ghost_fill_wave(const Geom& g, uint32_t N, const PState& 
         } else {
           B.r[t] = gp;
           B.vx[t] = B.vx[s]; B.vy[t] = B.vy[s]; B.vz[t] = B.vz[s];
-          B.fx[t] = 0.; B.fy[t] = 0.; B.fz[t] = 0.; B.ep[t] = 0.;
+          if (forces) {
+            B.fx[t] = B.fx[s]; B.fy[t] = B.fy[s]; B.fz[t] = B.fz[s]; B.ep[t] = B.ep[s];
+          } else {
+            B.fx[t] = 0.; B.fy[t] = 0.; B.fz[t] = 0.; B.ep[t] = 0.;
+          }
           B.id[t] = B.id[s];
           slot_cell[t] = c;
           ghost_src[t - N] = s;
@@ -397,10 +404,11 @@ __device__ inline void ghost_fill_wave(const Geom& g, uint32_t N, const PState& 
 __global__ void __launch_bounds__(256) k_ghost_fill(Geom g, uint32_t N, PState B, const uint32_t* __restrict__ goff,
                                                     uint32_t* __restrict__ cell_off, uint32_t* __restrict__ cell_n,
                                                     uint32_t* __restrict__ slot_cell, uint32_t* __restrict__ ghost_src,
-                                                    uint32_t* __restrict__ ghost_shift, uint32_t cap, uint32_t* __restrict__ status) {
+                                                    uint32_t* __restrict__ ghost_shift, uint32_t cap, uint32_t* __restrict__ status,
+                                                    int forces) {
   uint32_t c;
   if (!ghost_wave_cell(g, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
-  ghost_fill_wave(g, N, B, goff, cell_off, cell_n, slot_cell, ghost_src, ghost_shift, cap, status, c);
+  ghost_fill_wave(g, N, B, goff, cell_off, cell_n, slot_cell, ghost_src, ghost_shift, cap, status, c, forces != 0);
 }
 
 // ghost_update_r (update_ghosts.cu:58): re-copy shifted positions along the stored scheme
@@ -928,9 +936,10 @@ void launch_ghost_count(const Geom& g, const double4* r, const uint32_t* cell_of
   k_ghost_count<<<nblk(ghost_cell_count(g) * 64u, 256), 256, 0, st>>>(g, r, cell_off, cell_n, gcnt);
 }
 void launch_ghost_fill(const Geom& g, uint32_t N, const PState& B, const uint32_t* goff, uint32_t* cell_off, uint32_t* cell_n,
-                       uint32_t* slot_cell, uint32_t* ghost_src, uint32_t* ghost_shift, uint32_t cap, uint32_t* status, hipStream_t st) {
+                       uint32_t* slot_cell, uint32_t* ghost_src, uint32_t* ghost_shift, uint32_t cap, uint32_t* status, bool forces,
+                       hipStream_t st) {
   k_ghost_fill<<<nblk(ghost_cell_count(g) * 64u, 256), 256, 0, st>>>(g, N, B, goff, cell_off, cell_n, slot_cell, ghost_src,
-                                                                       ghost_shift, cap, status);
+                                                                       ghost_shift, cap, status, forces ? 1 : 0);
 }
 void launch_ghost_update_r(const Geom& g, uint32_t N, uint32_t G, double4* r, const uint32_t* ghost_src,
                            const uint32_t* ghost_shift, hipStream_t st) {
@@ -1191,7 +1200,7 @@ __global__ void __launch_bounds__(kBatchSlotBlock) kb_ghost_fill(const BoxFrame*
   const BoxFrame& f = F[box];
   if (!ghost_wave_cell(f.g, idx, c)) return;
   const FrameState X = frame_state(f, par);
-  ghost_fill_wave(f.g, f.N, X.A, f.goff, f.cell_off, f.cell_n, X.scA, f.ghost_src, f.ghost_shift, f.cap, f.words + 0, c);
+  ghost_fill_wave(f.g, f.N, X.A, f.goff, f.cell_off, f.cell_n, X.scA, f.ghost_src, f.ghost_shift, f.cap, f.words + 0, c, false);
 }
 
 __global__ void __launch_bounds__(kBatchSlotBlock) kb_ghost_update_r(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,

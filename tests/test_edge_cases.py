@@ -1,0 +1,130 @@
+"""Edge cases of the hot path, GPU vs CPU oracle, bitwise: empty and one-atom boxes, a
+non-cubic grid with odd cell counts, lists longer than the initial capacity, and atoms that
+sit exactly on cell faces or on the domain's upper bound."""
+import numpy as np
+import pytest
+
+from exanbody_amd import PRECISION_FP64, cubic_domain
+from tests.helpers import RCUT, SKIN, RHO, assert_same_layout, fcc_inputs, fcc_lattice, make_pair, make_pair_domain
+
+pytestmark = pytest.mark.gpu
+
+A_FCC = (4.0 / RHO) ** (1.0 / 3.0)  # FCC lattice constant at the reference density
+
+
+def _assert_inner_streams(o, b):
+    """CS=1 streams of every inner cell equal the oracle's; returns the atoms they cover."""
+    gs, gmax = b.streams()
+    os_ = o.streams()
+    dims, gl, nc = b.grid_info()
+    counts = b.cell_counts()
+    covered = 0
+    for c in range(nc):
+        k, rem = divmod(c, dims[0] * dims[1])
+        j, i = divmod(rem, dims[0])
+        if all(gl <= x < d - gl for x, d in zip((i, j, k), dims)):
+            np.testing.assert_array_equal(gs[c], os_[c], err_msg=f"stream of cell {c}")
+            covered += counts[c]
+    assert gmax == o.max_neighbors()
+    return covered, gmax
+
+
+def _assert_same_state(o, b, what, ghost_forces=True):
+    """Layout with ghost images, velocities, and forces and energies: of every slot, or of the
+    inner atoms only (ghost_forces False: after a run, whose ghost images carry f = ep = 0
+    while the oracle's carry copies of their sources' previous-step values)."""
+    oe, ge = o.export(ghosts=True), b.export(ghosts=True)
+    assert_same_layout(oe, ge, what)
+    np.testing.assert_array_equal(oe["v"].view(np.uint64), ge["v"].view(np.uint64), err_msg=f"{what}: v")
+    oi, gi = (oe, ge) if ghost_forces else (o.export(), b.export())
+    for k in ("f", "ep"):
+        np.testing.assert_array_equal(oi[k].view(np.uint64), gi[k].view(np.uint64), err_msg=f"{what}: {k}")
+
+
+def _run_both(o, b, steps, dt, rebuild_every):
+    ro = o.run(steps, dt, 1.0, 1.0, RCUT, SKIN, rebuild_every=rebuild_every)
+    rg = b.run(steps, dt, 1.0, 1.0, RCUT, SKIN, precision=PRECISION_FP64, rebuild_every=rebuild_every)
+    assert ro == rg
+    return ro
+
+
+@pytest.mark.parametrize("n", [0, 1])
+def test_empty_and_single_atom_boxes(n):
+    """A box with no atoms, and one with a single atom whose periodic images lie beyond the
+    cutoff: init and a run with rebuilds succeed and end equal to the oracle (the lone atom
+    drifts with zero force)."""
+    dom = cubic_domain(9.0, RCUT + SKIN)
+    r = np.array([[4.5, 0.01, 8.99]])[:n]
+    v = np.array([[0.3, -0.7, 0.4]])[:n]
+    o, b = make_pair_domain(r, v, np.arange(n, dtype=np.uint64), dom)
+    o.init(1.0, 1.0, RCUT, SKIN)
+    b.init(1.0, 1.0, RCUT, SKIN, PRECISION_FP64)
+    _assert_same_state(o, b, "init")
+    _run_both(o, b, 40, 0.005, 5)
+    _assert_same_state(o, b, "after run", ghost_forces=False)
+    assert b.energies() == o.energies()
+    if n:
+        assert not b.export()["f"].any()
+
+
+@pytest.mark.parametrize("rebuild_every", [0, 10])
+def test_non_cubic_grid_bitwise(rebuild_every):
+    """10 x 14 x 6 FCC cells in a 5 x 7 x 3 cell grid: uneven brick cover on every axis."""
+    r, v, ids, lengths = fcc_lattice((10, 14, 6), A_FCC, jitter=0.02, temperature=0.9, seed=5)
+    cs = 2.0 * A_FCC
+    dims = tuple(int(round(x / cs)) for x in lengths)
+    assert dims == (5, 7, 3)
+    dom = dict(bounds_min=(0.0,) * 3, bounds_max=tuple(lengths), grid_dims=dims, cell_size=cs, periodic=(1, 1, 1))
+    o, b = make_pair_domain(r, v, ids, dom)
+    o.init(1.0, 1.0, RCUT, SKIN)
+    b.init(1.0, 1.0, RCUT, SKIN, PRECISION_FP64)
+    _assert_same_state(o, b, "init")
+    covered, _ = _assert_inner_streams(o, b)
+    assert covered == len(r)
+    res = _run_both(o, b, 30, 0.005, rebuild_every)
+    assert res["rebuilds"] >= 2
+    _assert_same_state(o, b, "after run", ghost_forces=False)
+    # the ghost_update_all operator: images copy f and ep of their sources too
+    o.ghost_update_all()
+    b.ghost_update_all()
+    _assert_same_state(o, b, "ghost_update_all")
+    assert np.abs(b.export(ghosts=True)["f"]).max() > 0
+
+
+@pytest.mark.parametrize("bricks", [None, "0"])
+def test_lists_longer_than_initial_capacity(monkeypatch, bricks):
+    """A fluid at density 1.6 has up to 140 neighbours within rc + skin, more than the lists
+    are first sized for: both list builders grow their capacity and match the oracle."""
+    if bricks is None:
+        monkeypatch.delenv("XNB_BRICKS", raising=False)
+    else:
+        monkeypatch.setenv("XNB_BRICKS", bricks)
+    r, v, ids, L = fcc_inputs(10, rho=1.6, jitter=0.02, temperature=0.72, seed=4)
+    o, b, dom = make_pair(r, v, ids, L)
+    o.init(1.0, 1.0, RCUT, SKIN)
+    b.init(1.0, 1.0, RCUT, SKIN, PRECISION_FP64)
+    _assert_same_state(o, b, "init")
+    covered, gmax = _assert_inner_streams(o, b)
+    assert gmax > 96 and covered == len(r)
+    _run_both(o, b, 10, 0.001, 5)
+    _assert_same_state(o, b, "after run", ghost_forces=False)
+
+
+@pytest.mark.parametrize("drift", [0.0, 2e-3])
+def test_atoms_on_cell_faces(drift):
+    """An unjittered FCC lattice whose planes coincide with cell faces (cell edge = two lattice
+    constants), with the atoms of the x = 0 plane given at exactly x = L; zero or uniform
+    (+d, -d, +d) velocities and re-binning at every step.  Cell of each atom, order within
+    cells, ghosts and forces stay equal to the oracle's."""
+    r, _, ids, lengths = fcc_lattice((8, 8, 8), A_FCC)
+    L = lengths[0]
+    r[r[:, 0] == 0.0, 0] = L
+    v = np.tile(np.array([drift, -drift, drift]), (len(r), 1))
+    dom = cubic_domain(L, RCUT + SKIN)
+    assert dom["grid_dims"] == (4, 4, 4)
+    o, b = make_pair_domain(r, v, ids, dom)
+    o.init(1.0, 1.0, RCUT, SKIN)
+    b.init(1.0, 1.0, RCUT, SKIN, PRECISION_FP64)
+    _assert_same_state(o, b, "init")
+    _run_both(o, b, 20, 0.005, 1)
+    _assert_same_state(o, b, "after run", ghost_forces=False)

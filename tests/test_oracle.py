@@ -285,6 +285,31 @@ def test_move_invariants_and_free_flight():
             assert lo - 1e-12 <= got[p, ax] <= lo + cs + 1e-12
 
 
+def test_lone_atom_and_empty_runs():
+    """A single atom whose images lie beyond the cutoff flies ballistically through a run with
+    rebuilds (velocity unchanged, position r0 + n dt v wrapped into the box); an empty box
+    runs and reports zero energies."""
+    L, steps, dt = 9.0, 40, 0.005
+    r0 = np.array([[4.5, 0.01, 8.99]])
+    v0 = np.array([[0.3, -0.7, 0.4]])
+    o = Oracle([0, 0, 0], [L] * 3, [3] * 3, L / 3, ghost_dist=2.8)
+    o.set_particles(r0, v0)
+    o.init(1.0, 1.0, 2.5, 0.3)
+    res = o.run(steps, dt, 1.0, 1.0, 2.5, 0.3, rebuild_every=5)
+    assert res["rebuilds"] == steps // 5
+    e = o.export()
+    np.testing.assert_array_equal(e["v"], v0)
+    assert not e["f"].any() and not e["ep"].any()
+    dd = e["r"] - np.mod(r0 + steps * dt * v0, L)
+    assert np.abs(dd).max() < 1e-12
+    assert np.all(e["r"] >= 0) and np.all(e["r"] < L)
+    o = Oracle([0, 0, 0], [L] * 3, [3] * 3, L / 3, ghost_dist=2.8)
+    o.set_particles(np.zeros((0, 3)))
+    o.init(1.0, 1.0, 2.5, 0.3)
+    o.run(steps, dt, 1.0, 1.0, 2.5, 0.3, rebuild_every=5)
+    assert o.num_particles() == 0 and o.energies() == (0.0, 0.0)
+
+
 def test_multimat_single_type_equals_singlemat():
     a = (4 / RHO) ** (1 / 3)
     r, L = fcc(4, a, jitter=0.05, seed=8)
