@@ -17,7 +17,6 @@ XNB_E_STATE = -4
 XNB_E_HIP = -5
 XNB_E_NOMEM = -6
 XNB_E_OVERFLOW = -7
-XNB_E_AMR = -8
 XNB_E_CAPACITY = -9
 
 PRECISION_FP64 = 0
@@ -64,6 +63,7 @@ _SIGS = {
     "xnb_get_cell_counts": (C.c_int, [P, P]),
     "xnb_move_particles": (C.c_int, [P]),
     "xnb_backup_r": (C.c_int, [P]),
+    "xnb_rebuild_amr": (C.c_int, [P, C.c_double, C.c_int]),
     "xnb_ghost_update_all": (C.c_int, [P]),
     "xnb_ghost_update_r": (C.c_int, [P]),
     "xnb_particle_displ_over": (C.c_int, [P, C.c_double, C.POINTER(C.c_uint64)]),

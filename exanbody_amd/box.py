@@ -125,6 +125,9 @@ class Box:
     def backup_r(self):
         self._check(self.L.xnb_backup_r(self._h), "backup_r")
 
+    def rebuild_amr(self, sub_grid_density=6.5, enforced_ordering=1):
+        self._check(self.L.xnb_rebuild_amr(self._h, float(sub_grid_density), int(enforced_ordering)), "rebuild_amr")
+
     def ghost_update_all(self):
         self._check(self.L.xnb_ghost_update_all(self._h), "ghost_update_all")
 

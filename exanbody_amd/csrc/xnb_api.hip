@@ -277,6 +277,15 @@ int op_ghost_all(xnb_ctx* c, bool move_check, bool forces) {
   return XNB_OK;
 }
 
+// rebuild_amr after a move (ghost cells empty): dense inner cells sorted by sub-cell through
+// the spare buffers; pk is free per-slot scratch between moves
+int op_amr(xnb_ctx* c, double density, bool forces) {
+  if (c->N == 0) return XNB_OK;
+  launch_rebuild_amr(c->g, density, c->A, c->B, c->cell_off.p, c->cell_n.p, c->pk.p, forces, c->st);
+  HIPCHK(c, hipGetLastError());
+  return XNB_OK;
+}
+
 int op_backup(xnb_ctx* c) {
   launch_backup_r(c->g, c->N, c->A.r, c->slot_cell_A.p, c->backup.p, c->st);
   HIPCHK(c, hipGetLastError());
@@ -463,6 +472,7 @@ int launch_force_multimat(xnb_ctx* c, int precision, const LJMulti& mm, int ntyp
 int full_update(xnb_ctx* c, bool forces) {
   int e;
   if ((e = op_move(c, forces, false))) return e;
+  if ((e = op_amr(c, kSubGridDensity, forces))) return e;
   if ((e = op_backup(c))) return e;
   if ((e = op_ghost_all(c, true, forces))) return e;
   xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
@@ -496,7 +506,6 @@ const char* xnb_error_string(int code) {
     case XNB_E_HIP: return "HIP runtime error";
     case XNB_E_NOMEM: return "device allocation failed";
     case XNB_E_OVERFLOW: return "neighbour list overflow";
-    case XNB_E_AMR: return "cell needs AMR sub-grids";
     case XNB_E_CAPACITY: return "capacity exceeded";
     default: return "unknown error";
   }
@@ -925,6 +934,19 @@ int xnb_backup_r(xnb_ctx* c) {
   int e;
   if ((e = check_ctx(c))) return e;
   return op_backup(c);
+}
+
+int xnb_rebuild_amr(xnb_ctx* c, double sub_grid_density, int enforced_ordering) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (!(sub_grid_density > 0.)) return fail(c, XNB_E_ARG, "sub_grid_density must be positive");
+  if (enforced_ordering > 1) return fail(c, XNB_E_ARG, "rebuild_amr: only enforced_ordering 1 (one sub-cell level) is implemented");
+  if (c->G > 0) return fail(c, XNB_E_STATE, "rebuild_amr runs after move_particles, before ghost_update_all");
+  if ((e = op_amr(c, sub_grid_density, true))) return e;
+  c->lists_valid = false;
+  c->ghosts_valid = false;
+  c->backup_valid = false;
+  return XNB_OK;
 }
 
 int xnb_ghost_update_all(xnb_ctx* c) {

@@ -419,7 +419,7 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
     }
     if ((e = upload_tables(B, T1))) return e;
     const uint32_t* W = B->dwork.p;
-    // 2. rebuilds (move_particles, backup_r, ghosts, chunk_neighbors), one launch each
+    // 2. rebuilds (move_particles, rebuild_amr, backup_r, ghosts, chunk_neighbors), one launch each
     if (!reb.empty()) {
       launch_batch_zero_words(B->dwords.p, W + o_list, (uint32_t)reb.size(), B->st);
       launch_batch_move(B->dF.p, W + o_rs, (uint32_t)n_rs, W + o_rc, (uint32_t)n_rc, W + o_list, (uint32_t)reb.size(), par, B->st);
@@ -428,7 +428,8 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
         B->box[i].f.rcur ^= 1u;
         B->box[i].f.G = 0;
       }
-      launch_batch_ghosts(B->dF.p, W + o_rs, (uint32_t)n_rs, W + o_rg, (uint32_t)n_rg, W + o_list, (uint32_t)reb.size(), par, B->st);
+      launch_batch_ghosts(B->dF.p, W + o_rs, (uint32_t)n_rs, W + o_rc, (uint32_t)n_rc, W + o_rg, (uint32_t)n_rg, W + o_list,
+                          (uint32_t)reb.size(), par, B->st);
       hipError_t he = launch_batch_nbh(B->dF.p, W + o_rb, (uint32_t)n_rb, Treb, ldsreb, par, B->st);
       if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched neighbour build: ") + hipGetErrorString(he));
       BHIP(B, hipMemcpyAsync(B->hwords, B->dwords.p, 4 * kMaxBatchBoxes * sizeof(uint32_t), hipMemcpyDeviceToHost, B->st));

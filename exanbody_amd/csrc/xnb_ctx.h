@@ -135,6 +135,7 @@ int ensure_capacity(xnb_ctx* c, size_t ncap, size_t keep);
 int check_ctx(xnb_ctx* c);
 int op_ghost_all(xnb_ctx* c, bool move_check = false, bool forces = true);
 int op_backup(xnb_ctx* c);
+int op_amr(xnb_ctx* c, double density, bool forces);
 int op_nbh(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& cfg, int all_cells);
 int full_update(xnb_ctx* c, bool forces = true);  // forces false: f and ep neither carried by the move nor copied to ghosts
 void tstart(xnb_ctx* c, int id);

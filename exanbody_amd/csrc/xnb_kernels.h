@@ -89,8 +89,10 @@ void launch_batch_zero_cells(const BoxFrame* F, const uint32_t* work, uint32_t n
 void launch_batch_zero_words(uint32_t* words, const uint32_t* boxes, uint32_t n, hipStream_t st);
 void launch_batch_move(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wcells, uint32_t ncells,
                        const uint32_t* boxes, uint32_t nboxes, uint32_t par, hipStream_t st);
-void launch_batch_ghosts(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wghost, uint32_t nghost,
-                         const uint32_t* boxes, uint32_t nboxes, uint32_t par, hipStream_t st);
+// rebuild_amr, backup_r, ghost_comm_scheme + ghost_update_all of the listed boxes after their move
+void launch_batch_ghosts(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wcells, uint32_t ncells,
+                         const uint32_t* wghost, uint32_t nghost, const uint32_t* boxes, uint32_t nboxes, uint32_t par,
+                         hipStream_t st);
 void launch_batch_ghost_update_r(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t par, hipStream_t st);
 void launch_batch_first_half(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t par, uint32_t gstep, hipStream_t st);
 void launch_batch_trigger(const BoxFrame* F, const uint32_t* boxes, uint32_t n, uint32_t gstep, uint32_t* out, hipStream_t st);
@@ -116,6 +118,13 @@ void launch_move_incoming(const Geom& g, const PState& A, const PState& B, const
                           bool forces, hipStream_t st);
 void launch_set_inner_cells(const Geom& g, const uint32_t* new_n, const uint32_t* new_off, uint32_t* cell_n, uint32_t* cell_off,
                             hipStream_t st);
+// rebuild_amr: dense inner cells of A sorted by sub-cell, through B (spare) and back; key: a
+// per-slot scratch array
+void launch_rebuild_amr(const Geom& g, double density, const PState& A, const PState& B, const uint32_t* cell_off,
+                        const uint32_t* cell_n, uint32_t* key, bool forces, hipStream_t st);
+// sub_grid_density of rebuild_amr in the reference's update deck (update-particles.msp:8-11),
+// used by the run loops and batches
+constexpr double kSubGridDensity = 6.5;
 void launch_ghost_count(const Geom& g, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, uint32_t* gcnt,
                         hipStream_t st);
 void launch_ghost_fill(const Geom& g, uint32_t N, const PState& B, const uint32_t* goff, uint32_t* cell_off, uint32_t* cell_n,

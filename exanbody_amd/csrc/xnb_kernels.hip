@@ -288,6 +288,89 @@ __global__ void k_set_inner_cells(uint32_t ncells, const uint32_t* __restrict__ 
   set_inner_cells_item(ncells, new_n, new_off, g, cell_n, cell_off, c);
 }
 
+// -------------------------------------------------------------------- rebuild_amr
+// rebuild_amr (rebuild_amr.cpp:34-69, enforced_ordering 1, no Z order) =
+// project_particles_in_sub_grids (amr_grid_algorithm.h:112-374).  A cell of n particles has a
+// sub-grid of side sub_grid_size(n, density) (amr_grid_algorithm.h:66-77); a cell whose side
+// exceeds 1 gets its particles stably sorted by sub-cell (k s + j) s + i, each index
+// clamp((long)(pcoord * s), 0, s - 1) with pcoord = (r - cell_low) / cell_size
+// (Grid::particle_pcoord, grid.h:184-190).  Runs after move_particles, when ghost cells are
+// empty, so only inner cells are visited.
+
+// floor(cbrt(n / density)), 1 below 2, at most 16, as the largest s with s^3 <= n / density:
+// the same integer wherever n / density is not within an ulp below a perfect cube (never for
+// density 6.5: 2n / 13 is an integer or at least 1/13 from one)
+__host__ __device__ inline int amr_side(uint32_t n, double density) {
+  if (n == 0) return 0;
+  const double x = (double)n / density;
+  if (!(x >= 8.0)) return 1;
+  int s = 2;
+  while (s < 16 && (double)((s + 1) * (s + 1) * (s + 1)) <= x) s++;
+  return s;
+}
+
+// One thread per cell: a dense inner cell's particles copied from A into B at their sorted
+// slots (the same slot range); key[] holds each slot's sub-cell index.  Stable rank: keys
+// below, plus equal keys earlier in the cell.  O(n^2) per dense cell.
+__device__ inline void amr_sort_item(const Geom& g, double density, const PState& A, const PState& B,
+                                     const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
+                                     uint32_t* __restrict__ key, uint32_t c, bool forces) {
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  if (is_ghost(g, i, j, k)) return;
+  const uint32_t n = cell_n[c];
+  const int sg = amr_side(n, density);
+  if (sg <= 1) return;
+  const double lo[3] = {g.ox + (double)(i - g.gl) * g.cs, g.oy + (double)(j - g.gl) * g.cs, g.oz + (double)(k - g.gl) * g.cs};
+  const uint32_t off = cell_off[c];
+  for (uint32_t p = 0; p < n; p++) {
+    const double4 r = A.r[off + p];
+    const double x[3] = {r.x, r.y, r.z};
+    int q[3];
+    for (int a = 0; a < 3; a++) {
+      const long long v = (long long)(((x[a] - lo[a]) / g.cs) * (double)sg);
+      q[a] = v < 0 ? 0 : (v > sg - 1 ? sg - 1 : (int)v);
+    }
+    key[off + p] = (uint32_t)((q[2] * sg + q[1]) * sg + q[0]);
+  }
+  for (uint32_t p = 0; p < n; p++) {
+    const uint32_t kp = key[off + p];
+    uint32_t rank = 0;
+    for (uint32_t t = 0; t < n; t++) {
+      const uint32_t kt = key[off + t];
+      rank += (kt < kp || (kt == kp && t < p)) ? 1u : 0u;
+    }
+    copy_particle(A, off + p, B, off + rank, forces);
+  }
+}
+
+// the sorted cells back from B into A (slot cells are unchanged)
+__device__ inline void amr_back_item(const Geom& g, double density, const PState& A, const PState& B,
+                                     const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n, uint32_t c,
+                                     bool forces) {
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  if (is_ghost(g, i, j, k)) return;
+  const uint32_t n = cell_n[c];
+  if (amr_side(n, density) <= 1) return;
+  const uint32_t off = cell_off[c];
+  for (uint32_t p = 0; p < n; p++) copy_particle(B, off + p, A, off + p, forces);
+}
+
+__global__ void k_amr_sort(Geom g, double density, PState A, PState B, const uint32_t* __restrict__ cell_off,
+                           const uint32_t* __restrict__ cell_n, uint32_t* __restrict__ key, int forces) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncells) return;
+  amr_sort_item(g, density, A, B, cell_off, cell_n, key, c, forces != 0);
+}
+
+__global__ void k_amr_back(Geom g, double density, PState A, PState B, const uint32_t* __restrict__ cell_off,
+                           const uint32_t* __restrict__ cell_n, int forces) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncells) return;
+  amr_back_item(g, density, A, B, cell_off, cell_n, c, forces != 0);
+}
+
 // ------------------------------------------------------------------------ ghosts
 // ghost_comm_scheme + ghost_update_all on one rank (update_ghosts_comm_scheme.cpp:400-470):
 // ghost cell at domain location gd takes inner cell gd - shift*D; a source particle is kept
@@ -931,6 +1014,11 @@ void launch_set_inner_cells(const Geom& g, const uint32_t* new_n, const uint32_t
                             hipStream_t st) {
   k_set_inner_cells<<<nblk(g.ncells, 256), 256, 0, st>>>(g.ncells, new_n, new_off, g, cell_n, cell_off);
 }
+void launch_rebuild_amr(const Geom& g, double density, const PState& A, const PState& B, const uint32_t* cell_off,
+                        const uint32_t* cell_n, uint32_t* key, bool forces, hipStream_t st) {
+  k_amr_sort<<<nblk(g.ncells, 256), 256, 0, st>>>(g, density, A, B, cell_off, cell_n, key, forces ? 1 : 0);
+  k_amr_back<<<nblk(g.ncells, 256), 256, 0, st>>>(g, density, A, B, cell_off, cell_n, forces ? 1 : 0);
+}
 void launch_ghost_count(const Geom& g, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, uint32_t* gcnt,
                         hipStream_t st) {
   k_ghost_count<<<nblk(ghost_cell_count(g) * 64u, 256), 256, 0, st>>>(g, r, cell_off, cell_n, gcnt);
@@ -1162,6 +1250,27 @@ __global__ void __launch_bounds__(kBatchSlotBlock) kb_set_inner_cells(const BoxF
   set_inner_cells_item(f.g.ncells, f.new_n, f.new_off, f.g, f.cell_n, f.cell_off, c);
 }
 
+// rebuild_amr of the listed cells (after kb_flip: X.A current, X.B spare); work over cells
+__global__ void __launch_bounds__(kBatchSlotBlock) kb_amr_sort(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
+                                                                uint32_t par) {
+  uint32_t box, c;
+  if (!batch_item(work, box, c)) return;
+  const BoxFrame& f = F[box];
+  if (c >= f.g.ncells) return;
+  const FrameState X = frame_state(f, par);
+  amr_sort_item(f.g, kSubGridDensity, X.A, X.B, f.cell_off, f.cell_n, f.pk, c, false);
+}
+
+__global__ void __launch_bounds__(kBatchSlotBlock) kb_amr_back(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
+                                                                uint32_t par) {
+  uint32_t box, c;
+  if (!batch_item(work, box, c)) return;
+  const BoxFrame& f = F[box];
+  if (c >= f.g.ncells) return;
+  const FrameState X = frame_state(f, par);
+  amr_back_item(f.g, kSubGridDensity, X.A, X.B, f.cell_off, f.cell_n, c, false);
+}
+
 // after move_particles: the B arrays become the current ones (the one-box swap_buffers)
 __global__ void kb_flip(BoxFrame* __restrict__ F, const uint32_t* __restrict__ boxes, uint32_t nboxes) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1252,9 +1361,12 @@ void launch_batch_move(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, con
   kb_flip<<<(nboxes + 255) / 256, 256, 0, st>>>(F, boxes, nboxes);
 }
 
-void launch_batch_ghosts(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wghost, uint32_t nghost,
-                         const uint32_t* boxes, uint32_t nboxes, uint32_t par, hipStream_t st) {
+void launch_batch_ghosts(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wcells, uint32_t ncells,
+                         const uint32_t* wghost, uint32_t nghost, const uint32_t* boxes, uint32_t nboxes, uint32_t par,
+                         hipStream_t st) {
   if (!nboxes) return;
+  if (ncells) kb_amr_sort<<<ncells, kBatchSlotBlock, 0, st>>>(F, wcells, par);
+  if (ncells) kb_amr_back<<<ncells, kBatchSlotBlock, 0, st>>>(F, wcells, par);
   if (nslots) kb_backup_r<<<nslots, kBatchSlotBlock, 0, st>>>(F, wslots, par);
   if (nghost) kb_ghost_count<<<nghost, kBatchSlotBlock, 0, st>>>(F, wghost, par);
   kb_scan_cells<<<nboxes, 1024, 0, st>>>(F, boxes, 1);

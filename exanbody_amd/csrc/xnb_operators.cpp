@@ -503,7 +503,13 @@ static void register_ops(Runner& R) {
     r.check(xnb_chunk_neighbors(r.need_ctx("chunk_neighbors"), r.num(p, "nbh_dist_lab", 0, true), &c, 0), "chunk_neighbors");
   };
   // single-rank no-ops of update_particles_full / parallel_update_particles
-  for (const char* nm : {"amr_grid_pairs", "rebuild_amr", "resize_particle_locks", "migrate_cell_particles", "extend_domain",
+  // rebuild_amr (src/amr/rebuild_amr.cpp:34-69): slot defaults 5.0 and 1 as there
+  R.ops["rebuild_amr"] = [](Runner& r, const Node& p) {
+    if (r.flag(p, "amr_z_curve", false)) throw DeckError("rebuild_amr: amr_z_curve is not implemented");
+    r.check(xnb_rebuild_amr(r.need_ctx("rebuild_amr"), r.num(p, "sub_grid_density", 5.0), (int)r.num(p, "enforced_ordering", 1.0)),
+            "rebuild_amr");
+  };
+  for (const char* nm : {"amr_grid_pairs", "resize_particle_locks", "migrate_cell_particles", "extend_domain",
                          "simple_cost_model", "load_balance_rcb", "grid_stats", "chunk_neighbors_stats", "print_grid"})
     R.ops[nm] = [](Runner&, const Node&) {};
   R.ops["trigger_load_balance"] = [](Runner& r, const Node&) { r.out_bool("trigger_load_balance", false); };

@@ -33,7 +33,6 @@ extern "C" {
 #define XNB_E_HIP (-5)
 #define XNB_E_NOMEM (-6)
 #define XNB_E_OVERFLOW (-7)
-#define XNB_E_AMR (-8)
 #define XNB_E_CAPACITY (-9)
 
 #define XNB_PRECISION_FP64 0  /* reference arithmetic, FP64 throughout */
@@ -103,6 +102,13 @@ int xnb_get_cell_counts(xnb_ctx* ctx, uint32_t* counts);
 int xnb_move_particles(xnb_ctx* ctx);
 /* backup_r (src/io/backup_r.cpp:35-116) */
 int xnb_backup_r(xnb_ctx* ctx);
+/* rebuild_amr (src/amr/rebuild_amr.cpp:34-69; project_particles_in_sub_grids,
+ * include/exanb/amr/amr_grid_algorithm.h:66-434): particles of each cell whose sub-grid side
+ * sub_grid_size(n, sub_grid_density) exceeds 1 are stably reordered by sub-cell.  Only
+ * enforced_ordering 1 (values <= 1); runs between move_particles and ghost_update_all, as in
+ * the reference's update deck.  init, runs and batches apply it with density 6.5
+ * (update-particles.msp:8-11). */
+int xnb_rebuild_amr(xnb_ctx* ctx, double sub_grid_density, int enforced_ordering);
 /* ghost_comm_scheme + ghost_update_all (src/mpi/update_ghosts_comm_scheme.cpp:84-489,
  * src/mpi/update_ghosts.cu:45-64) on one rank: periodic self images */
 int xnb_ghost_update_all(xnb_ctx* ctx);

@@ -14,8 +14,6 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libxnb_oracle.so")
 _lib = None
 
-OP_E_AMR = -6
-
 
 class OracleError(RuntimeError):
     pass
@@ -55,6 +53,8 @@ def lib():
         L.op_export.argtypes = [P, I, P, P, P, P, P, P, P]; L.op_export.restype = SZ
         for name in ("op_move_particles", "op_ghost_update_all", "op_ghost_update_r", "op_backup_r"):
             getattr(L, name).argtypes = [P]
+        L.op_rebuild_amr.argtypes = [P, D]
+        L.op_sub_grid_size.argtypes = [C.c_uint32, D]; L.op_sub_grid_size.restype = I
         L.op_particle_displ_over.argtypes = [P, D]; L.op_particle_displ_over.restype = I64
         L.op_chunk_neighbors.argtypes = [P, D, C.POINTER(NbhConfig)]
         L.op_zero_force.argtypes = [P, I]
@@ -161,14 +161,16 @@ class Oracle:
     def backup_r(self):
         _check(self.L.op_backup_r(self._h), "backup_r")
 
+    def rebuild_amr(self, sub_grid_density=6.5):
+        _check(self.L.op_rebuild_amr(self._h, float(sub_grid_density)), "rebuild_amr")
+
     def particle_displ_over(self, threshold):
         return int(self.L.op_particle_displ_over(self._h, float(threshold)))
 
     def chunk_neighbors(self, nbh_dist, build_particle_offset=1, dual_particle_offset=0, random_access=0,
-                        half_symmetric=0, skip_ghosts=0, allow_amr=False):
+                        half_symmetric=0, skip_ghosts=0):
         cfg = NbhConfig(build_particle_offset, dual_particle_offset, random_access, half_symmetric, skip_ghosts)
-        return _check(self.L.op_chunk_neighbors(self._h, float(nbh_dist), C.byref(cfg)), "chunk_neighbors",
-                      allow=(OP_E_AMR,) if allow_amr else ())
+        return _check(self.L.op_chunk_neighbors(self._h, float(nbh_dist), C.byref(cfg)), "chunk_neighbors")
 
     def zero_force(self, ghost=False):
         _check(self.L.op_zero_force(self._h, int(ghost)), "zero_force")
@@ -217,6 +219,10 @@ class Oracle:
 
 def encode_double_u32(x, o, r):
     return lib().op_encode_double_u32(float(x), float(o), float(r))
+
+
+def sub_grid_size(n, density=6.5):
+    return int(lib().op_sub_grid_size(int(n), float(density)))
 
 
 def restore_u32_double(x, o, r):

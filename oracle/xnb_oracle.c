@@ -589,17 +589,74 @@ static int cmp_u32(const void* a, const void* b) {
   return (x > y) - (x < y);
 }
 
-/* sub_grid_size (amr_grid_algorithm.h:66-77), density 6.5 (update-particles.msp:8-11) */
-static int sub_grid_side(uint32_t n) {
+/* sub_grid_size (amr_grid_algorithm.h:66-77): cbrt(n / density), 1 below 2, at most
+ * MAX_SUB_GRID_RESOLUTION = 16 (amr_grid_algorithm.h:46), no power-of-two rounding (:47) */
+static int sub_grid_size(uint32_t n, double density) {
   if (n == 0) return 0;
-  double side = cbrt((double)n / 6.5);
+  double side = cbrt((double)n / density);
   if (side < 2.0) return 1;
-  return (int)floor(side);
+  int s = (int)floor(side);
+  return s < 16 ? s : 16;
 }
 
-/* chunk_neighbors_execute (chunk_neighbors_execute.h:40-423), CS = 1, sub-grid side 1
- * (one sub-cell pair per cell pair, amr_grid_algorithm.cpp:102-218), NeighborFilterHalfSymGhost
- * (neighbor_filter_func.h:35-52).  Streams are built for every cell, ghosts included. */
+int op_sub_grid_size(uint32_t n, double density) { return sub_grid_size(n, density); }
+
+/* rebuild_amr (rebuild_amr.cpp:34-69; enforced_ordering 1: one sub-resolution level, no
+ * Z order) = rebuild_sub_grids + project_particles_in_sub_grids (amr_grid_algorithm.h:
+ * 112-434, 377-434).  In every cell whose side sg = sub_grid_size(n, density) exceeds 1 the
+ * particles are reordered by sub-cell, stably: sub-cell (i, j, k) with each index
+ * clamp((ssize_t)(pcoord * sg), 0, sg - 1), pcoord = (r - cell_low) / cell_size
+ * (Grid::particle_pcoord, grid.h:184-190), index (k * sg + j) * sg + i
+ * (sg_cell_index, amr_grid_algorithm.h:80-93); particles of one sub-cell keep their order
+ * (the counting placement of amr_grid_algorithm.h:225-233 applied by swaps, :246-265). */
+int op_rebuild_amr(op_sim* s, double density) {
+  if (!(density > 0.)) return OP_E_ARG;
+  int err = OP_OK;
+#pragma omp parallel for schedule(dynamic) reduction(| : err)
+  for (size_t c = 0; c < s->ncells; c++) {
+    op_cell* C = &s->cells[c];
+    const uint32_t n = C->n;
+    const int sg = sub_grid_size(n, density);
+    if (sg <= 1) continue;
+    int64_t ci, cj, ck;
+    index_ijk(s->dims, (int64_t)c, &ci, &cj, &ck);
+    const double lo[3] = {cell_pos(s, 0, ci), cell_pos(s, 1, cj), cell_pos(s, 2, ck)};
+    const uint32_t nsub = (uint32_t)(sg * sg * sg);
+    uint32_t* key = (uint32_t*)malloc((size_t)n * sizeof(uint32_t));
+    uint32_t* start = (uint32_t*)calloc((size_t)nsub + 1, sizeof(uint32_t));
+    op_tuple* tmp = (op_tuple*)malloc((size_t)n * sizeof(op_tuple));
+    if (!key || !start || !tmp) {
+      free(key); free(start); free(tmp);
+      err |= 1;
+      continue;
+    }
+    const double* rr[3] = {C->rx, C->ry, C->rz};
+    for (uint32_t p = 0; p < n; p++) {
+      int64_t q[3];
+      for (int a = 0; a < 3; a++) {
+        const double pc = (rr[a][p] - lo[a]) / s->cell_size;
+        int64_t v = (int64_t)(pc * (double)sg);
+        q[a] = v < 0 ? 0 : (v > sg - 1 ? sg - 1 : v);
+      }
+      key[p] = (uint32_t)((q[2] * sg + q[1]) * sg + q[0]);
+      start[key[p] + 1]++;
+    }
+    for (uint32_t t = 0; t < nsub; t++) start[t + 1] += start[t];
+    for (uint32_t p = 0; p < n; p++) cell_get(C, p, &tmp[start[key[p]]++]);
+    for (uint32_t p = 0; p < n; p++) cell_put(C, p, &tmp[p]);
+    free(key); free(start); free(tmp);
+  }
+  s->ghosts_valid = 0;
+  s->streams_valid = 0;
+  return err ? OP_E_ARG : OP_OK;
+}
+
+/* chunk_neighbors_execute (chunk_neighbors_execute.h:40-423), CS = 1, NeighborFilterHalfSymGhost
+ * (neighbor_filter_func.h:35-52).  Streams are built for every cell, ghosts included.  Every
+ * particle pair of two neighbouring cells is tested: the reference's sub-cell pairs
+ * (amr_grid_algorithm.cpp:102-218) only skip pairs of sub-cells farther apart than nbh_dist,
+ * and each particle's entries are sorted by (cell, index) (chunk_neighbors_execute.h:305-324),
+ * so the streams are the same for any sub-grid side. */
 int op_chunk_neighbors(op_sim* s, double nbh_dist_lab, const op_nbh_config* cfg) {
   op_nbh_config dflt = {1, 0, 0, 0, 0};
   if (!cfg) cfg = &dflt;
@@ -611,10 +668,9 @@ int op_chunk_neighbors(op_sim* s, double nbh_dist_lab, const op_nbh_config* cfg)
   free_streams(s);
   for (size_t c = 0; c < s->ncells; c++)
     if (s->cells[c].n >= 65535u) return OP_E_OVERFLOW;
-  int amr_err = 0;
   uint32_t max_nbh = 0;
   const int64_t* d = s->dims;
-#pragma omp parallel reduction(max : max_nbh) reduction(| : amr_err)
+#pragma omp parallel reduction(max : max_nbh)
   {
     uint32_t** plist = NULL;
     uint32_t* pn = NULL;
@@ -628,7 +684,6 @@ int op_chunk_neighbors(op_sim* s, double nbh_dist_lab, const op_nbh_config* cfg)
       index_ijk(d, (int64_t)ca, &ai, &aj, &ak);
       const op_cell* A = &s->cells[ca];
       const uint32_t na = A->n;
-      if (sub_grid_side(na) > 1) amr_err = 1;
       if (na > palloc) {
         plist = (uint32_t**)realloc(plist, na * sizeof(uint32_t*));
         pn = (uint32_t*)realloc(pn, na * sizeof(uint32_t));
@@ -743,7 +798,7 @@ int op_chunk_neighbors(op_sim* s, double nbh_dist_lab, const op_nbh_config* cfg)
   }
   s->max_neighbors = max_nbh;
   s->streams_valid = 1;
-  return amr_err ? OP_E_AMR : OP_OK;
+  return OP_OK;
 }
 
 size_t op_stream_size(const op_sim* s, size_t c) { return c < s->ncells ? s->stream_len[c] : 0; }
@@ -995,17 +1050,17 @@ int op_push_f_v(op_sim* s, double dt, double dt_scale) {
 
 /* ------------------------------------------------------------ composite */
 
-/* parallel_update_particles after move_particles (update-particles.msp:47-53,62-68):
- * backup_r, ghost_comm_scheme + ghost_update_all, chunk_neighbors */
+/* parallel_update_particles after move_particles (update-particles.msp:47-53,59-65):
+ * rebuild_amr (sub_grid_density 6.5, update-particles.msp:8-11), backup_r,
+ * ghost_comm_scheme + ghost_update_all, chunk_neighbors */
 static int full_update(op_sim* s) {
   int e;
   if ((e = op_move_particles(s))) return e;
+  if ((e = op_rebuild_amr(s, OP_SUB_GRID_DENSITY))) return e;
   if ((e = op_backup_r(s))) return e;
   if ((e = op_ghost_update_all(s))) return e;
   op_nbh_config cfg = {1, 0, 0, 0, 0};
-  e = op_chunk_neighbors(s, s->nbh_dist_lab, &cfg);
-  if (e && e != OP_E_AMR) return e;
-  return OP_OK;
+  return op_chunk_neighbors(s, s->nbh_dist_lab, &cfg);
 }
 
 /* nbh_dist (src/particle_neighbors/nbh_dist.cpp:45-72) + init_particles + first forces

@@ -29,7 +29,9 @@ typedef struct op_sim op_sim;
 #define OP_E_OUTSIDE (-3)
 #define OP_E_STATE (-4)
 #define OP_E_OVERFLOW (-5)
-#define OP_E_AMR (-6)
+
+/* sub_grid_density of rebuild_amr in the reference's update deck (update-particles.msp:8-11) */
+#define OP_SUB_GRID_DENSITY 6.5
 
 /* chunk_neighbors config, subset of ChunkNeighborsConfig
  * (src/particle_neighbors/include/exanb/particle_neighbors/chunk_neighbors_config.h:27-39) */
@@ -74,6 +76,8 @@ int op_move_particles(op_sim* s);                       /* move_particles */
 int op_ghost_update_all(op_sim* s);                     /* ghost_comm_scheme + ghost_update_all */
 int op_ghost_update_r(op_sim* s);                       /* ghost_update_r */
 int op_backup_r(op_sim* s);                             /* backup_r */
+int op_rebuild_amr(op_sim* s, double sub_grid_density); /* rebuild_amr (enforced_ordering 1) */
+int op_sub_grid_size(uint32_t n, double density);       /* sub_grid_size of a cell of n particles */
 int64_t op_particle_displ_over(op_sim* s, double threshold); /* particle_displ_over: count */
 int op_chunk_neighbors(op_sim* s, double nbh_dist_lab, const op_nbh_config* cfg); /* chunk_neighbors */
 int op_zero_force(op_sim* s, int ghost);                /* zero_particle_force */

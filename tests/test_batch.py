@@ -22,6 +22,8 @@ JOBS = [
     # rebuilt on the displacement trigger (hot, so atoms pass max_displ every few steps)
     (5, 0.80, 2.50, 17, 40, PRECISION_FP64, 0),
     (6, 0.75, 3.00, 18, 35, PRECISION_MIXED, 0),
+    # about 62 atoms per cell: rebuild_amr reorders every cell at each rebuild
+    (10, 1.60, 0.72, 19, 15, PRECISION_FP64, 5),
 ]
 
 

@@ -91,10 +91,27 @@ def test_non_cubic_grid_bitwise(rebuild_every):
     assert np.abs(b.export(ghosts=True)["f"]).max() > 0
 
 
+def test_rebuild_amr_operator():
+    """rebuild_amr after the initial binning: cells of about 62 particles (sub-grid side 2) are
+    reordered by sub-cell exactly as by the oracle; the order differs from the input order."""
+    r, v, ids, L = fcc_inputs(10, rho=1.6, jitter=0.05, temperature=0.72, seed=6)
+    o, b, dom = make_pair(r, v, ids, L)
+    before = b.export()
+    assert b.cell_counts().max() >= 52
+    o.rebuild_amr()
+    b.rebuild_amr()
+    assert_same_layout(o.export(), b.export(), "rebuild_amr")
+    assert not np.array_equal(before["id"], b.export()["id"])
+    with pytest.raises(Exception):
+        b.rebuild_amr(enforced_ordering=2)
+
+
 @pytest.mark.parametrize("bricks", [None, "0"])
 def test_lists_longer_than_initial_capacity(monkeypatch, bricks):
     """A fluid at density 1.6 has up to 140 neighbours within rc + skin, more than the lists
-    are first sized for: both list builders grow their capacity and match the oracle."""
+    are first sized for: both list builders grow their capacity and match the oracle.  Its
+    cells hold about 62 atoms, so every rebuild also reorders them by sub-cell
+    (rebuild_amr)."""
     if bricks is None:
         monkeypatch.delenv("XNB_BRICKS", raising=False)
     else:
@@ -103,6 +120,7 @@ def test_lists_longer_than_initial_capacity(monkeypatch, bricks):
     o, b, dom = make_pair(r, v, ids, L)
     o.init(1.0, 1.0, RCUT, SKIN)
     b.init(1.0, 1.0, RCUT, SKIN, PRECISION_FP64)
+    assert b.cell_counts().max() >= 52
     _assert_same_state(o, b, "init")
     covered, gmax = _assert_inner_streams(o, b)
     assert gmax > 96 and covered == len(r)

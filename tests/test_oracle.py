@@ -10,7 +10,7 @@ import os
 import numpy as np
 import pytest
 
-from oracle.oracle import Oracle, encode_double_u32, restore_u32_double
+from oracle.oracle import Oracle, encode_double_u32, restore_u32_double, sub_grid_size
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 RHO = 0.8442
@@ -362,6 +362,62 @@ def test_verlet_energy_error_is_second_order():
         dev.append(worst)
         assert worst < 1e-6 * abs(e0) * (dt / 0.004) ** 2 * 1.5
     assert 3.6 < dev[0] / dev[1] < 4.4 and 3.6 < dev[1] / dev[2] < 4.4, dev
+
+
+def test_sub_grid_size_closed_form():
+    """sub_grid_size(n, 6.5) = floor(cbrt(n / 6.5)), 1 below 2 and at most 16, equals the
+    integer form: the largest s with 13 s^3 <= 2 n (for n >= 52), for every n up to 60,000."""
+    for n in range(0, 60000):
+        if n == 0:
+            want = 0
+        elif 13 * 8 > 2 * n:
+            want = 1
+        else:
+            k = 2
+            while 13 * (k + 1) ** 3 <= 2 * n:
+                k += 1
+            want = min(k, 16)
+        assert sub_grid_size(n) == want, n
+
+
+def test_rebuild_amr_stable_sub_cell_order():
+    """rebuild_amr on a dense fluid (about 62 particles per cell, sub-grid side 2): in each
+    cell of 52 or more particles, the new order is a permutation of the old one, sorted by
+    sub-cell index (k s + j) s + i of the clamped cell-relative coordinates, and particles of
+    one sub-cell keep their previous relative order; sparser cells are unchanged."""
+    a = (4 / 1.6) ** (1 / 3)
+    r, L = fcc(10, a, jitter=0.05, seed=5)
+    o = box(L, r)
+    before = o.export()
+    o.rebuild_amr()
+    after = o.export()
+    dims = o.dims
+    gl = o.ghost_layers
+    cs = L / (dims[0] - 2 * gl)
+    counts = o.cell_counts()
+    dense = 0
+    for c in np.unique(before["cell"]):
+        sel_b, sel_a = before["cell"] == c, after["cell"] == c
+        ib, ia = before["id"][sel_b], after["id"][sel_a]
+        assert sorted(ib.tolist()) == sorted(ia.tolist())
+        n = int(counts[c])
+        sg = sub_grid_size(n)
+        if sg <= 1:
+            np.testing.assert_array_equal(ib, ia)
+            continue
+        dense += 1
+        k, rem = divmod(int(c), dims[0] * dims[1])
+        j, i = divmod(rem, dims[0])
+        lo = np.array([(i - gl) * cs, (j - gl) * cs, (k - gl) * cs])
+        q = np.clip(np.trunc((after["r"][sel_a] - lo) / cs * sg), 0, sg - 1).astype(np.int64)
+        key = (q[:, 2] * sg + q[:, 1]) * sg + q[:, 0]
+        assert np.all(np.diff(key) >= 0)
+        rank_before = {int(x): t for t, x in enumerate(ib)}
+        for kk in np.unique(key):
+            pos = [rank_before[int(x)] for x in ia[key == kk]]
+            assert pos == sorted(pos)
+        assert len(np.unique(key)) > 1
+    assert dense > 0
 
 
 def test_multimat_single_type_equals_singlemat():
