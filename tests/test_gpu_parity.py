@@ -62,6 +62,32 @@ def test_fp64_trajectory_bitwise(rebuild_every):
     assert abs(ko - kg) <= 1e-12 * abs(ko) and abs(po - pg) <= 1e-12 * abs(po)
 
 
+def test_fcc32_full_size_100_steps_bitwise():
+    """The reference's CPU-runnable workload at full size: FCC 32^3 (131,072 atoms, 19^3 cells),
+    rho 0.8442, T 0.72, jitter 0.01, 100 steps of dt 0.005 rebuilt every 10.  The FP64 run
+    equals the oracle bit for bit (layout, velocities, per-atom forces and energies, rebuild and
+    guard counts; the box totals, summed in another order, within 1e-12); mixed-precision
+    forces at the final positions are within 1e-5 RMS of FP64."""
+    r, v, ids, L = fcc_inputs(32, temperature=0.72, jitter=0.01, seed=1)
+    o, b, dom = make_pair(r, v, ids, L)
+    assert dom["grid_dims"] == (19, 19, 19)
+    o.init(1.0, 1.0, RCUT, SKIN)
+    b.init(1.0, 1.0, RCUT, SKIN, PRECISION_FP64)
+    ro = o.run(100, 0.005, 1.0, 1.0, RCUT, SKIN, rebuild_every=10)
+    rg = b.run(100, 0.005, 1.0, 1.0, RCUT, SKIN, precision=PRECISION_FP64, rebuild_every=10)
+    assert ro == rg and rg["rebuilds"] == 10
+    oe, ge = o.export(), b.export()
+    assert_same_layout(oe, ge, "after 100 steps")
+    for k in ("v", "f", "ep"):
+        np.testing.assert_array_equal(oe[k].view(np.uint64), ge[k].view(np.uint64), err_msg=k)
+    (ko, po), (kg, pg) = o.energies(), b.energies()
+    assert abs(ko - kg) <= 1e-12 * abs(ko) and abs(po - pg) <= 1e-12 * abs(po)
+    b.zero_particle_force()
+    b.lennard_jones_force(1.0, 1.0, RCUT, precision=PRECISION_MIXED)
+    mx, rms = rel_err(b.export()["f"], ge["f"])
+    assert rms <= 1e-5, (mx, rms)
+
+
 def test_mixed_force_error():
     o, b = _init_both(8, temperature=0.72, seed=3)
     b.run(50, 0.005, 1.0, 1.0, RCUT, SKIN, precision=PRECISION_FP64, rebuild_every=10)
