@@ -46,6 +46,12 @@ class Batch:
         self._check(self.L.xnb_batch_box_recoveries(self._h, int(slot), C.byref(n)), "recoveries")
         return n.value
 
+    def graph_stats(self):
+        """(steps replayed from a captured HIP graph, graphs captured)."""
+        a, b = C.c_int64(), C.c_int64()
+        self._check(self.L.xnb_batch_graph_stats(self._h, C.byref(a), C.byref(b)), "graph_stats")
+        return a.value, b.value
+
     def detach(self, slot):
         """Detach a box; returns dict(rebuilds, violations) like Box.run."""
         p = _lib.RunParams()

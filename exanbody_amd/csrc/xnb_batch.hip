@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -68,7 +69,19 @@ struct xnb_batch {
   int hcur = 0;
   uint32_t par = 0;
   uint32_t gstep = 1;  // global step number of the next step
+  DArr<uint32_t> dclock;  // {par, gstep} on the device, read by the quiet-step kernels
   uint32_t lds_limit = 65536;
+  // HIP graph of a quiet step (no box fresh, rebuilt on displacement or rebuilding): ghost
+  // refresh, force, clock advance, replayed while the step's work table stays the same
+  bool use_graph = true;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint32_t> gtab, last_quiet;  // table of the captured graph / of the last quiet step
+  size_t g_og = 0, g_ng = 0, g_of = 0, g_nf = 0;
+  uint32_t g_Tf = 0, g_lds = 0;
+  DArr<uint32_t> dgwork[2];  // device copies of graph tables (the current graph's and the next)
+  int gcur = 0;
+  std::vector<hipGraphExec_t> retired;  // destroyed after the next stream synchronisation
+  int64_t n_graph_steps = 0, n_captures = 0;
   std::string err;
 };
 
@@ -196,9 +209,8 @@ std::vector<uint32_t> bricks_table(const std::vector<int>& boxes, const std::vec
   return t;
 }
 
-int upload_tables(xnb_batch* B, const Tables& T) {
-  const size_t n = T.v.size();
-  if (n == 0) return XNB_OK;
+// pinned staging (two buffers) and B->dwork hold at least n words
+int ensure_staging(xnb_batch* B, size_t n) {
   if (n > B->hwork_cap) {
     BHIP(B, hipStreamSynchronize(B->st));
     for (int k = 0; k < 2; k++) {
@@ -213,12 +225,79 @@ int upload_tables(xnb_batch* B, const Tables& T) {
     B->dwork.n = cap;
     B->hwork_cap = cap;
   }
+  return XNB_OK;
+}
+
+// T through a staging buffer into dst (n words), in stream order
+int stage_copy(xnb_batch* B, const Tables& T, uint32_t* dst) {
+  const size_t n = T.v.size();
   const int k = B->hcur;
   B->hcur ^= 1;
   BHIP(B, hipEventSynchronize(B->hev[k]));  // the copy that last read this staging buffer is done
   std::memcpy(B->hwork[k], T.v.data(), n * sizeof(uint32_t));
-  BHIP(B, hipMemcpyAsync(B->dwork.p, B->hwork[k], n * sizeof(uint32_t), hipMemcpyHostToDevice, B->st));
+  BHIP(B, hipMemcpyAsync(dst, B->hwork[k], n * sizeof(uint32_t), hipMemcpyHostToDevice, B->st));
   BHIP(B, hipEventRecord(B->hev[k], B->st));
+  return XNB_OK;
+}
+
+int upload_tables(xnb_batch* B, const Tables& T) {
+  const size_t n = T.v.size();
+  if (n == 0) return XNB_OK;
+  int e;
+  if ((e = ensure_staging(B, n))) return e;
+  return stage_copy(B, T, B->dwork.p);
+}
+
+void destroy_retired(xnb_batch* B) {  // after a stream synchronisation
+  for (hipGraphExec_t x : B->retired) (void)hipGraphExecDestroy(x);
+  B->retired.clear();
+}
+
+// Capture the quiet step of table T2 (force table at o_f, ghost refresh at o_g) into a new
+// graph.  Its table goes to the graph buffer not used by the current graph: graphs launched
+// earlier that read that buffer run before the copy (stream order).
+int capture_quiet(xnb_batch* B, const Tables& T2, size_t o_g, size_t n_g, size_t o_f, size_t n_f, uint32_t Tf, uint32_t lds) {
+  const int j = B->gcur ^ 1;
+  const size_t n = T2.v.size();
+  int e;
+  if ((e = ensure_staging(B, n))) return e;
+  if (B->dgwork[j].n < n) {
+    BHIP(B, hipStreamSynchronize(B->st));
+    destroy_retired(B);
+    if (B->dgwork[j].p) (void)hipFree(B->dgwork[j].p);
+    B->dgwork[j].p = nullptr;
+    B->dgwork[j].n = 0;
+    BHIP(B, hipMalloc((void**)&B->dgwork[j].p, B->hwork_cap * sizeof(uint32_t)));
+    B->dgwork[j].n = B->hwork_cap;
+  }
+  if ((e = stage_copy(B, T2, B->dgwork[j].p))) return e;
+  BHIP(B, batch_lj_set_lds(lds));  // attribute set outside the capture
+  const uint32_t* W = B->dgwork[j].p;
+  BHIP(B, hipStreamBeginCapture(B->st, hipStreamCaptureModeRelaxed));
+  launch_batch_ghost_update_r(B->dF.p, W + o_g, (uint32_t)n_g, B->dclock.p, B->st);
+  const hipError_t le = launch_batch_lj(B->dF.p, W + o_f, (uint32_t)n_f, Tf, lds, B->dclock.p, B->st, false);
+  launch_batch_advance(B->dclock.p, B->st);
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(B->st, &g);
+  if (le != hipSuccess || ce != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return bfail(B, XNB_E_HIP, std::string("quiet-step capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+  }
+  hipGraphExec_t x = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess) return bfail(B, XNB_E_HIP, std::string("quiet-step graph: ") + hipGetErrorString(ie));
+  if (B->gexec) B->retired.push_back(B->gexec);
+  B->gexec = x;
+  B->gcur = j;
+  B->gtab = T2.v;
+  B->g_og = o_g;
+  B->g_ng = n_g;
+  B->g_of = o_f;
+  B->g_nf = n_f;
+  B->g_Tf = Tf;
+  B->g_lds = lds;
+  B->n_captures++;
   return XNB_OK;
 }
 
@@ -268,6 +347,13 @@ int xnb_batch_create(xnb_batch** out, int device) {
   ok = ok && hipMalloc((void**)&B->dtrig.p, kMaxBatchBoxes * sizeof(uint32_t)) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->htrig, kMaxBatchBoxes * sizeof(uint32_t), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipEventCreate(&B->hev[0]) == hipSuccess && hipEventCreate(&B->hev[1]) == hipSuccess;
+  ok = ok && hipMalloc((void**)&B->dclock.p, 2 * sizeof(uint32_t)) == hipSuccess;
+  if (ok) {
+    B->dclock.n = 2;
+    const uint32_t clock0[2] = {B->par, B->gstep};
+    ok = hipMemcpy(B->dclock.p, clock0, sizeof(clock0), hipMemcpyHostToDevice) == hipSuccess;
+  }
+  if (const char* s = std::getenv("XNB_BATCH_GRAPH")) B->use_graph = std::atoi(s) != 0;
   if (ok) ok = hipMemset(B->dwords.p, 0, 4 * kMaxBatchBoxes * sizeof(uint32_t)) == hipSuccess;
   if (ok) ok = hipMemset(B->dtrig.p, 0, kMaxBatchBoxes * sizeof(uint32_t)) == hipSuccess;
   // both staging events start recorded, so the first waits on them return at once
@@ -294,7 +380,11 @@ int xnb_batch_destroy(xnb_batch* B) {
   if (B->dtrig.p) (void)hipFree(B->dtrig.p);
   if (B->htrig) (void)hipHostFree(B->htrig);
   if (B->dwork.p) (void)hipFree(B->dwork.p);
+  destroy_retired(B);
+  if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+  if (B->dclock.p) (void)hipFree(B->dclock.p);
   for (int k = 0; k < 2; k++) {
+    if (B->dgwork[k].p) (void)hipFree(B->dgwork[k].p);
     if (B->hwork[k]) (void)hipHostFree(B->hwork[k]);
     if (B->hev[k]) (void)hipEventDestroy(B->hev[k]);
   }
@@ -434,6 +524,7 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
       if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched neighbour build: ") + hipGetErrorString(he));
       BHIP(B, hipMemcpyAsync(B->hwords, B->dwords.p, 4 * kMaxBatchBoxes * sizeof(uint32_t), hipMemcpyDeviceToHost, B->st));
       BHIP(B, hipStreamSynchronize(B->st));
+      destroy_retired(B);
       for (int i : reb) {
         BatchBox& bb = B->box[i];
         const uint32_t* w = B->hwords + 4 * (size_t)i;
@@ -456,15 +547,20 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
         }
       }
     }
-    // 3. ghost refresh of boxes that did not rebuild, then the force of every box
+    // 3. ghost refresh of boxes that did not rebuild, then the force of every box, then the
+    // device clock; a quiet step replays a captured graph of these three launches.  Ghost
+    // refresh blocks cover each box's ghost capacity (the kernel stops at its ghost count), so
+    // the table does not change when rebuilds change ghost counts.
     Tables T2;
     std::vector<int> q2;
     std::vector<uint32_t> nG;
-    for (int i : quiet)
-      if (B->box[i].f.G) {
+    for (int i : quiet) {
+      const BoxFrame& f = B->box[i].f;
+      if (f.cap > f.N) {
         q2.push_back(i);
-        nG.push_back(B->box[i].f.G);
+        nG.push_back(f.cap - f.N);
       }
+    }
     const size_t o_g = T2.add(blocks_table(q2, nG));
     const size_t n_g = T2.v.size() - o_g;
     std::vector<int> frc;
@@ -484,11 +580,36 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
     }
     const size_t o_f = T2.add(bricks_table(frc, nbr, tag));
     const size_t n_f = T2.v.size() - o_f;
-    if ((e = upload_tables(B, T2))) return e;
-    W = B->dwork.p;
-    launch_batch_ghost_update_r(B->dF.p, W + o_g, (uint32_t)n_g, par, B->st);
-    hipError_t he = launch_batch_lj(B->dF.p, W + o_f, (uint32_t)n_f, Tf, ldsf, par, gstep, B->st);
-    if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched force: ") + hipGetErrorString(he));
+    const bool quiet_step = fresh.empty() && disp.empty() && reb.empty() && !T2.v.empty();
+    bool graphed = false;
+    if (quiet_step && B->use_graph) {
+      auto same = [&]() {
+        return B->gexec && T2.v == B->gtab && o_g == B->g_og && n_g == B->g_ng && o_f == B->g_of && n_f == B->g_nf &&
+               Tf == B->g_Tf && ldsf == B->g_lds;
+      };
+      // capture a table seen on two quiet steps in a row (not one-off tables such as a box's last step)
+      if (!same() && T2.v == B->last_quiet && (e = capture_quiet(B, T2, o_g, n_g, o_f, n_f, Tf, ldsf))) return e;
+      if (same()) {
+        BHIP(B, hipGraphLaunch(B->gexec, B->st));
+        graphed = true;
+        B->n_graph_steps++;
+      }
+      B->last_quiet = T2.v;
+      if (B->retired.size() > 4) {
+        BHIP(B, hipStreamSynchronize(B->st));
+        destroy_retired(B);
+      }
+    } else {
+      B->last_quiet.clear();
+    }
+    if (!graphed) {
+      if ((e = upload_tables(B, T2))) return e;
+      W = B->dwork.p;
+      launch_batch_ghost_update_r(B->dF.p, W + o_g, (uint32_t)n_g, B->dclock.p, B->st);
+      hipError_t he = launch_batch_lj(B->dF.p, W + o_f, (uint32_t)n_f, Tf, ldsf, B->dclock.p, B->st);
+      if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched force: ") + hipGetErrorString(he));
+      launch_batch_advance(B->dclock.p, B->st);
+    }
     for (size_t q = 0; q < frc.size(); q++) {
       BatchBox& bb = B->box[frc[q]];
       bb.done++;
@@ -526,6 +647,13 @@ int xnb_batch_box_status(xnb_batch* B, int slot, int64_t* done, int* state) {
 int xnb_batch_box_recoveries(xnb_batch* B, int slot, int64_t* n) {
   if (!B || slot < 0 || slot >= (int)kMaxBatchBoxes || !B->box[slot].used || !n) return XNB_E_ARG;
   *n = B->box[slot].n_recover;
+  return XNB_OK;
+}
+
+int xnb_batch_graph_stats(xnb_batch* B, int64_t* graph_steps, int64_t* captures) {
+  if (!B) return XNB_E_ARG;
+  if (graph_steps) *graph_steps = B->n_graph_steps;
+  if (captures) *captures = B->n_captures;
   return XNB_OK;
 }
 

@@ -953,10 +953,11 @@ __device__ __forceinline__ void lj_batch_case(const BoxFrame& f, uint32_t b, uin
 
 // Force of every brick of every box; precision and epilogue are per box (uniform per
 // workgroup).  EPI_KICK closes a box's run; EPI_NEXT fuses its next step.
-__global__ void __launch_bounds__(1024) kb_lj_brick(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work, uint32_t par,
-                                                    uint32_t gstep) {
+__global__ void __launch_bounds__(1024) kb_lj_brick(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
+                                                    const uint32_t* __restrict__ clock) {
   uint32_t box, b;
   if (!batch_brick(work, box, b)) return;  // whole workgroup
+  const uint32_t par = clock[0], gstep = clock[1];
   const uint32_t epi = b >> kWorkEpiShift;
   b &= kWorkBrickMask;
   const BoxFrame& f = F[box];
@@ -979,12 +980,16 @@ hipError_t launch_batch_nbh(const BoxFrame* F, const uint32_t* work, uint32_t nw
   return hipGetLastError();
 }
 
-hipError_t launch_batch_lj(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds, uint32_t par,
-                           uint32_t gstep, hipStream_t st) {
+hipError_t batch_lj_set_lds(uint32_t lds) { return set_lds((const void*)kb_lj_brick, lds); }
+
+hipError_t launch_batch_lj(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds, const uint32_t* clock,
+                           hipStream_t st, bool set) {
   if (!nwork) return hipSuccess;
-  hipError_t e = set_lds((const void*)kb_lj_brick, lds);
-  if (e != hipSuccess) return e;
-  kb_lj_brick<<<nwork, T, lds, st>>>(F, work, par, gstep);
+  if (set) {
+    hipError_t e = set_lds((const void*)kb_lj_brick, lds);
+    if (e != hipSuccess) return e;
+  }
+  kb_lj_brick<<<nwork, T, lds, st>>>(F, work, clock);
   return hipGetLastError();
 }
 

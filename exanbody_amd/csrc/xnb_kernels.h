@@ -93,13 +93,17 @@ void launch_batch_move(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, con
 void launch_batch_ghosts(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wcells, uint32_t ncells,
                          const uint32_t* wghost, uint32_t nghost, const uint32_t* boxes, uint32_t nboxes, uint32_t par,
                          hipStream_t st);
-void launch_batch_ghost_update_r(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t par, hipStream_t st);
+// clock: device words {parity, global step} of the step being run (kb_advance moves them on)
+void launch_batch_ghost_update_r(const BoxFrame* F, const uint32_t* work, uint32_t nwork, const uint32_t* clock, hipStream_t st);
+void launch_batch_advance(uint32_t* clock, hipStream_t st);
 void launch_batch_first_half(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t par, uint32_t gstep, hipStream_t st);
 void launch_batch_trigger(const BoxFrame* F, const uint32_t* boxes, uint32_t n, uint32_t gstep, uint32_t* out, hipStream_t st);
 hipError_t launch_batch_nbh(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds, uint32_t par,
                             hipStream_t st);
-hipError_t launch_batch_lj(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds, uint32_t par,
-                           uint32_t gstep, hipStream_t st);
+// set_lds false: the workgroup LDS attribute was already raised to lds (launches under capture)
+hipError_t launch_batch_lj(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds, const uint32_t* clock,
+                           hipStream_t st, bool set_lds = true);
+hipError_t batch_lj_set_lds(uint32_t lds);
 constexpr uint32_t kBatchSlotBlock = 256;  // threads per workgroup of the per-slot / per-cell batched kernels
 
 void scan_exclusive(const uint32_t* in, uint32_t n, uint32_t add, uint32_t* out, uint32_t* bsum, uint32_t* total, hipStream_t st);

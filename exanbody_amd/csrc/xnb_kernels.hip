@@ -1312,12 +1312,14 @@ __global__ void __launch_bounds__(kBatchSlotBlock) kb_ghost_fill(const BoxFrame*
   ghost_fill_wave(f.g, f.N, X.A, f.goff, f.cell_off, f.cell_n, X.scA, f.ghost_src, f.ghost_shift, f.cap, f.words + 0, c, false);
 }
 
+// clock: the batch's step state {parity, global step}, advanced by kb_advance after every step
 __global__ void __launch_bounds__(kBatchSlotBlock) kb_ghost_update_r(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
-                                                                      uint32_t par) {
+                                                                      const uint32_t* __restrict__ clock) {
   uint32_t box, q;
   if (!batch_item(work, box, q)) return;
   const BoxFrame& f = F[box];
-  if (q >= f.G) return;
+  if (q >= f.G) return;  // blocks cover the box's ghost capacity
+  const uint32_t par = clock[0];
   ghost_update_r_item(f.g, f.N, f.G, f.rbuf[(f.rcur ^ par) & 1u], f.ghost_src, f.ghost_shift, q);
 }
 
@@ -1373,9 +1375,19 @@ void launch_batch_ghosts(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, c
   if (nghost) kb_ghost_fill<<<nghost, kBatchSlotBlock, 0, st>>>(F, wghost, par);
 }
 
-void launch_batch_ghost_update_r(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t par, hipStream_t st) {
-  if (nwork) kb_ghost_update_r<<<nwork, kBatchSlotBlock, 0, st>>>(F, work, par);
+void launch_batch_ghost_update_r(const BoxFrame* F, const uint32_t* work, uint32_t nwork, const uint32_t* clock, hipStream_t st) {
+  if (nwork) kb_ghost_update_r<<<nwork, kBatchSlotBlock, 0, st>>>(F, work, clock);
 }
+
+// end of a batched step: parity flips, global step + 1 (one thread)
+__global__ void kb_advance(uint32_t* __restrict__ clock) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    clock[0] ^= 1u;
+    clock[1] += 1u;
+  }
+}
+
+void launch_batch_advance(uint32_t* clock, hipStream_t st) { kb_advance<<<1, 64, 0, st>>>(clock); }
 
 // displacement trigger of each listed box for the step at gstep: its particle_displ_over
 // count (local step k = gstep - gstep0 + 1 keeps it at counts[k - 1]) is positive

@@ -36,11 +36,15 @@ def test_deck_bad_yaml():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("trigger", ["displacement", "interval"])
-def test_deck_run_matches_oracle(trigger):
+@pytest.mark.parametrize("trigger,rho", [("displacement", 0.8442), ("interval", 0.8442), ("interval", 1.6)])
+def test_deck_run_matches_oracle(trigger, rho):
+    """rho 1.6: about 62 atoms per cell, so the deck's rebuild_amr step re-sorts every cell."""
     from exanbody_amd import cubic_domain, generate_fcc
     from oracle.oracle import Oracle
-    r, v, ids, L = generate_fcc(7, 0.8442, jitter=0.05, temperature=1.3, seed=17)
+    if rho > 1.0:
+        r, v, ids, L = generate_fcc(10, rho, jitter=0.05, temperature=0.72, seed=19)
+    else:
+        r, v, ids, L = generate_fcc(7, rho, jitter=0.05, temperature=1.3, seed=17)
     dom = cubic_domain(L, 2.8)
     extra = ""
     if trigger == "interval":
