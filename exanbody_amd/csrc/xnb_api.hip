@@ -261,8 +261,9 @@ int op_ghost_all(xnb_ctx* c, bool move_check, bool forces) {
   launch_ghost_count(g, c->A.r, c->cell_off.p, c->cell_n.p, c->gcnt.p, c->st);
   if ((e = scan(c, c->gcnt.p, g.ncells, 0, c->goff.p))) return e;
   tstop(c);
-  uint32_t w[2] = {0, 0};  // u32s[0]: out-of-domain flag, u32s[1]: scan total = ghost count
+  uint32_t w[2] = {0, 0};  // u32s[0]: bit 1 out of the domain, bit 2 too dense; u32s[1]: ghost count
   if ((e = read_u32(c, c->u32s.p, w, 2))) return e;
+  if (w[0] & 2u) return fail(c, XNB_E_CAPACITY, "rebuild_amr: a cell holds more than 4096 particles");
   if (move_check && w[0]) return fail(c, XNB_E_OUTSIDE, "a particle left the domain along a non-periodic axis");
   const uint32_t G = w[1];
   if ((e = ensure_capacity(c, (size_t)c->N + G, c->N))) return e;
@@ -278,10 +279,14 @@ int op_ghost_all(xnb_ctx* c, bool move_check, bool forces) {
 }
 
 // rebuild_amr after a move (ghost cells empty): dense inner cells sorted by sub-cell through
-// the spare buffers; pk is free per-slot scratch between moves
+// the spare buffers; u32s[8] counts the listed cells, a cell too dense to sort sets bit 2 of
+// u32s[0] (reported with the ghost count, or by xnb_rebuild_amr)
 int op_amr(xnb_ctx* c, double density, bool forces) {
   if (c->N == 0) return XNB_OK;
-  launch_rebuild_amr(c->g, density, c->A, c->B, c->cell_off.p, c->cell_n.p, c->pk.p, forces, c->st);
+  int e;
+  if ((e = dalloc(c, c->amr_list, c->g.ncells))) return e;
+  HIPCHK(c, hipMemsetAsync(c->u32s.p + 8, 0, sizeof(uint32_t), c->st));
+  launch_rebuild_amr(c->g, density, c->A, c->B, c->cell_off.p, c->cell_n.p, c->amr_list.p, c->u32s.p + 8, c->u32s.p, forces, c->st);
   HIPCHK(c, hipGetLastError());
   return XNB_OK;
 }
@@ -543,7 +548,7 @@ int xnb_create(xnb_ctx** out, const xnb_domain* d, int device) {
     return fail(nullptr, XNB_E_HIP, "stream creation failed");
   }
   int e;
-  if ((e = set_geometry(c, 0.0)) || (e = dalloc(c, c->u32s, 8)) || (e = dalloc(c, c->u64s, 8)) || (e = dalloc(c, c->red_out, 2))) {
+  if ((e = set_geometry(c, 0.0)) || (e = dalloc(c, c->u32s, 16)) || (e = dalloc(c, c->u64s, 8)) || (e = dalloc(c, c->red_out, 2))) {
     xnb_destroy(c);
     return e;
   }
@@ -587,7 +592,7 @@ int xnb_destroy(xnb_ctx* c) {
   dfree(c->eA); dfree(c->eB); dfree(c->idA); dfree(c->idB);
   for (DArr<uint32_t>* a : {&c->cell_off, &c->cell_n, &c->cnt_out, &c->cnt_in, &c->new_n, &c->new_off, &c->in_cur, &c->gcnt, &c->goff,
                             &c->clen, &c->coff, &c->mv_dst, &c->pk, &c->newloc, &c->in_list, &c->plen, &c->pstart, &c->ghost_src,
-                            &c->ghost_shift, &c->backup, &c->nbr, &c->nbr_cnt, &c->bsum, &c->u32s})
+                            &c->ghost_shift, &c->backup, &c->nbr, &c->nbr_cnt, &c->bsum, &c->u32s, &c->amr_list})
     dfree(*a);
   dfree(c->blist); dfree(c->bcnt);
   dfree(c->stream_buf); dfree(c->u64s); dfree(c->step_counts); dfree(c->red_part); dfree(c->red_out);
@@ -946,6 +951,9 @@ int xnb_rebuild_amr(xnb_ctx* c, double sub_grid_density, int enforced_ordering) 
   c->lists_valid = false;
   c->ghosts_valid = false;
   c->backup_valid = false;
+  uint32_t flag = 0;
+  if ((e = read_u32(c, c->u32s.p, &flag))) return e;
+  if (flag & 2u) return fail(c, XNB_E_CAPACITY, "rebuild_amr: a cell holds more than 4096 particles");
   return XNB_OK;
 }
 

@@ -38,6 +38,7 @@ struct BatchBox {
   int every = 10;         // rebuild every this many steps; 0: when an atom passed max_displ
   bool fresh = true;      // the next step's first half still has to be launched
   uint32_t last_par = 0;  // parity of the box's last step
+  uint32_t ghost_span = 0;  // ghost slots covered by its ghost-refresh blocks (>= G)
   int64_t n_rebuilds = 0, n_violations = 0, n_recover = 0;
   std::vector<char> rebuilt;
   DArr<unsigned long long> counts;
@@ -70,6 +71,7 @@ struct xnb_batch {
   uint32_t par = 0;
   uint32_t gstep = 1;  // global step number of the next step
   DArr<uint32_t> dclock;  // {par, gstep} on the device, read by the quiet-step kernels
+  DArr<uint32_t> amr_list, amr_count;  // rebuild_amr: cells of rebuilding boxes to sort, their number
   uint32_t lds_limit = 65536;
   // HIP graph of a quiet step (no box fresh, rebuilt on displacement or rebuilding): ghost
   // refresh, force, clock advance, replayed while the step's work table stays the same
@@ -348,6 +350,7 @@ int xnb_batch_create(xnb_batch** out, int device) {
   ok = ok && hipHostMalloc((void**)&B->htrig, kMaxBatchBoxes * sizeof(uint32_t), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipEventCreate(&B->hev[0]) == hipSuccess && hipEventCreate(&B->hev[1]) == hipSuccess;
   ok = ok && hipMalloc((void**)&B->dclock.p, 2 * sizeof(uint32_t)) == hipSuccess;
+  ok = ok && hipMalloc((void**)&B->amr_count.p, sizeof(uint32_t)) == hipSuccess;
   if (ok) {
     B->dclock.n = 2;
     const uint32_t clock0[2] = {B->par, B->gstep};
@@ -383,6 +386,8 @@ int xnb_batch_destroy(xnb_batch* B) {
   destroy_retired(B);
   if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
   if (B->dclock.p) (void)hipFree(B->dclock.p);
+  if (B->amr_count.p) (void)hipFree(B->amr_count.p);
+  if (B->amr_list.p) (void)hipFree(B->amr_list.p);
   for (int k = 0; k < 2; k++) {
     if (B->dgwork[k].p) (void)hipFree(B->dgwork[k].p);
     if (B->hwork[k]) (void)hipHostFree(B->hwork[k]);
@@ -401,6 +406,8 @@ int xnb_batch_attach(xnb_batch* B, xnb_ctx* c, const xnb_run_params* p, int* slo
   if ((e = check_ctx(c))) return e;
   if (c->device != B->device) return bfail(B, XNB_E_ARG, "box on another device");
   if (!c->lists_valid || !c->ghosts_valid || !c->backup_valid) return bfail(B, XNB_E_STATE, "xnb_init must run first");
+  // work and rebuild_amr list entries hold a cell or block index in kWorkBoxShift bits
+  if (c->g.ncells > kWorkBlockMask) return bfail(B, XNB_E_CAPACITY, "box has too many cells for a batch");
   int i = -1;
   for (int q = 0; q < (int)kMaxBatchBoxes; q++)
     if (!B->box[q].used) {
@@ -518,8 +525,18 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
         B->box[i].f.rcur ^= 1u;
         B->box[i].f.G = 0;
       }
+      size_t cells = 0;
+      for (uint32_t x : nC) cells += x;
+      if (B->amr_list.n < cells) {
+        if (B->amr_list.p) (void)hipFree(B->amr_list.p);  // waits for work still using it
+        B->amr_list.p = nullptr;
+        B->amr_list.n = 0;
+        BHIP(B, hipMalloc((void**)&B->amr_list.p, cells * sizeof(uint32_t)));
+        B->amr_list.n = cells;
+      }
+      BHIP(B, hipMemsetAsync(B->amr_count.p, 0, sizeof(uint32_t), B->st));
       launch_batch_ghosts(B->dF.p, W + o_rs, (uint32_t)n_rs, W + o_rc, (uint32_t)n_rc, W + o_rg, (uint32_t)n_rg, W + o_list,
-                          (uint32_t)reb.size(), par, B->st);
+                          (uint32_t)reb.size(), B->amr_list.p, B->amr_count.p, par, B->st);
       hipError_t he = launch_batch_nbh(B->dF.p, W + o_rb, (uint32_t)n_rb, Treb, ldsreb, par, B->st);
       if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched neighbour build: ") + hipGetErrorString(he));
       BHIP(B, hipMemcpyAsync(B->hwords, B->dwords.p, 4 * kMaxBatchBoxes * sizeof(uint32_t), hipMemcpyDeviceToHost, B->st));
@@ -538,6 +555,11 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
           bb.err = "a particle left the domain along a non-periodic axis";
           continue;
         }
+        if (w[0] & 8u) {
+          bb.failed = true;
+          bb.err = "rebuild_amr: a cell holds more than 4096 particles";
+          continue;
+        }
         if (w[0] || w[1] > bb.f.bp.cap) {
           bb.n_recover++;
           if ((e = recover_box(B, i, bb.p))) {
@@ -549,16 +571,18 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
     }
     // 3. ghost refresh of boxes that did not rebuild, then the force of every box, then the
     // device clock; a quiet step replays a captured graph of these three launches.  Ghost
-    // refresh blocks cover each box's ghost capacity (the kernel stops at its ghost count), so
-    // the table does not change when rebuilds change ghost counts.
+    // refresh blocks of a box cover 10 % more than the largest ghost count it has had (the
+    // kernel stops at the current count), so the table stays the same over rebuilds that move
+    // ghost counts a little.
     Tables T2;
     std::vector<int> q2;
     std::vector<uint32_t> nG;
     for (int i : quiet) {
-      const BoxFrame& f = B->box[i].f;
-      if (f.cap > f.N) {
+      BatchBox& bb = B->box[i];
+      if (bb.f.G > bb.ghost_span) bb.ghost_span = bb.f.G + bb.f.G / 10 + 1;
+      if (bb.ghost_span) {
         q2.push_back(i);
-        nG.push_back(f.cap - f.N);
+        nG.push_back(bb.ghost_span);
       }
     }
     const size_t o_g = T2.add(blocks_table(q2, nG));

@@ -61,7 +61,9 @@ struct xnb_ctx {
   DArr<uint16_t> stream_buf;
 
   // scan / scalar scratch
-  DArr<uint32_t> bsum, u32s;            // u32s: [0] err, [1] scan total, [2] max count, [3..4] brick stats, [6] brick status
+  DArr<uint32_t> bsum, u32s;            // u32s: [0] err, [1] scan total, [2] max count, [3..4] brick stats, [6] brick status,
+                                        // [7] ghost status, [8] rebuild_amr cell count
+  DArr<uint32_t> amr_list;              // rebuild_amr: cells to sort
   DArr<unsigned long long> u64s;        // [0] displacement count, [1..2] pair counts
   DArr<unsigned long long> step_counts; // per-step displacement counts of xnb_run
   DArr<double> red_part, red_out;

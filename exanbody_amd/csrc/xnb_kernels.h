@@ -89,10 +89,12 @@ void launch_batch_zero_cells(const BoxFrame* F, const uint32_t* work, uint32_t n
 void launch_batch_zero_words(uint32_t* words, const uint32_t* boxes, uint32_t n, hipStream_t st);
 void launch_batch_move(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wcells, uint32_t ncells,
                        const uint32_t* boxes, uint32_t nboxes, uint32_t par, hipStream_t st);
-// rebuild_amr, backup_r, ghost_comm_scheme + ghost_update_all of the listed boxes after their move
+// rebuild_amr, backup_r, ghost_comm_scheme + ghost_update_all of the listed boxes after their
+// move; amr_list holds one word per cell of the listed boxes, *amr_count is 0 on entry; a box
+// with a cell too dense for rebuild_amr gets status bit 8
 void launch_batch_ghosts(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wcells, uint32_t ncells,
-                         const uint32_t* wghost, uint32_t nghost, const uint32_t* boxes, uint32_t nboxes, uint32_t par,
-                         hipStream_t st);
+                         const uint32_t* wghost, uint32_t nghost, const uint32_t* boxes, uint32_t nboxes, uint32_t* amr_list,
+                         uint32_t* amr_count, uint32_t par, hipStream_t st);
 // clock: device words {parity, global step} of the step being run (kb_advance moves them on)
 void launch_batch_ghost_update_r(const BoxFrame* F, const uint32_t* work, uint32_t nwork, const uint32_t* clock, hipStream_t st);
 void launch_batch_advance(uint32_t* clock, hipStream_t st);
@@ -122,10 +124,11 @@ void launch_move_incoming(const Geom& g, const PState& A, const PState& B, const
                           bool forces, hipStream_t st);
 void launch_set_inner_cells(const Geom& g, const uint32_t* new_n, const uint32_t* new_off, uint32_t* cell_n, uint32_t* cell_off,
                             hipStream_t st);
-// rebuild_amr: dense inner cells of A sorted by sub-cell, through B (spare) and back; key: a
-// per-slot scratch array
+// rebuild_amr: dense inner cells of A sorted by sub-cell, through B (spare) and back.  list: one
+// word per cell; *count must be 0 on entry; a cell of more than 4096 particles sets bit 2 of
+// *too_dense and stays unsorted
 void launch_rebuild_amr(const Geom& g, double density, const PState& A, const PState& B, const uint32_t* cell_off,
-                        const uint32_t* cell_n, uint32_t* key, bool forces, hipStream_t st);
+                        const uint32_t* cell_n, uint32_t* list, uint32_t* count, uint32_t* too_dense, bool forces, hipStream_t st);
 // sub_grid_density of rebuild_amr in the reference's update deck (update-particles.msp:8-11),
 // used by the run loops and batches
 constexpr double kSubGridDensity = 6.5;

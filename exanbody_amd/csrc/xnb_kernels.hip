@@ -309,21 +309,43 @@ __host__ __device__ inline int amr_side(uint32_t n, double density) {
   return s;
 }
 
-// One thread per cell: a dense inner cell's particles copied from A into B at their sorted
-// slots (the same slot range); key[] holds each slot's sub-cell index.  Stable rank: keys
-// below, plus equal keys earlier in the cell.  O(n^2) per dense cell.
-__device__ inline void amr_sort_item(const Geom& g, double density, const PState& A, const PState& B,
-                                     const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
-                                     uint32_t* __restrict__ key, uint32_t c, bool forces) {
+// Dense cells are listed first (one thread per cell, rare atomics), then each listed cell is
+// sorted by one wave: sub-cell keys in LDS, each lane's particles ranked against all keys
+// (broadcast LDS reads; stable: keys below plus equal keys earlier in the cell) and copied
+// from A into B at their rank, and in a last pass copied back from B into A.  The copy-back
+// pass is its own launch, so every wave's writes to B are complete before it reads them.
+constexpr uint32_t kAmrMaxKeys = 4096;  // particles of one cell whose keys fit the wave's LDS
+constexpr uint32_t kAmrGrid = 1024;     // waves of the sort and copy-back launches
+
+// inner cell c: appended to list (entry tag | c) if its sub-grid side exceeds 1; a cell of
+// more than kAmrMaxKeys particles sets too_dense instead
+__device__ inline void amr_mark_item(const Geom& g, double density, const uint32_t* __restrict__ cell_n, uint32_t c, uint32_t tag,
+                                     uint32_t* __restrict__ list, uint32_t* __restrict__ count, uint32_t* __restrict__ too_dense,
+                                     uint32_t flag) {
   int i, j, k;
   cell_ijk(g, c, i, j, k);
   if (is_ghost(g, i, j, k)) return;
   const uint32_t n = cell_n[c];
+  if (amr_side(n, density) <= 1) return;
+  if (n > kAmrMaxKeys) {
+    atomicOr(too_dense, flag);
+    return;
+  }
+  list[atomicAdd(count, 1u)] = tag | c;
+}
+
+// one wave (the whole 64-thread workgroup) sorts cell c of A into B
+__device__ inline void amr_sort_wave(const Geom& g, double density, const PState& A, const PState& B,
+                                     const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n, uint32_t c,
+                                     uint32_t* keys, bool forces) {
+  const uint32_t lane = threadIdx.x;
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  const uint32_t n = cell_n[c];
   const int sg = amr_side(n, density);
-  if (sg <= 1) return;
   const double lo[3] = {g.ox + (double)(i - g.gl) * g.cs, g.oy + (double)(j - g.gl) * g.cs, g.oz + (double)(k - g.gl) * g.cs};
   const uint32_t off = cell_off[c];
-  for (uint32_t p = 0; p < n; p++) {
+  for (uint32_t p = lane; p < n; p += 64) {
     const double4 r = A.r[off + p];
     const double x[3] = {r.x, r.y, r.z};
     int q[3];
@@ -331,44 +353,48 @@ __device__ inline void amr_sort_item(const Geom& g, double density, const PState
       const long long v = (long long)(((x[a] - lo[a]) / g.cs) * (double)sg);
       q[a] = v < 0 ? 0 : (v > sg - 1 ? sg - 1 : (int)v);
     }
-    key[off + p] = (uint32_t)((q[2] * sg + q[1]) * sg + q[0]);
+    keys[p] = (uint32_t)((q[2] * sg + q[1]) * sg + q[0]);
   }
-  for (uint32_t p = 0; p < n; p++) {
-    const uint32_t kp = key[off + p];
+  __syncthreads();
+  for (uint32_t p = lane; p < n; p += 64) {
+    const uint32_t kp = keys[p];
     uint32_t rank = 0;
     for (uint32_t t = 0; t < n; t++) {
-      const uint32_t kt = key[off + t];
+      const uint32_t kt = keys[t];
       rank += (kt < kp || (kt == kp && t < p)) ? 1u : 0u;
     }
     copy_particle(A, off + p, B, off + rank, forces);
   }
+  __syncthreads();  // keys are rewritten by the workgroup's next cell
 }
 
-// the sorted cells back from B into A (slot cells are unchanged)
-__device__ inline void amr_back_item(const Geom& g, double density, const PState& A, const PState& B,
-                                     const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n, uint32_t c,
-                                     bool forces) {
-  int i, j, k;
-  cell_ijk(g, c, i, j, k);
-  if (is_ghost(g, i, j, k)) return;
-  const uint32_t n = cell_n[c];
-  if (amr_side(n, density) <= 1) return;
-  const uint32_t off = cell_off[c];
-  for (uint32_t p = 0; p < n; p++) copy_particle(B, off + p, A, off + p, forces);
+__device__ inline void amr_back_wave(const PState& A, const PState& B, const uint32_t* __restrict__ cell_off,
+                                     const uint32_t* __restrict__ cell_n, uint32_t c, bool forces) {
+  const uint32_t n = cell_n[c], off = cell_off[c];
+  for (uint32_t p = threadIdx.x; p < n; p += 64) copy_particle(B, off + p, A, off + p, forces);
 }
 
-__global__ void k_amr_sort(Geom g, double density, PState A, PState B, const uint32_t* __restrict__ cell_off,
-                           const uint32_t* __restrict__ cell_n, uint32_t* __restrict__ key, int forces) {
+__global__ void k_amr_mark(Geom g, double density, const uint32_t* __restrict__ cell_n, uint32_t* __restrict__ list,
+                           uint32_t* __restrict__ count, uint32_t* __restrict__ too_dense) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.ncells) return;
-  amr_sort_item(g, density, A, B, cell_off, cell_n, key, c, forces != 0);
+  amr_mark_item(g, density, cell_n, c, 0u, list, count, too_dense, 2u);
 }
 
-__global__ void k_amr_back(Geom g, double density, PState A, PState B, const uint32_t* __restrict__ cell_off,
-                           const uint32_t* __restrict__ cell_n, int forces) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.ncells) return;
-  amr_back_item(g, density, A, B, cell_off, cell_n, c, forces != 0);
+__global__ void __launch_bounds__(64) k_amr_sort(Geom g, double density, PState A, PState B, const uint32_t* __restrict__ cell_off,
+                                                 const uint32_t* __restrict__ cell_n, const uint32_t* __restrict__ list,
+                                                 const uint32_t* __restrict__ count, int forces) {
+  __shared__ uint32_t keys[kAmrMaxKeys];
+  const uint32_t m = *count;
+  for (uint32_t q = blockIdx.x; q < m; q += gridDim.x)
+    amr_sort_wave(g, density, A, B, cell_off, cell_n, list[q], keys, forces != 0);
+}
+
+__global__ void __launch_bounds__(64) k_amr_back(PState A, PState B, const uint32_t* __restrict__ cell_off,
+                                                 const uint32_t* __restrict__ cell_n, const uint32_t* __restrict__ list,
+                                                 const uint32_t* __restrict__ count, int forces) {
+  const uint32_t m = *count;
+  for (uint32_t q = blockIdx.x; q < m; q += gridDim.x) amr_back_wave(A, B, cell_off, cell_n, list[q], forces != 0);
 }
 
 // ------------------------------------------------------------------------ ghosts
@@ -1015,9 +1041,10 @@ void launch_set_inner_cells(const Geom& g, const uint32_t* new_n, const uint32_t
   k_set_inner_cells<<<nblk(g.ncells, 256), 256, 0, st>>>(g.ncells, new_n, new_off, g, cell_n, cell_off);
 }
 void launch_rebuild_amr(const Geom& g, double density, const PState& A, const PState& B, const uint32_t* cell_off,
-                        const uint32_t* cell_n, uint32_t* key, bool forces, hipStream_t st) {
-  k_amr_sort<<<nblk(g.ncells, 256), 256, 0, st>>>(g, density, A, B, cell_off, cell_n, key, forces ? 1 : 0);
-  k_amr_back<<<nblk(g.ncells, 256), 256, 0, st>>>(g, density, A, B, cell_off, cell_n, forces ? 1 : 0);
+                        const uint32_t* cell_n, uint32_t* list, uint32_t* count, uint32_t* too_dense, bool forces, hipStream_t st) {
+  k_amr_mark<<<nblk(g.ncells, 256), 256, 0, st>>>(g, density, cell_n, list, count, too_dense);
+  k_amr_sort<<<kAmrGrid, 64, 0, st>>>(g, density, A, B, cell_off, cell_n, list, count, forces ? 1 : 0);
+  k_amr_back<<<kAmrGrid, 64, 0, st>>>(A, B, cell_off, cell_n, list, count, forces ? 1 : 0);
 }
 void launch_ghost_count(const Geom& g, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, uint32_t* gcnt,
                         hipStream_t st) {
@@ -1250,25 +1277,39 @@ __global__ void __launch_bounds__(kBatchSlotBlock) kb_set_inner_cells(const BoxF
   set_inner_cells_item(f.g.ncells, f.new_n, f.new_off, f.g, f.cell_n, f.cell_off, c);
 }
 
-// rebuild_amr of the listed cells (after kb_flip: X.A current, X.B spare); work over cells
-__global__ void __launch_bounds__(kBatchSlotBlock) kb_amr_sort(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
-                                                                uint32_t par) {
+// rebuild_amr of the rebuilding boxes (after kb_flip: X.A current, X.B spare): dense cells of
+// every box listed (entry box << kWorkBoxShift | cell) over the cells work table, then sorted and
+// copied back by one wave each
+__global__ void __launch_bounds__(kBatchSlotBlock) kb_amr_mark(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
+                                                                uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
   uint32_t box, c;
   if (!batch_item(work, box, c)) return;
   const BoxFrame& f = F[box];
   if (c >= f.g.ncells) return;
-  const FrameState X = frame_state(f, par);
-  amr_sort_item(f.g, kSubGridDensity, X.A, X.B, f.cell_off, f.cell_n, f.pk, c, false);
+  amr_mark_item(f.g, kSubGridDensity, f.cell_n, c, box << kWorkBoxShift, list, count, f.words + 0, 8u);
 }
 
-__global__ void __launch_bounds__(kBatchSlotBlock) kb_amr_back(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
-                                                                uint32_t par) {
-  uint32_t box, c;
-  if (!batch_item(work, box, c)) return;
-  const BoxFrame& f = F[box];
-  if (c >= f.g.ncells) return;
-  const FrameState X = frame_state(f, par);
-  amr_back_item(f.g, kSubGridDensity, X.A, X.B, f.cell_off, f.cell_n, c, false);
+__global__ void __launch_bounds__(64) kb_amr_sort(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ list,
+                                                  const uint32_t* __restrict__ count, uint32_t par) {
+  __shared__ uint32_t keys[kAmrMaxKeys];
+  const uint32_t m = *count;
+  for (uint32_t q = blockIdx.x; q < m; q += gridDim.x) {
+    const uint32_t e = list[q];
+    const BoxFrame& f = F[e >> kWorkBoxShift];
+    const FrameState X = frame_state(f, par);
+    amr_sort_wave(f.g, kSubGridDensity, X.A, X.B, f.cell_off, f.cell_n, e & kWorkBlockMask, keys, false);
+  }
+}
+
+__global__ void __launch_bounds__(64) kb_amr_back(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ list,
+                                                  const uint32_t* __restrict__ count, uint32_t par) {
+  const uint32_t m = *count;
+  for (uint32_t q = blockIdx.x; q < m; q += gridDim.x) {
+    const uint32_t e = list[q];
+    const BoxFrame& f = F[e >> kWorkBoxShift];
+    const FrameState X = frame_state(f, par);
+    amr_back_wave(X.A, X.B, f.cell_off, f.cell_n, e & kWorkBlockMask, false);
+  }
 }
 
 // after move_particles: the B arrays become the current ones (the one-box swap_buffers)
@@ -1364,11 +1405,14 @@ void launch_batch_move(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, con
 }
 
 void launch_batch_ghosts(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wcells, uint32_t ncells,
-                         const uint32_t* wghost, uint32_t nghost, const uint32_t* boxes, uint32_t nboxes, uint32_t par,
-                         hipStream_t st) {
+                         const uint32_t* wghost, uint32_t nghost, const uint32_t* boxes, uint32_t nboxes, uint32_t* amr_list,
+                         uint32_t* amr_count, uint32_t par, hipStream_t st) {
   if (!nboxes) return;
-  if (ncells) kb_amr_sort<<<ncells, kBatchSlotBlock, 0, st>>>(F, wcells, par);
-  if (ncells) kb_amr_back<<<ncells, kBatchSlotBlock, 0, st>>>(F, wcells, par);
+  if (ncells) {
+    kb_amr_mark<<<ncells, kBatchSlotBlock, 0, st>>>(F, wcells, amr_list, amr_count);
+    kb_amr_sort<<<kAmrGrid, 64, 0, st>>>(F, amr_list, amr_count, par);
+    kb_amr_back<<<kAmrGrid, 64, 0, st>>>(F, amr_list, amr_count, par);
+  }
   if (nslots) kb_backup_r<<<nslots, kBatchSlotBlock, 0, st>>>(F, wslots, par);
   if (nghost) kb_ghost_count<<<nghost, kBatchSlotBlock, 0, st>>>(F, wghost, par);
   kb_scan_cells<<<nboxes, 1024, 0, st>>>(F, boxes, 1);
