@@ -29,6 +29,8 @@ HBM_PEAK_GBS = 8000.0
 # workload: 2 x FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc, calibrated on the copy probe
 # (profiles/lj_c2_force_pmc.md)
 FORCE_TRAFFIC_BYTES = {"mixed": 2.882e9, "fp64": 3.215e9}
+# rocprofv3 VALUBusy (%) of the same kernel on this workload (profiles/lj_c2_force_valu_lds_pmc.md)
+FORCE_VALU_BUSY_PCT = {"mixed": 70.7, "fp64": 70.0}
 VALU_PEAK_TFLOPS = 78.6  # FP64 vector; scalar FP32 issues at the same rate (packed FP32 disabled)
 
 WORKLOADS = {
@@ -303,9 +305,11 @@ def main():
                     kernel=f"k_lj_brick<{'true' if head_prec == 'mixed' else 'false'}, false, 2, 8> (LJ force + push_f_v + "
                            "next step's push_f_v_r/push_f_v and displacement count, f and ep not stored)",
                     algorithmic_flops_per_atom=flops_per_atom, avg_launch_ms=fk["avg_ms"],
+                    valu_busy_pct=FORCE_VALU_BUSY_PCT[head_prec] if args.workload == "c2" else None,
                     note="VALU pair kernel (no MFMA): flops = 8/list entry + 24/in-cutoff pair; peak = FP64 vector "
                          "rate, which scalar FP32 shares with packed FP32 disabled; traffic = measured HBM bytes per "
-                         "launch, traffic_gbs = traffic / average launch time (HBM peak 8000 GB/s)")
+                         "launch, traffic_gbs = traffic / average launch time (HBM peak 8000 GB/s); valu_busy_pct = "
+                         "rocprofv3 VALUBusy of the kernel")
     b_alg = 144.0
     step_ceiling = min(HBM_PEAK_GBS * 1e9 / b_alg, VALU_PEAK_TFLOPS * 1e12 * (2.0 if head_prec == "mixed" else 1.0) /
                        flops_per_atom)
