@@ -101,6 +101,34 @@ def test_mixed_force_error():
     assert pe_rel <= 1e-5
 
 
+@pytest.mark.parametrize("precision", [PRECISION_FP64, PRECISION_MIXED])
+def test_lj_force_on_ghost_cells(precision):
+    """lennard_jones_force with its ghost slot set (lennard_jones.cu:174-181) over lists of every
+    cell (chunk_neighbors over all cells; ghost cells see the neighbours inside the grid):
+    forces and energies of inner and ghost particles equal the oracle's, bitwise for FP64 and
+    within the mixed tolerance otherwise."""
+    o, b = _init_both(6, temperature=0.9, seed=23)
+    o.chunk_neighbors(RCUT + SKIN)
+    b.chunk_neighbors(RCUT + SKIN, all_cells=True)
+    o.zero_force(ghost=True)
+    b.zero_particle_force(ghost=True)
+    o.lj_force(1.0, 1.0, RCUT, ghost=True)
+    b.lennard_jones_force(1.0, 1.0, RCUT, precision=precision, ghost=True)
+    oe, ge = o.export(ghosts=True), b.export(ghosts=True)
+    assert_same_layout(oe, ge, "inner+ghost")
+    dims, gl, _ = b.grid_info()
+    c = ge["cell"].astype(np.int64)
+    ijk = np.stack([c % dims[0], (c // dims[0]) % dims[1], c // (dims[0] * dims[1])], 1)
+    ghost = np.any((ijk < gl) | (ijk >= np.asarray(dims) - gl), axis=1)
+    assert ghost.any() and np.abs(ge["f"][ghost]).max() > 0
+    if precision == PRECISION_FP64:
+        for k in ("f", "ep"):
+            np.testing.assert_array_equal(oe[k].view(np.uint64), ge[k].view(np.uint64), err_msg=k)
+    else:
+        mx, rms = rel_err(ge["f"], oe["f"])
+        assert rms <= 1e-5, (mx, rms)
+
+
 def _with_bricks(monkeypatch, value):
     if value is None:
         monkeypatch.delenv("XNB_BRICKS", raising=False)
