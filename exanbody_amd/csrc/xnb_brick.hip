@@ -540,8 +540,9 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) 
 // pair id, a group without accepted pairs not added.  For a fixed type a, pid increases with
 // type b, so group t is the group of neighbours of type t.  Every lane updates all NT groups
 // per pair, the non-matching ones by +0.0 (sums start at +0.0 and stay out of -0.0, so this
-// is bitwise neutral).  ACCUM adds to the stored f (after zero_particle_force); otherwise f
-// starts from 0.0 like a zeroed force.  EPI as in k_lj_brick.
+// is bitwise neutral).  The mixed form keeps one sum in stream order instead (it has a
+// tolerance, not the reference's rounding).  ACCUM adds to the stored f (after
+// zero_particle_force); otherwise f starts from 0.0 like a zeroed force.  EPI as in k_lj_brick.
 template <bool MIXED, int NT, bool ACCUM, int EPI, int U>
 __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams& bp, uint32_t b, const double4* __restrict__ r,
                                                 const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
@@ -667,21 +668,24 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
           coef = in[u] ? de : 0.0;
           ce = (ENERGY && in[u]) ? pe4 * (ratio12 - ratio6) : 0.0;
         }
+        if (MIXED) {
+          // one sum in stream order, f = fma(c, d, f) as in the singlemat mixed form (c and the
+          // energy are +-0 outside the cutoff): the tolerance form needs no per-type grouping
+          gx[0] = fma(coef, dx[u], gx[0]);
+          gy[0] = fma(coef, dy[u], gy[0]);
+          gz[0] = fma(coef, dz[u], gz[0]);
+          if (ENERGY) ge[0] += ce;
+        } else {
 #pragma unroll
-        for (int q = 0; q < NT; q++) {
-          const bool mine = in[u] && tb[u] == (unsigned)q;
-          const double cq = mine ? coef : 0.0;
-          if (MIXED) {  // f = fma(c, d, f), the singlemat mixed form
-            gx[q] = fma(cq, dx[u], gx[q]);
-            gy[q] = fma(cq, dy[u], gy[q]);
-            gz[q] = fma(cq, dz[u], gz[q]);
-          } else {  // f += (de / r) * d, the plain form
+          for (int q = 0; q < NT; q++) {  // f += (de / r) * d per neighbour-type group, the plain form
+            const bool mine = in[u] && tb[u] == (unsigned)q;
+            const double cq = mine ? coef : 0.0;
             gx[q] += cq * dx[u];
             gy[q] += cq * dy[u];
             gz[q] += cq * dz[u];
+            if (ENERGY) ge[q] += mine ? ce : 0.0;
+            any[q] = any[q] || mine;
           }
-          if (ENERGY) ge[q] += mine ? ce : 0.0;
-          any[q] = any[q] || mine;
         }
       }
     }
@@ -693,7 +697,7 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
     double fx = ACCUM ? A.fx[s] : 0.0, fy = ACCUM ? A.fy[s] : 0.0, fz = ACCUM ? A.fz[s] : 0.0, fe = ACCUM ? A.ep[s] : 0.0;
 #pragma unroll
     for (int q = 0; q < NT; q++)
-      if (any[q]) {
+      if (MIXED ? q == 0 : any[q]) {
         fx += gx[q];
         fy += gy[q];
         fz += gz[q];
