@@ -66,18 +66,35 @@ def test_fcc32_full_size_100_steps_bitwise():
     """The reference's CPU-runnable workload at full size: FCC 32^3 (131,072 atoms, 19^3 cells),
     rho 0.8442, T 0.72, jitter 0.01, 100 steps of dt 0.005 rebuilt every 10.  The FP64 run
     equals the oracle bit for bit (layout, velocities, per-atom forces and energies, rebuild and
-    guard counts; the box totals, summed in another order, within 1e-12); mixed-precision
-    forces at the final positions are within 1e-5 RMS of FP64."""
+    guard counts, the CS=1 streams of all 6,859 inner cells at init and after the last rebuild;
+    the box totals, summed in another order, within 1e-12); mixed-precision forces at the final
+    positions are within 1e-5 RMS of FP64."""
     r, v, ids, L = fcc_inputs(32, temperature=0.72, jitter=0.01, seed=1)
     o, b, dom = make_pair(r, v, ids, L)
     assert dom["grid_dims"] == (19, 19, 19)
     o.init(1.0, 1.0, RCUT, SKIN)
     b.init(1.0, 1.0, RCUT, SKIN, PRECISION_FP64)
+
+    def same_streams(what):
+        gs, gmax = b.streams()
+        os_ = o.streams()
+        dims, gl, nc = b.grid_info()
+        cells = 0
+        for c in range(nc):
+            k, rem = divmod(c, dims[0] * dims[1])
+            j, i = divmod(rem, dims[0])
+            if all(gl <= x < d - gl for x, d in zip((i, j, k), dims)):
+                np.testing.assert_array_equal(gs[c], os_[c], err_msg=f"{what}: stream of cell {c}")
+                cells += 1
+        assert cells == 19 ** 3 and gmax == o.max_neighbors()
+
+    same_streams("init")
     ro = o.run(100, 0.005, 1.0, 1.0, RCUT, SKIN, rebuild_every=10)
     rg = b.run(100, 0.005, 1.0, 1.0, RCUT, SKIN, precision=PRECISION_FP64, rebuild_every=10)
     assert ro == rg and rg["rebuilds"] == 10
     oe, ge = o.export(), b.export()
     assert_same_layout(oe, ge, "after 100 steps")
+    same_streams("after the run")
     for k in ("v", "f", "ep"):
         np.testing.assert_array_equal(oe[k].view(np.uint64), ge[k].view(np.uint64), err_msg=k)
     (ko, po), (kg, pg) = o.energies(), b.energies()
