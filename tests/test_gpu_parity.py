@@ -105,6 +105,37 @@ def test_fcc32_full_size_100_steps_bitwise():
     assert rms <= 1e-5, (mx, rms)
 
 
+def test_fcc128_full_size_properties():
+    """The 8,388,608-atom FCC 128^3 workload at full size, through properties that hold at any
+    size: with jitter within +-0.01 no FCC shell crosses 2.5 or 2.8, so every atom has exactly
+    78 list neighbours and 54 within the cutoff; the pair forces of full lists cancel, so the
+    total force is zero up to the rounding of the per-atom sums; ids are conserved; mixed
+    forces are within 1e-5 RMS of the FP64 ones."""
+    import ctypes as C
+
+    from exanbody_amd import Box, cubic_domain
+    r, v, ids, L = fcc_inputs(128, temperature=0.72, jitter=0.01, seed=2)
+    n = len(r)
+    dom = cubic_domain(L, RCUT + SKIN)
+    assert dom["grid_dims"] == (76, 76, 76) and n == 8388608
+    b = Box(dom["bounds_min"], dom["bounds_max"], dom["grid_dims"], dom["cell_size"])
+    b.nbh_dist(RCUT, SKIN)
+    b.set_particles(r, v, ids)
+    b.init(1.0, 1.0, RCUT, SKIN, PRECISION_FP64)
+    assert b.pair_counts() == (78 * n, 54 * n)
+    tot, mx = C.c_size_t(), C.c_uint32()
+    b._check(b.L.xnb_chunk_neighbors_stream(b._h, None, 0, None, C.byref(tot), C.byref(mx)), "stream size")
+    assert mx.value == 78
+    e = b.export()
+    np.testing.assert_array_equal(np.sort(e["id"]), np.arange(n, dtype=np.uint64))
+    f = e["f"]
+    assert np.all(np.abs(f.sum(axis=0)) <= 1e-9 * np.abs(f).sum(axis=0))
+    b.zero_particle_force()
+    b.lennard_jones_force(1.0, 1.0, RCUT, precision=PRECISION_MIXED)
+    mxe, rms = rel_err(b.export()["f"], f)
+    assert rms <= 1e-5, (mxe, rms)
+
+
 def test_mixed_force_error():
     o, b = _init_both(8, temperature=0.72, seed=3)
     b.run(50, 0.005, 1.0, 1.0, RCUT, SKIN, precision=PRECISION_FP64, rebuild_every=10)
