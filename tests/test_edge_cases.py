@@ -146,3 +146,17 @@ def test_atoms_on_cell_faces(drift):
     _assert_same_state(o, b, "init")
     _run_both(o, b, 20, 0.005, 1)
     _assert_same_state(o, b, "after run", ghost_forces=False)
+
+
+def test_particle_leaving_open_axis_is_an_error():
+    """A particle that crosses the open face of a non-periodic axis stops the run with
+    XNB_E_OUTSIDE ("left the domain") at the next re-binning; the reference's single-rank
+    migrate_cell_particles would drop it (migrate_cell_particles.cpp:138-141)."""
+    from exanbody_amd import XnbError
+    r, v, ids, L = fcc_inputs(6, temperature=0.5, seed=9)
+    k = int(np.argmax(r[:, 2]))
+    v[k] = (0.0, 0.0, 50.0)  # 0.25 sigma per step towards z = L
+    o, b, dom = make_pair(r, v, ids, L, periodic=(1, 1, 0))
+    b.init(1.0, 1.0, RCUT, SKIN, PRECISION_FP64)
+    with pytest.raises(XnbError, match="left the domain"):
+        b.run(10, 0.005, 1.0, 1.0, RCUT, SKIN, precision=PRECISION_FP64, rebuild_every=1)
