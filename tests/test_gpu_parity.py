@@ -319,12 +319,16 @@ def test_fp64_sqrt_recip_sequences_are_ieee():
     b.close()
 
 
-@pytest.mark.parametrize("precision", [PRECISION_FP64, PRECISION_MIXED])
-def test_snapshot_restart_is_bitwise(precision):
+@pytest.mark.parametrize("precision,rho", [(PRECISION_FP64, 0.8442), (PRECISION_MIXED, 0.8442), (PRECISION_FP64, 1.6)])
+def test_snapshot_restart_is_bitwise(precision, rho):
     """run(12); snapshot; restore into a fresh box; run(12) == run(24) without interruption
-    (rebuild every 6: the restart falls on a rebuild step boundary, so the schedule is kept)."""
+    (rebuild every 6: the restart falls on a rebuild step boundary, so the schedule is kept).
+    rho 1.6: cells of about 62 atoms, restored in their sub-cell order (rebuild_amr)."""
     from exanbody_amd import Box
-    r, v, ids, L = fcc_inputs(7, temperature=1.0, jitter=0.05, seed=41)
+    if rho > 1.0:
+        r, v, ids, L = fcc_inputs(10, rho=rho, temperature=0.72, jitter=0.02, seed=41)
+    else:
+        r, v, ids, L = fcc_inputs(7, temperature=1.0, jitter=0.05, seed=41)
     _, a, dom = make_pair(r, v, ids, L)
     _, b, _ = make_pair(r, v, ids, L)
     for x in (a, b):
