@@ -1,4 +1,4 @@
-"""Shared test helpers: paired oracle / GPU boxes built from the same seeded inputs."""
+"""Shared test helpers: an oracle and a device Box built from the same seeded inputs."""
 import numpy as np
 
 from exanbody_amd import Box, cubic_domain, generate_fcc
@@ -32,7 +32,7 @@ def fcc_lattice(cells, a, jitter=0.0, temperature=0.0, seed=1):
 
 
 def make_pair_domain(r, v, ids, dom, types=None, rcut=RCUT, skin=SKIN, gpu=True):
-    """Oracle and (optionally) GPU box of domain dom over the same particles; nbh_dist
+    """Oracle and (optionally) device Box of domain dom over the same particles; nbh_dist
     applied to both."""
     periodic = dom["periodic"]
     o = Oracle(dom["bounds_min"], dom["bounds_max"], dom["grid_dims"], dom["cell_size"], periodic=periodic,
@@ -47,7 +47,7 @@ def make_pair_domain(r, v, ids, dom, types=None, rcut=RCUT, skin=SKIN, gpu=True)
 
 
 def make_pair(r, v, ids, L, types=None, min_cell=RCUT + SKIN, rcut=RCUT, skin=SKIN, gpu=True, periodic=(1, 1, 1)):
-    """Oracle and (optionally) GPU box over the same particles in a cubic box of edge L."""
+    """Oracle and (optionally) device Box over the same particles in a cubic box of edge L."""
     dom = cubic_domain(L, min_cell, periodic=periodic)
     o, b = make_pair_domain(r, v, ids, dom, types, rcut, skin, gpu)
     return o, b, dom
