@@ -12,7 +12,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libxnb_oracle.so")
-_lib = None
+_libs = {}
 
 
 class OracleError(RuntimeError):
@@ -34,12 +34,13 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def lib(path=None):
+    """The oracle library (path: another build of the same source, e.g. a planted-mistake one)."""
+    path = path or _LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(_LIB_PATH)
+        L = C.CDLL(path)
         P, D, I, I64, SZ = C.c_void_p, C.c_double, C.c_int, C.c_int64, C.c_size_t
         L.op_create.argtypes = [C.POINTER(P), P, P, P, D, P, D]
         L.op_destroy.argtypes = [P]
@@ -71,8 +72,8 @@ def lib():
         L.op_encode_double_u32.argtypes = [D, D, D]; L.op_encode_double_u32.restype = C.c_uint32
         L.op_restore_u32_double.argtypes = [C.c_uint32, D, D]; L.op_restore_u32_double.restype = D
         L.op_num_threads.restype = I
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _ptr(a):
@@ -88,8 +89,8 @@ def _check(rc, what, allow=()):
 class Oracle:
     """One periodic box on the CPU oracle (per-cell SoA grid with ghost layers)."""
 
-    def __init__(self, origin, extent, grid_dims, cell_size, periodic=(1, 1, 1), ghost_dist=2.8):
-        L = lib()
+    def __init__(self, origin, extent, grid_dims, cell_size, periodic=(1, 1, 1), ghost_dist=2.8, libpath=None):
+        L = lib(libpath)
         self._h = C.c_void_p()
         o = np.ascontiguousarray(origin, dtype=np.float64)
         e = np.ascontiguousarray(extent, dtype=np.float64)

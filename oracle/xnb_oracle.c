@@ -389,12 +389,22 @@ int op_move_particles(op_sim* s) {
       int32_t* pack = (int32_t*)malloc((size_t)n * sizeof(int32_t));
       for (int32_t j = 0; j < n; j++) pack[j] = j;
       for (int32_t j = 0; j < n_out; j++) pack[removed[ci][j]] = -1;
+#ifdef OP_MUTANT_STABLE_PACK
+      /* planted mistake for the order-fixture tests: stable compaction of the stayers */
+      {
+        int32_t w = 0;
+        for (int32_t j = 0; j < n; j++)
+          if (pack[j] != -1) pack[w++] = pack[j];
+        n = w;
+      }
+#else
       for (int32_t j = n - 1; j >= 0; j--) {
         if (pack[j] == -1) {
           if (j < n - 1) pack[j] = pack[n - 1];
           --n;
         }
       }
+#endif
       for (int32_t j = 0; j < n; j++) {
         if (j != pack[j]) {
           op_tuple t;
@@ -465,7 +475,13 @@ int op_ghost_update_all(op_sim* s) {
     free(G->pidx);
     G->pidx = (uint32_t*)malloc((S->n ? S->n : 1) * sizeof(uint32_t));
     if (cell_reserve(C, S->n)) { err = OP_E_ARG; continue; }
+#ifdef OP_MUTANT_REVERSE_GHOSTS
+    /* planted mistake for the order-fixture tests: images in reversed source order */
+    for (uint32_t pr = S->n; pr-- > 0;) {
+      const uint32_t p = pr;
+#else
     for (uint32_t p = 0; p < S->n; p++) {
+#endif
       double gr[3] = {shift_coord(S->rx[p], G->shift[0], s->origin[0], s->extent[0]),
                       shift_coord(S->ry[p], G->shift[1], s->origin[1], s->extent[1]),
                       shift_coord(S->rz[p], G->shift[2], s->origin[2], s->extent[2])};
