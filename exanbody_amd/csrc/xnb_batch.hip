@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -76,6 +77,9 @@ struct xnb_batch {
   // HIP graph of a quiet step (no box fresh, rebuilt on displacement or rebuilding): ghost
   // refresh, force, clock advance, replayed while the step's work table stays the same
   bool use_graph = true;
+  // head room of attached boxes (XNB_BATCH_MARGINS="brick,slot" at create; 1,1 leaves none, so
+  // most rebuilds that grow a brick or the ghosts take the one-box recovery)
+  double brick_margin = kBrickMargin, slot_margin = kSlotMargin;
   hipGraphExec_t gexec = nullptr;
   std::vector<uint32_t> gtab, last_quiet;  // table of the captured graph / of the last quiet step
   size_t g_og = 0, g_ng = 0, g_of = 0, g_nf = 0;
@@ -201,7 +205,7 @@ std::vector<uint32_t> blocks_table(const std::vector<int>& boxes, const std::vec
 std::vector<uint32_t> bricks_table(const std::vector<int>& boxes, const std::vector<uint32_t>& nbricks, const std::vector<uint32_t>& tag) {
   std::vector<uint32_t> all;
   for (size_t q = 0; q < boxes.size(); q++)
-    for (uint32_t b = 0; b < nbricks[q]; b++) all.push_back(((uint32_t)boxes[q] << kWorkBoxShift) | (tag[q] << kWorkEpiShift) | b);
+    for (uint32_t b = 0; b < nbricks[q]; b++) all.push_back(work_brick_entry((uint32_t)boxes[q], tag[q], b));
   const size_t n = all.size(), per = (n + kXcds - 1) / kXcds;
   std::vector<uint32_t> t(per * kXcds, NONE);
   for (size_t j = 0; j < t.size(); j++) {
@@ -314,12 +318,13 @@ int recover_box(xnb_batch* B, int i, const xnb_run_params& p) {
   xnb_ctx* c = bb.c;
   ctx_from_frame(B, i, B->par);
   int e;
-  c->brick_margin = kBrickMargin;
-  if ((e = ensure_capacity(c, (size_t)std::ceil((c->N + c->G) * kSlotMargin) + 256, c->N))) return e;
+  c->brick_margin = B->brick_margin;
+  if ((e = ensure_capacity(c, (size_t)std::ceil((c->N + c->G) * B->slot_margin) + 256, c->N))) return e;
   if ((e = op_ghost_all(c, false, false))) return e;  // f and ep of the run's boxes are not carried
   xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
   if ((e = op_nbh(c, c->nbh_dist_lab, cfg, 0))) return e;
   if (c->list_kind != 1) return fail(c, XNB_E_STATE, "box no longer fits the brick path");
+  if (c->bp.nbricks > kWorkBrickMask + 1) return fail(c, XNB_E_CAPACITY, "box has too many bricks for a batch");
   BHIP(B, hipStreamSynchronize(c->st));
   frame_from_ctx(B, i, p);
   return upload_frame(B, i);
@@ -357,6 +362,15 @@ int xnb_batch_create(xnb_batch** out, int device) {
     ok = hipMemcpy(B->dclock.p, clock0, sizeof(clock0), hipMemcpyHostToDevice) == hipSuccess;
   }
   if (const char* s = std::getenv("XNB_BATCH_GRAPH")) B->use_graph = std::atoi(s) != 0;
+  if (const char* s = std::getenv("XNB_BATCH_MARGINS")) {
+    double bm = 0, sm = 0;
+    if (std::sscanf(s, "%lf,%lf", &bm, &sm) != 2 || !(bm >= 1.0 && bm <= 4.0) || !(sm >= 1.0 && sm <= 4.0)) {
+      xnb_batch_destroy(B);
+      return XNB_E_ARG;
+    }
+    B->brick_margin = bm;
+    B->slot_margin = sm;
+  }
   if (ok) ok = hipMemset(B->dwords.p, 0, 4 * kMaxBatchBoxes * sizeof(uint32_t)) == hipSuccess;
   if (ok) ok = hipMemset(B->dtrig.p, 0, kMaxBatchBoxes * sizeof(uint32_t)) == hipSuccess;
   // both staging events start recorded, so the first waits on them return at once
@@ -416,12 +430,14 @@ int xnb_batch_attach(xnb_batch* B, xnb_ctx* c, const xnb_run_params* p, int* slo
     }
   if (i < 0) return bfail(B, XNB_E_CAPACITY, "batch is full");
   // head room: ghost slots, and bricks / lists rebuilt with margins
-  c->brick_margin = kBrickMargin;
-  if ((e = ensure_capacity(c, (size_t)std::ceil((c->N + c->G) * kSlotMargin) + 256, (size_t)c->N + c->G))) return e;
+  c->brick_margin = B->brick_margin;
+  if ((e = ensure_capacity(c, (size_t)std::ceil((c->N + c->G) * B->slot_margin) + 256, (size_t)c->N + c->G))) return e;
   xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
   if ((e = op_nbh(c, c->nbh_dist_lab, cfg, 0))) return e;
   if (c->list_kind != 1) return bfail(B, XNB_E_STATE, "box geometry is not on the brick path (cell edge < nbh_dist)");
   if (brick_lds_bytes(c->bp) > B->lds_limit) return bfail(B, XNB_E_CAPACITY, "box bricks exceed LDS");
+  // force-table entries hold the brick index in kWorkEpiShift bits
+  if (c->bp.nbricks > kWorkBrickMask + 1) return bfail(B, XNB_E_CAPACITY, "box has too many bricks for a batch");
   if (hipStreamSynchronize(c->st) != hipSuccess) return bfail(B, XNB_E_HIP, "box stream");
   BatchBox& bb = B->box[i];
   bb = BatchBox{};
@@ -702,6 +718,13 @@ int xnb_batch_detach(xnb_batch* B, int slot, xnb_run_params* p) {
   if (p) {
     p->n_rebuilds = bb.n_rebuilds;
     p->n_violations = viol;
+  }
+  if (!bb.finished) {
+    // positions and velocities already hold the first half of the next step: the context is
+    // not a completed step, so xnb_run and the force operators refuse it (XNB_E_STATE) until
+    // it is initialised again
+    bb.c->lists_valid = false;
+    bb.c->backup_valid = false;
   }
   const int rc = bb.failed ? XNB_E_STATE : XNB_OK;
   if (bb.failed) B->err = bb.err;

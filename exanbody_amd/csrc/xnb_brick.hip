@@ -962,8 +962,8 @@ __global__ void __launch_bounds__(1024) kb_lj_brick(const BoxFrame* __restrict__
   uint32_t box, b;
   if (!batch_brick(work, box, b)) return;  // whole workgroup
   const uint32_t par = clock[0], gstep = clock[1];
-  const uint32_t epi = b >> kWorkEpiShift;
-  b &= kWorkBrickMask;
+  const uint32_t epi = work_entry_epi(b);
+  b = work_entry_brick(b);
   const BoxFrame& f = F[box];
   const uint32_t key = (f.mixed ? 4u : 0u) | epi;
   switch (key) {

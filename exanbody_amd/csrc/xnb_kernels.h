@@ -84,6 +84,20 @@ constexpr uint32_t kWorkBlockMask = (1u << kWorkBoxShift) - 1;
 constexpr uint32_t kWorkEpiShift = 18;
 constexpr uint32_t kWorkBrickMask = (1u << kWorkEpiShift) - 1;
 constexpr uint32_t kMaxBatchBoxes = 4096;  // box index < 2^12
+// force-table entry of brick `brick` of box `box` with epilogue `epi`; a batched box may have
+// at most kWorkBrickMask + 1 bricks (attach and recovery reject more)
+__host__ __device__ constexpr uint32_t work_brick_entry(uint32_t box, uint32_t epi, uint32_t brick) {
+  return (box << kWorkBoxShift) | (epi << kWorkEpiShift) | brick;
+}
+__host__ __device__ constexpr uint32_t work_entry_box(uint32_t w) { return w >> kWorkBoxShift; }
+__host__ __device__ constexpr uint32_t work_entry_epi(uint32_t w) { return (w & kWorkBlockMask) >> kWorkEpiShift; }
+__host__ __device__ constexpr uint32_t work_entry_brick(uint32_t w) { return w & kWorkBrickMask; }
+// round trips at the field limits (checked by the compiler)
+static_assert(work_entry_box(work_brick_entry(kMaxBatchBoxes - 1, 3, kWorkBrickMask)) == kMaxBatchBoxes - 1, "box field");
+static_assert(work_entry_epi(work_brick_entry(kMaxBatchBoxes - 1, 3, kWorkBrickMask)) == 3, "epilogue field");
+static_assert(work_entry_brick(work_brick_entry(kMaxBatchBoxes - 1, 3, kWorkBrickMask)) == kWorkBrickMask, "brick field");
+static_assert(work_entry_epi(work_brick_entry(0, 2, 0)) == 2 && work_entry_brick(work_brick_entry(1, 0, 1)) == 1, "fields");
+static_assert((kMaxBatchBoxes - 1) <= (0xffffffffu >> kWorkBoxShift), "box index fits above the block field");
 
 void launch_batch_zero_cells(const BoxFrame* F, const uint32_t* work, uint32_t nwork, hipStream_t st);
 void launch_batch_zero_words(uint32_t* words, const uint32_t* boxes, uint32_t n, hipStream_t st);

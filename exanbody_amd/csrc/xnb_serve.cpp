@@ -49,6 +49,12 @@ int xnb_serve_run(const xnb_job* jobs, int njobs, const xnb_serve_config* cfg, x
   if (njobs < 0 || (njobs > 0 && (!jobs || !results)) || !cfg || cfg->steps_per_iter < 1) return XNB_E_ARG;
   for (int q = 1; q < njobs; q++)
     if (jobs[q].arrival_s < jobs[q - 1].arrival_s) return XNB_E_ARG;  // arrival order
+  // every result is written, also when the loop stops on an error: a job that never
+  // completed keeps status XNB_E_STATE and is counted as failed
+  for (int q = 0; q < njobs; q++) {
+    results[q] = xnb_job_result{};
+    results[q].status = XNB_E_STATE;
+  }
   xnb_batch* B = nullptr;
   int e = xnb_batch_create(&B, cfg->device);
   if (e) return e;
@@ -144,7 +150,9 @@ int xnb_serve_run(const xnb_job* jobs, int njobs, const xnb_serve_config* cfg, x
   for (const Active& a : act) {  // left over after an error
     xnb_batch_detach(B, a.slot, nullptr);
     xnb_destroy(a.ctx);
+    finish(a.job, e ? e : XNB_E_STATE, wall);
   }
+  for (; next < njobs; next++) finish(next, e ? e : XNB_E_STATE, wall);  // never admitted
   xnb_batch_destroy(B);
   if (stats) {
     std::vector<double> lat;

@@ -242,7 +242,9 @@ int xnb_batch_box_recoveries(xnb_batch* batch, int slot, int64_t* n);
  * at xnb_batch_create launches every step eagerly. */
 int xnb_batch_graph_stats(xnb_batch* batch, int64_t* graph_steps, int64_t* captures);
 /* Detach the box of a slot; its xnb_ctx then holds the state xnb_run would have produced
- * (for a finished box) and p->n_rebuilds / n_violations are filled as by xnb_run. */
+ * (for a finished box) and p->n_rebuilds / n_violations are filled as by xnb_run.  A box
+ * detached before its last step holds part of its next step: its lists and backup are marked
+ * invalid, so xnb_run and the force operators return XNB_E_STATE until xnb_init runs again. */
 int xnb_batch_detach(xnb_batch* batch, int slot, xnb_run_params* p);
 
 /* ---- serving loop ---------------------------------------------------------------- */

@@ -140,3 +140,56 @@ def test_batch_quiet_steps_from_graph(monkeypatch, graph):
         _same(b.export(), ref[q][0], f"job {q}")
         b.close()
     batch.close()
+
+
+def test_batch_recovery_path(monkeypatch):
+    """With no head room (XNB_BATCH_MARGINS=1,1) rebuilds that grow a brick, a region or the
+    ghosts overflow the sizes a box was attached with, and the batch redoes them with the
+    one-box operators (recover_box, the ghost-overflow skip of the batched build): recoveries
+    happen, and every box still ends bitwise equal to the same box run alone."""
+    monkeypatch.setenv("XNB_BATCH_MARGINS", "1.0,1.0")
+    jobs = [
+        (5, 0.80, 2.50, 21, 30, PRECISION_FP64, 0),
+        (6, 0.75, 3.00, 22, 30, PRECISION_MIXED, 5),
+        (4, 0.70, 2.00, 23, 25, PRECISION_FP64, 4),
+    ]
+    ref = [_alone(j) for j in jobs]
+    batch = Batch()
+    boxes, slots = [], []
+    for j in jobs:
+        b = _box(j)
+        boxes.append(b)
+        slots.append(batch.attach(b, j[4], DT, 1.0, 1.0, RCUT, SKIN, precision=j[5], rebuild_every=j[6]))
+    while batch.step(4):
+        pass
+    rec = [batch.recoveries(s) for s in slots]
+    assert sum(rec) > 0, f"no recovery happened: {rec}"
+    for q, (j, b, s) in enumerate(zip(jobs, boxes, slots)):
+        assert batch.status(s) == (j[4], 1)
+        assert batch.detach(s) == ref[q][1], f"job {q}: rebuild / violation counts"
+        _same(b.export(), ref[q][0], f"job {q} ({rec[q]} recoveries)")
+        b.close()
+    batch.close()
+
+
+def test_detach_unfinished_box_refuses_run():
+    """A box detached before its last step holds part of its next step; xnb_run refuses it."""
+    from exanbody_amd import XnbError
+    j = JOBS[0]
+    batch = Batch()
+    b = _box(j)
+    s = batch.attach(b, j[4], DT, 1.0, 1.0, RCUT, SKIN, precision=j[5], rebuild_every=j[6])
+    batch.step(3)
+    assert batch.status(s) == (3, 0)
+    batch.detach(s)
+    with pytest.raises(XnbError):
+        b.run(5, DT, 1.0, 1.0, RCUT, SKIN, precision=j[5], rebuild_every=j[6])
+    b.close()
+    batch.close()
+
+
+def test_margins_knob_is_checked(monkeypatch):
+    monkeypatch.setenv("XNB_BATCH_MARGINS", "0.5,1.0")
+    from exanbody_amd import XnbError
+    with pytest.raises(XnbError):
+        Batch()

@@ -89,3 +89,50 @@ def test_generator_partial_fill_density():
     r, v, ids, L = generate_fcc(5, 0.8, n_atoms=300, jitter=0.0, temperature=1.0, seed=1)
     assert len(r) == 300 and L ** 3 * 0.8 == pytest.approx(300, rel=1e-12)
     assert len({tuple(x) for x in np.round(r, 9)}) == 300
+
+
+# C struct of include/xnb_amd.h -> its ctypes mirror in the package
+def _ctypes_mirrors():
+    from exanbody_amd import serve
+    return {"xnb_domain": _lib.Domain, "xnb_lj_params": _lib.LJParams,
+            "xnb_chunk_neighbors_config": _lib.ChunkNeighborsConfig, "xnb_run_params": _lib.RunParams,
+            "xnb_job": serve.Job, "xnb_job_result": serve.JobResult, "xnb_serve_config": serve.ServeConfig,
+            "xnb_serve_stats": serve.ServeStats}
+
+
+def test_header_compiles_as_c_and_ctypes_layouts_match(tmp_path):
+    """include/xnb_amd.h is a C header (gcc -std=c99 -pedantic -Werror); every struct the
+    bindings mirror has the C compiler's size and field offsets in its ctypes mirror."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("no C compiler")
+    mirrors = _ctypes_mirrors()
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "xnb_amd.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in py._fields_:
+            lines.append(f'  printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-pedantic", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    "-o", str(exe), str(src)], check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == C.sizeof(py), f"{cname}: C size {got[(cname, 'size')]} != ctypes {C.sizeof(py)}"
+        for f in py._fields_:
+            assert got[(cname, f[0])] == getattr(py, f[0]).offset, f"{cname}.{f[0]}: offset differs"
+    # every field of the C struct is mirrored (no C field left out of the ctypes table)
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "xnb_amd.h")).read(), flags=re.S)
+    for cname, py in mirrors.items():
+        body = re.search(r"typedef struct \{([^{}]*)\}\s*" + cname + ";", text).group(1)
+        names = []
+        for decl in body.split(";"):
+            for part in decl.split(","):
+                m = re.search(r"([A-Za-z_]\w*)\s*(?:\[[^\]]*\])?\s*$", part.strip())
+                if m:
+                    names.append(m.group(1))
+        assert names == [f[0] for f in py._fields_], f"{cname}: {names}"
