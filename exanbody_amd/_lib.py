@@ -72,12 +72,14 @@ _SIGS = {
     "xnb_zero_particle_force": (C.c_int, [P, C.c_int]),
     "xnb_lennard_jones_force": (C.c_int, [P, C.POINTER(LJParams), C.c_double, C.c_int, C.c_int]),
     "xnb_lennard_jones_force_multimat": (C.c_int, [P, C.c_int, P, P, P, C.c_int]),
+    "xnb_gravitational_force": (C.c_int, [P, C.c_double, C.c_int, P, C.c_double, C.c_int]),
     "xnb_push_f_v_r": (C.c_int, [P, C.c_double, C.c_double]),
     "xnb_push_f_v": (C.c_int, [P, C.c_double, C.c_double]),
     "xnb_simulation_energies": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "xnb_init": (C.c_int, [P, C.POINTER(LJParams), C.c_double, C.c_double, C.c_int]),
     "xnb_run": (C.c_int, [P, C.POINTER(RunParams)]),
     "xnb_run_multimat": (C.c_int, [P, C.POINTER(RunParams), C.c_int, P, P, P]),
+    "xnb_run_step_counts": (C.c_int, [P, P, C.c_size_t]),
     "xnb_timer_enable": (C.c_int, [P, C.c_int]),
     "xnb_timer_reset": (C.c_int, [P]),
     "xnb_timer_read": (C.c_int, [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),

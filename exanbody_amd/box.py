@@ -58,6 +58,7 @@ class Box:
             d.periodic[a] = int(periodic[a])
         d.cell_size = cell_size
         self._h = C.c_void_p()
+        self._last_steps = 0
         self._check(self.L.xnb_create(C.byref(self._h), C.byref(d), int(device)), "xnb_create", ctx=False)
 
     def _check(self, rc, what, ctx=True):
@@ -145,8 +146,9 @@ class Box:
                                         skip_ghosts)
         self._check(self.L.xnb_chunk_neighbors(self._h, nbh_dist, C.byref(cfg), int(all_cells)), "chunk_neighbors")
 
-    def streams(self):
-        """Per-cell CS=1 uint16 streams (list of arrays, one per cell) and the max neighbour count."""
+    def streams(self, cells=None):
+        """Per-cell CS=1 uint16 streams and the max neighbour count: a list of arrays, one per
+        cell, or a dict {cell: array} for the given cells."""
         tot, mx = C.c_size_t(), C.c_uint32()
         self._check(self.L.xnb_chunk_neighbors_stream(self._h, None, 0, None, C.byref(tot), C.byref(mx)), "stream size")
         _, _, nc = self.grid_info()
@@ -154,6 +156,8 @@ class Box:
         offs = np.zeros(nc + 1, np.uint64)
         self._check(self.L.xnb_chunk_neighbors_stream(self._h, _ptr(buf), buf.size, _ptr(offs), C.byref(tot),
                                                       C.byref(mx)), "stream export")
+        if cells is not None:
+            return {int(c): buf[offs[c]:offs[c + 1]].copy() for c in cells}, mx.value
         return [buf[offs[c]:offs[c + 1]].copy() for c in range(nc)], mx.value
 
     def zero_particle_force(self, ghost=False):
@@ -170,6 +174,12 @@ class Box:
         rc = np.ascontiguousarray(rcut_pair, np.float64)
         self._check(self.L.xnb_lennard_jones_force_multimat(self._h, int(ntypes), _ptr(e), _ptr(s), _ptr(rc),
                                                             int(precision)), "lennard_jones_force_multimat")
+
+    def gravitational_force(self, G, type_mass, rcut, ghost=False):
+        """f += G m_a m_b dr / r^3 over pairs within rcut, masses per particle type."""
+        m = np.ascontiguousarray(type_mass, np.float64)
+        self._check(self.L.xnb_gravitational_force(self._h, float(G), int(m.size), _ptr(m), float(rcut), int(bool(ghost))),
+                    "gravitational_force")
 
     def push_f_v_r(self, dt, dt_scale=1.0):
         self._check(self.L.xnb_push_f_v_r(self._h, dt, dt_scale), "push_f_v_r")
@@ -191,7 +201,15 @@ class Box:
         p = _lib.RunParams(dt, _lib.LJParams(epsilon, sigma), rcut, rcut_inc, int(precision), int(rebuild_every),
                            int(steps), 0, 0)
         self._check(self.L.xnb_run(self._h, C.byref(p)), "run")
+        self._last_steps = int(steps)
         return dict(rebuilds=p.n_rebuilds, violations=p.n_violations)
+
+    def run_step_counts(self, n=None):
+        """particle_displ_over count of every step of the last run (index k - 1 for step k)."""
+        n = self._last_steps if n is None else n
+        out = np.zeros(n, np.uint64)
+        self._check(self.L.xnb_run_step_counts(self._h, _ptr(out), n), "run_step_counts")
+        return out
 
     def run_multimat(self, steps, dt, ntypes, eps_pair, sigma_pair, rcut_pair, rcut_inc, precision=_lib.PRECISION_FP64,
                      rebuild_every=0):
@@ -201,6 +219,7 @@ class Box:
         p = _lib.RunParams(dt, _lib.LJParams(1.0, 1.0), float(rc.max()), rcut_inc, int(precision), int(rebuild_every), int(steps),
                            0, 0)
         self._check(self.L.xnb_run_multimat(self._h, C.byref(p), int(ntypes), _ptr(e), _ptr(s), _ptr(rc)), "run_multimat")
+        self._last_steps = int(steps)
         return dict(rebuilds=p.n_rebuilds, violations=p.n_violations)
 
     def autotune(self, epsilon, sigma, rcut, precision=_lib.PRECISION_FP64):

@@ -1079,6 +1079,39 @@ int xnb_lennard_jones_force_multimat(xnb_ctx* c, int ntypes, const double* eps, 
   return XNB_OK;
 }
 
+int xnb_gravitational_force(xnb_ctx* c, double G, int ntypes, const double* type_mass, double rcut, int ghost) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (ntypes < 1 || ntypes > kGravityTypes || !type_mass || !(rcut > 0.0))
+    return fail(c, XNB_E_ARG, "bad gravitational_force arguments (1..16 types, rcut > 0)");
+  if (!c->lists_valid) return fail(c, XNB_E_STATE, "no neighbour lists (chunk_neighbors)");
+  const uint32_t n_atoms = ghost ? c->N + c->G : c->N;
+  // every type indexes the mass table (the reference reads m_type_mass[type] unchecked)
+  HIPCHK(c, hipMemsetAsync(c->u32s.p + 9, 0, sizeof(uint32_t), c->st));
+  launch_max_type(n_atoms, c->A.r, c->u32s.p + 9, c->st);
+  HIPCHK(c, hipGetLastError());
+  uint32_t tmax = 0;
+  if ((e = read_u32(c, c->u32s.p + 9, &tmax))) return e;
+  if ((int)tmax >= ntypes) return fail(c, XNB_E_ARG, "a particle type has no mass");
+  GravitationalForceFunctor gf{};
+  gf.G = G;
+  for (int q = 0; q < ntypes; q++) gf.mass[q] = type_mass[q];
+  const double rc2 = rcut * rcut;
+  tstart(c, XNB_TIMER_FORCE);
+  if (c->list_kind == 1 && !ghost) {
+    hipError_t he = launch_gravity_brick(c->g, c->bp, c->A, c->cell_off.p, c->cell_n.p, c->blist.p, c->bcnt.p, gf, rc2, c->st);
+    if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick gravitational kernel: ") + hipGetErrorString(he));
+  } else {
+    if ((e = ensure_slot_lists(c))) return e;
+    if (c->list_atoms < n_atoms) return fail(c, XNB_E_STATE, "neighbour lists do not cover the requested cells");
+    launch_gravity_slots(n_atoms, c->A, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, gf, rc2, c->st);
+  }
+  tstop(c);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
+
 int xnb_push_f_v_r(xnb_ctx* c, double dt, double dt_scale) {
   int e;
   if ((e = check_ctx(c))) return e;
@@ -1155,6 +1188,7 @@ int run_loop(xnb_ctx* c, xnb_run_params* p, const LJMulti* mm, int ntypes) {
   c->last_rcut = p->rcut;
   p->n_rebuilds = 0;
   p->n_violations = 0;
+  c->last_run_steps = 0;
   if (p->steps > 0) {
     if ((e = dalloc(c, c->step_counts, (size_t)p->steps))) return e;
     HIPCHK(c, hipMemsetAsync(c->step_counts.p, 0, (size_t)p->steps * sizeof(unsigned long long), c->st));
@@ -1211,6 +1245,7 @@ int run_loop(xnb_ctx* c, xnb_run_params* p, const LJMulti* mm, int ntypes) {
       if (h[q] > 0 && !rebuilt[q]) p->n_violations++;
   }
   HIPCHK(c, hipStreamSynchronize(c->st));
+  c->last_run_steps = p->steps;
   return XNB_OK;
 }
 
@@ -1219,6 +1254,16 @@ int run_loop(xnb_ctx* c, xnb_run_params* p, const LJMulti* mm, int ntypes) {
 extern "C" {
 
 int xnb_run(xnb_ctx* c, xnb_run_params* p) { return run_loop(c, p, nullptr, 0); }
+
+int xnb_run_step_counts(xnb_ctx* c, uint64_t* counts, size_t n) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (n > (size_t)c->last_run_steps || (n && !counts)) return fail(c, XNB_E_ARG, "more counts than steps of the last run");
+  if (!n) return XNB_OK;
+  HIPCHK(c, hipMemcpyAsync(counts, c->step_counts.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
 
 int xnb_run_multimat(xnb_ctx* c, xnb_run_params* p, int ntypes, const double* eps, const double* sigma, const double* rcut) {
   int e;

@@ -175,13 +175,40 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
   }
   if (load_pos) {
     const uint32_t R = L.rpref[bp.nrc];
-    for (uint32_t idx = t; idx < R; idx += blockDim.x) {
-      const uint32_t rc = upper_index(L.rpref, (uint32_t)bp.nrc, idx);
-      const double4 p = r[L.roff[rc] + (idx - L.rpref[rc])];
-      L.px[3 * idx] = p.x;
-      L.py[3 * idx] = p.y;
-      L.pz[3 * idx] = p.z;
-      L.pt[idx] = (uint8_t)p.w;
+    // Region rows (RX cells along x) wave by wave.  In a row of inner cells the cells' slots
+    // follow each other, so region index idx maps to slot idx + delta for the whole row and
+    // the lanes copy it 64 atoms at a time; a row holding ghost cells goes cell by cell.
+    // (A search for each atom's cell cost ~70 VALU instructions per staged atom.)
+    const uint32_t lane = t & 63u, nw = blockDim.x >> 6, nrow = (uint32_t)(bp.nrc / bp.RX);
+    for (uint32_t q = t >> 6; q < nrow; q += nw) {
+      const uint32_t c0 = q * (uint32_t)bp.RX;
+      const bool mine = lane < (uint32_t)bp.RX;
+      const uint32_t cnt = mine ? L.rcnt[c0 + lane] : 0u;
+      const int32_t del = mine ? (int32_t)(L.roff[c0 + lane] - L.rpref[c0 + lane]) : 0;
+      const unsigned long long nz = __ballot(cnt != 0u);
+      if (nz == 0ull) continue;  // wave-uniform: an empty row
+      const int32_t d0 = __shfl(del, __ffsll((long long)nz) - 1, 64);
+      const uint32_t start = L.rpref[c0], end = L.rpref[c0 + bp.RX];
+      if (__all(cnt == 0u || del == d0)) {
+        for (uint32_t idx = start + lane; idx < end; idx += 64u) {
+          const double4 p = r[(uint32_t)((int32_t)idx + d0)];
+          L.px[3 * idx] = p.x;
+          L.py[3 * idx] = p.y;
+          L.pz[3 * idx] = p.z;
+          L.pt[idx] = (uint8_t)p.w;
+        }
+      } else {
+        for (uint32_t c = c0; c < c0 + (uint32_t)bp.RX; c++) {
+          const uint32_t b0 = L.rpref[c], b1 = L.rpref[c + 1], o = L.roff[c];
+          for (uint32_t idx = b0 + lane; idx < b1; idx += 64u) {
+            const double4 p = r[o + (idx - b0)];
+            L.px[3 * idx] = p.x;
+            L.py[3 * idx] = p.y;
+            L.pz[3 * idx] = p.z;
+            L.pt[idx] = (uint8_t)p.w;
+          }
+        }
+      }
     }
     if (t < kSentinels) {  // sentinel atoms R .. R + kSentinels - 1: every distance to them exceeds any cutoff
       L.px[3 * (R + t)] = 1e30;
@@ -396,8 +423,18 @@ __global__ void __launch_bounds__(1024) k_nbh_brick_mask(Geom g, BrickParams bp,
   nbh_brick_body<FILTER>(g, bp, b, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist, bcnt, max_cnt);
 }
 
-// ------------------------------------------------------------------ force
-// EPI selects the epilogue after the force store (f = 0 + sum, or f += sum with ACCUM):
+// ------------------------------------------------------------------ pair traversal
+// compute_pair_singlemat over brick lists (compute_cell_particle_pairs_impl_default.h:37-241)
+// with a force functor F (include/xnb_compute_pair.h): every lane walks its central atom's
+// list in stream order.  A Batched functor gets the entries U at a time and makes the cutoff
+// test itself (padding entries are the far-away sentinel); otherwise the traversal accepts a
+// pair iff 0 < d2 <= rc2 (d = r_b - r_a, d2 = (dx dx + dy dy) + dz dz) and calls the
+// buffer-less form.  A ComputeBufferCompatible functor sums from 0 and the sum is stored
+// (f = 0 + sum, or f += sum with ACCUM); a buffer-less-only functor adds into the stored
+// force (from 0 without ACCUM).  The per-atom energy is written only for functors with the
+// energy field.
+//
+// EPI selects the epilogue after the force store:
 //   EPI_NONE       nothing more;
 //   EPI_KICK       v += f dth, the closing push_f_v of the step (numerical-scheme.msp);
 //   EPI_NEXT       the closing push_f_v, then the next step's push_f_v_r / push_f_v
@@ -408,14 +445,24 @@ __global__ void __launch_bounds__(1024) k_nbh_brick_mask(Geom g, BrickParams bp,
 //                  them; the last step of a run uses EPI_KICK and stores them).
 // Positions are read from r (staged in LDS) and, with EPI_NEXT, written to a second buffer,
 // so no workgroup reads a position another one has already advanced.
-template <bool MIXED, bool ACCUM, int EPI, int U>
-__device__ __forceinline__ void lj_brick_body(const Geom& g, const BrickParams& bp, uint32_t b, const double4* __restrict__ r,
-                                              const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
-                                              const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt,
-                                              const LJConst& lj, const PState& A, double dth, const NextStep& nx) {
+template <class F, bool ACCUM, int EPI, int U>
+__device__ __forceinline__ void compute_pair_brick_body(const Geom& g, const BrickParams& bp, uint32_t b,
+                                                        const double4* __restrict__ r, const uint32_t* __restrict__ cell_off,
+                                                        const uint32_t* __restrict__ cell_n, const uint16_t* __restrict__ blist,
+                                                        const uint32_t* __restrict__ bcnt, const F& func, double rc2, const PState& A,
+                                                        double dth, const NextStep& nx) {
+  using Tr = ComputePairTraits<F>;
+  static_assert(Tr::Batched || Tr::BufferLessCompatible, "the traversal calls the batched or the buffer-less form");
+  static_assert(!(EPI == EPI_NEXT && ACCUM), "a fused step starts the force from 0");
+  constexpr bool BUFFERED = Tr::ComputeBufferCompatible;
+  constexpr bool ENERGY = Tr::CentralEnergy && EPI != EPI_NEXT;
   extern __shared__ __align__(16) char lds[];
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
+  if (EPI == EPI_NEXT && threadIdx.x == 0) {  // ordered before every wave's count by brick_setup's barriers
+    wave_group_counter()[0] = 0;
+    wave_group_counter()[1] = 0;
+  }
   if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, true)) return;
   const LaneAtom At = lane_atom(bp, L);
   const uint32_t t = threadIdx.x, T = bp.T;
@@ -433,6 +480,13 @@ __device__ __forceinline__ void lj_brick_body(const Geom& g, const BrickParams& 
   constexpr int W = U / 4;
   const uint2* lst = reinterpret_cast<const uint2*>(blist + (size_t)b * bp.cap * T) + t;
   double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
+  if (!BUFFERED && ACCUM && At.live) {  // buffer-less form: pairs go straight into the stored force
+    tx = A.fx[At.slot];
+    ty = A.fy[At.slot];
+    tz = A.fz[At.slot];
+    if (Tr::CentralEnergy) te = A.ep[At.slot];
+  }
+  const int type_a = Tr::CentralType ? (int)L.pt[At.ia] : 0;
   uint2 wn[W];
   if (nmax > 0) {
 #pragma unroll
@@ -447,42 +501,36 @@ __device__ __forceinline__ void lj_brick_body(const Geom& g, const BrickParams& 
 #pragma unroll
       for (int w = 0; w < W; w++) wn[w] = nxt[(size_t)w * T];
     }
-    uint32_t j[U];
+    NbhBatch<U> nb;
+    nb.pos = L.px;
+    nb.type = L.pt;
 #pragma unroll
     for (int w = 0; w < W; w++) {
-      j[4 * w + 0] = wc[w].x & 0xffffu;
-      j[4 * w + 1] = wc[w].x >> 16;
-      j[4 * w + 2] = wc[w].y & 0xffffu;
-      j[4 * w + 3] = wc[w].y >> 16;
+      nb.j[4 * w + 0] = wc[w].x & 0xffffu;
+      nb.j[4 * w + 1] = wc[w].x >> 16;
+      nb.j[4 * w + 2] = wc[w].y & 0xffffu;
+      nb.j[4 * w + 3] = wc[w].y >> 16;
     }
     // entries past a lane's count are the sentinel (rejected by the cutoff)
-    if (MIXED) {
-      double4 rb[U];
+    if constexpr (Tr::Batched) {
+      func.template batch<U, ENERGY>(ra, nb, tx, ty, tz, te);
+    } else {
+      const NbhCells cells{L.pt, nullptr};
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        rb[u].x = L.px[3 * j[u]];
-        rb[u].y = L.py[3 * j[u]];
-        rb[u].z = L.pz[3 * j[u]];
-        rb[u].w = 0.;
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) lj_pair_mixed<EPI != EPI_NEXT>(lj, ra, rb[u], tx, ty, tz, te);
-    } else {
-      // FP64: pairs by two, each pair's positions read just before its arithmetic, and the
-      // scheduler kept from hoisting the next pair's reads (with k_lj_brick's register cap:
-      // C2 FP64 force 1.91 -> 1.78 ms against eight pairs read up front)
-#pragma unroll
-      for (int h = 0; h < U; h += 2) {
-        double4 rb[2];
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-          rb[u].x = L.px[3 * j[h + u]];
-          rb[u].y = L.py[3 * j[h + u]];
-          rb[u].z = L.pz[3 * j[h + u]];
-          rb[u].w = 0.;
+        const double4 rb = nb(u);
+        const Vec3d dr{rb.x - ra.x, rb.y - ra.y, rb.z - ra.z};
+        const double d2 = (dr.x * dr.x + dr.y * dr.y) + dr.z * dr.z;
+        if (d2 > 0.0 && d2 <= rc2) {
+          if constexpr (Tr::CentralType && Tr::CentralEnergy)
+            func(dr, d2, type_a, tx, ty, tz, te, cells, 0, nb.j[u], 1.0);
+          else if constexpr (Tr::CentralType)
+            func(dr, d2, type_a, tx, ty, tz, cells, 0, nb.j[u], 1.0);
+          else if constexpr (Tr::CentralEnergy)
+            func(dr, d2, tx, ty, tz, te, cells, 0, nb.j[u], 1.0);
+          else
+            func(dr, d2, tx, ty, tz, cells, 0, nb.j[u], 1.0);
         }
-        lj_batch_fp64<2, EPI != EPI_NEXT>(lj, ra, rb, tx, ty, tz, te);
-        __builtin_amdgcn_sched_barrier(0);
       }
     }
   }
@@ -490,7 +538,19 @@ __device__ __forceinline__ void lj_brick_body(const Geom& g, const BrickParams& 
   bool over = false;
   if (At.live) {
     const uint32_t s = At.slot;
-    if (EPI != EPI_NEXT) store_force<ACCUM>(s, tx, ty, tz, te, A.fx, A.fy, A.fz, A.ep);
+    if (EPI != EPI_NEXT) {
+      if (BUFFERED) {  // f = 0 + sum (a zeroed force) or f += sum
+        A.fx[s] = (ACCUM ? A.fx[s] : 0.0) + tx;
+        A.fy[s] = (ACCUM ? A.fy[s] : 0.0) + ty;
+        A.fz[s] = (ACCUM ? A.fz[s] : 0.0) + tz;
+        if (Tr::CentralEnergy) A.ep[s] = (ACCUM ? A.ep[s] : 0.0) + te;
+      } else {
+        A.fx[s] = tx;
+        A.fy[s] = ty;
+        A.fz[s] = tz;
+        if (Tr::CentralEnergy) A.ep[s] = te;
+      }
+    }
     if (EPI != EPI_NONE) {
       // the force as stored (0 + sum with ACCUM false, which EPI_NEXT requires)
       const double ax = EPI == EPI_NEXT ? 0.0 + tx : A.fx[s];
@@ -518,7 +578,7 @@ __device__ __forceinline__ void lj_brick_body(const Geom& g, const BrickParams& 
       A.vz[s] = uz;
     }
   }
-  if (NEXT) wave_count_add(over, nx.count);
+  if (NEXT) wave_group_count_add(over, nx.count);
 }
 
 template <bool MIXED, bool ACCUM, int EPI, int U>
@@ -530,7 +590,20 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) 
                                                    LJConst lj, PState A, double dth, NextStep nx) {
   const uint32_t b = brick_of_block<MIXED>(bp);
   if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
-  lj_brick_body<MIXED, ACCUM, EPI, U>(g, bp, b, r, cell_off, cell_n, blist, bcnt, lj, A, dth, nx);
+  compute_pair_brick_body<LennardJonesForceFunctor<MIXED>, ACCUM, EPI, U>(g, bp, b, r, cell_off, cell_n, blist, bcnt,
+                                                                          LennardJonesForceFunctor<MIXED>{lj}, lj.rc2, A, dth, nx);
+}
+
+// Any other force functor on brick lists (no fused epilogue): f += pairs (ACCUM, after
+// zero_particle_force).
+template <class F>
+__global__ void __launch_bounds__(1024) k_compute_pair_brick(Geom g, BrickParams bp, const double4* __restrict__ r,
+                                                             const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
+                                                             const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt, F func,
+                                                             double rc2, PState A) {
+  const uint32_t b = brick_of_block<true>(bp);
+  if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
+  compute_pair_brick_body<F, true, EPI_NONE, LJB_UNROLL_B>(g, bp, b, r, cell_off, cell_n, blist, bcnt, func, rc2, A, 0.0, NextStep{});
 }
 
 // ------------------------------------------------------------------ multimat force
@@ -551,6 +624,10 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
   extern __shared__ __align__(16) char lds[];
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
+  if (EPI == EPI_NEXT && threadIdx.x == 0) {  // ordered before every wave's count by brick_setup's barriers
+    wave_group_counter()[0] = 0;
+    wave_group_counter()[1] = 0;
+  }
   if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, true)) return;
   const LaneAtom At = lane_atom(bp, L);
   const uint32_t t = threadIdx.x, T = bp.T;
@@ -735,7 +812,7 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
       A.vz[s] = uz;
     }
   }
-  if (NEXT) wave_count_add(over, nx.count);
+  if (NEXT) wave_group_count_add(over, nx.count);
 }
 
 template <bool MIXED, int NT, bool ACCUM, int EPI>
@@ -913,6 +990,17 @@ hipError_t launch_ljmm_brick(int precision, int ntypes, int epi, const Geom& g, 
   return ljmm_dispatch<false, 4>(epi, g, bp, lds, cell_off, cell_n, blist, bcnt, mm, A, dth, nx, st);
 }
 
+hipError_t launch_gravity_brick(const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off, const uint32_t* cell_n,
+                                const uint16_t* blist, const uint32_t* bcnt, const GravitationalForceFunctor& gf, double rc2,
+                                hipStream_t st) {
+  const uint32_t lds = brick_layout(bp);
+  hipError_t e = set_lds((const void*)k_compute_pair_brick<GravitationalForceFunctor>, lds);
+  if (e != hipSuccess) return e;
+  k_compute_pair_brick<GravitationalForceFunctor><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, A.r, cell_off, cell_n, blist, bcnt, gf,
+                                                                                         rc2, A);
+  return hipGetLastError();
+}
+
 hipError_t launch_brick_to_slots(const Geom& g, BrickParams bp, const uint32_t* cell_off, const uint32_t* cell_n, const uint16_t* blist,
                                  const uint32_t* bcnt, uint32_t* nbr, uint32_t* nbr_cnt, uint32_t cap, hipStream_t st) {
   const uint32_t lds = brick_layout(bp);
@@ -952,7 +1040,8 @@ __device__ __forceinline__ void lj_batch_case(const BoxFrame& f, uint32_t b, uin
   A.r = f.rbuf[cur];
   // the next step's count: local step k = gstep - gstep0 + 1, count of step k + 1 at [k]
   const NextStep nx{f.rbuf[cur ^ 1u], f.backup, f.dt, f.dt2, f.thr2, f.counts + (gstep - f.gstep0 + 1)};
-  lj_brick_body<MIXED, false, EPI, LJB_UNROLL_B>(f.g, f.bp, b, A.r, f.cell_off, f.cell_n, f.blist, f.bcnt, f.lj, A, f.dth, nx);
+  compute_pair_brick_body<LennardJonesForceFunctor<MIXED>, false, EPI, LJB_UNROLL_B>(
+      f.g, f.bp, b, A.r, f.cell_off, f.cell_n, f.blist, f.bcnt, LennardJonesForceFunctor<MIXED>{f.lj}, f.lj.rc2, A, f.dth, nx);
 }
 
 // Force of every brick of every box; precision and epilogue are per box (uniform per

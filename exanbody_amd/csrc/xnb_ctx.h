@@ -62,10 +62,11 @@ struct xnb_ctx {
 
   // scan / scalar scratch
   DArr<uint32_t> bsum, u32s;            // u32s: [0] err, [1] scan total, [2] max count, [3..4] brick stats, [6] brick status,
-                                        // [7] ghost status, [8] rebuild_amr cell count
+                                        // [7] ghost status, [8] rebuild_amr cell count, [9] largest particle type
   DArr<uint32_t> amr_list;              // rebuild_amr: cells to sort
   DArr<unsigned long long> u64s;        // [0] displacement count, [1..2] pair counts
   DArr<unsigned long long> step_counts; // per-step displacement counts of xnb_run
+  int64_t last_run_steps = 0;           // steps of the last xnb_run / xnb_run_multimat
   DArr<double> red_part, red_out;
   uint32_t* h_u32 = nullptr;            // pinned
   unsigned long long* h_u64 = nullptr;  // pinned

@@ -166,6 +166,12 @@ void launch_nbh_build(const Geom& g, uint32_t n_atoms, const double4* r, const u
                       const uint32_t* cell_n, double max2, int lay, int half_symmetric, int skip_ghosts, uint32_t* nbr,
                       uint32_t* nbr_cnt, uint32_t cap, uint32_t* max_cnt, hipStream_t st);
 uint32_t lj_list_granule();
+void launch_gravity_slots(uint32_t n_atoms, const PState& A, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap,
+                          const GravitationalForceFunctor& gf, double rc2, hipStream_t st);
+hipError_t launch_gravity_brick(const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off, const uint32_t* cell_n,
+                                const uint16_t* blist, const uint32_t* bcnt, const GravitationalForceFunctor& gf, double rc2,
+                                hipStream_t st);
+void launch_max_type(uint32_t n, const double4* r, uint32_t* out, hipStream_t st);
 void launch_lj(int precision, bool accum, bool push, uint32_t n_atoms, const PState& A, const uint32_t* nbr,
                const uint32_t* nbr_cnt, uint32_t cap, double eps, double sigma, double rcut, double dth, hipStream_t st);
 int launch_lj_multimat(int precision, uint32_t n_atoms, const PState& A, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap,

@@ -562,12 +562,12 @@ __global__ void k_backup_r(Geom g, uint32_t N, const double4* __restrict__ r, co
 }
 
 // particle_displ_over (particle_displ_over.cu:38-77)
-__global__ void k_displ_over(Geom g, uint32_t N, const double4* __restrict__ r, const uint32_t* __restrict__ slot_cell,
+__global__ void __launch_bounds__(1024) k_displ_over(Geom g, uint32_t N, const double4* __restrict__ r, const uint32_t* __restrict__ slot_cell,
                              const uint32_t* __restrict__ backup, double thr2, unsigned long long* __restrict__ count) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   bool over = false;
   if (s < N) over = displaced(g, r[s], slot_cell[s], backup, s, thr2);
-  wave_count_add(over, count);
+  block_count_add(over, count);
 }
 
 // -------------------------------------------------------------------- integrators
@@ -618,7 +618,7 @@ __device__ inline bool verlet_first_half_item(const Geom& g, double4* __restrict
   return displaced(g, p, slot_cell[s], backup, s, thr2);
 }
 
-__global__ void k_verlet_first_half(Geom g, uint32_t N, double4* __restrict__ r, double* __restrict__ vx,
+__global__ void __launch_bounds__(1024) k_verlet_first_half(Geom g, uint32_t N, double4* __restrict__ r, double* __restrict__ vx,
                                     double* __restrict__ vy, double* __restrict__ vz, const double* __restrict__ fx,
                                     const double* __restrict__ fy, const double* __restrict__ fz, double dt, double dt2,
                                     double dth, const uint32_t* __restrict__ slot_cell, const uint32_t* __restrict__ backup,
@@ -626,7 +626,7 @@ __global__ void k_verlet_first_half(Geom g, uint32_t N, double4* __restrict__ r,
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   bool over = false;
   if (s < N) over = verlet_first_half_item(g, r, vx, vy, vz, fx, fy, fz, dt, dt2, dth, slot_cell, backup, thr2, s);
-  wave_count_add(over, count);
+  block_count_add(over, count);
 }
 
 __global__ void k_zero_force(uint32_t n, double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
@@ -690,25 +690,41 @@ __global__ void __launch_bounds__(256) k_nbh_build(Geom g, uint32_t n_atoms, con
 // ------------------------------------------------------------------- LJ forces
 // Pair arithmetic (lj_pair_fp64 / lj_pair_mixed) lives in xnb_pair.h.
 
-// Thread per atom, neighbours in list (= stream) order.  The trip count is the wave's largest
-// list; lanes past their own count read their own slot, which the d2 > 0 test rejects exactly
-// as the reference rejects self pairs.  U list rows are loaded, then U positions, then U pairs
-// are evaluated, so each wave keeps 3U loads in flight.  Requires cap % U == 0.
-// ACCUM: add to existing f (lennard_jones_force after zero_particle_force); otherwise
-// f = 0 + sum (fused zero_particle_force).  PUSH: fused push_f_v(0.5) on the new force.
-template <bool MIXED, bool ACCUM, bool PUSH, int U>
-__global__ void __launch_bounds__(256) k_lj(uint32_t n_atoms, const double4* __restrict__ r, const uint32_t* __restrict__ nbr,
-                                            const uint32_t* __restrict__ nbr_cnt, uint32_t cap, LJConst lj, double* __restrict__ fx,
-                                            double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ ep,
-                                            double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz, double dth) {
+// compute_pair_singlemat over per-slot lists (all cells when the lists cover the ghosts) with a
+// force functor F (include/xnb_compute_pair.h), calling its buffer-less form.  Thread per atom,
+// neighbours in list (= stream) order.  The trip count is the wave's largest list; lanes past
+// their own count read their own slot, which the d2 > 0 test rejects exactly as the reference
+// rejects self pairs.  U list rows are loaded, then U positions, then U pairs are evaluated, so
+// each wave keeps 3U loads in flight.  Requires cap % U == 0.  Summation as in the brick
+// traversal: a ComputeBufferCompatible functor sums from 0 and stores f = 0 + sum (f += sum with
+// ACCUM, after zero_particle_force); a buffer-less-only functor adds into the stored force
+// (from 0 without ACCUM).  PUSH: fused push_f_v(0.5) on the new force.
+template <class F, bool ACCUM, bool PUSH, int U>
+__global__ void __launch_bounds__(256) k_compute_pair_slots(uint32_t n_atoms, const double4* __restrict__ r,
+                                                            const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ nbr_cnt,
+                                                            uint32_t cap, F func, double rc2, double* __restrict__ fx,
+                                                            double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ ep,
+                                                            double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+                                                            double dth) {
+  using Tr = ComputePairTraits<F>;
+  static_assert(Tr::BufferLessCompatible, "the per-slot traversal calls the buffer-less form");
+  constexpr bool BUFFERED = Tr::ComputeBufferCompatible;
   const uint32_t s0 = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = s0 < n_atoms;
   const uint32_t s = live ? s0 : n_atoms - 1;
   const uint32_t n = live ? nbr_cnt[s] : 0u;
   const uint32_t nmax = wave_max_u32(n);
   const double4 ra = r[s];
+  const int type_a = Tr::CentralType ? (int)ra.w : 0;
   const uint32_t* L = nbr + nbr_index(s, 0, cap);
   double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
+  if (!BUFFERED && ACCUM) {
+    tx = fx[s];
+    ty = fy[s];
+    tz = fz[s];
+    if (Tr::CentralEnergy) te = ep[s];
+  }
+  const NbhCells cells{nullptr, r};
   for (uint32_t q0 = 0; q0 < nmax; q0 += U) {
     uint32_t j[U];
 #pragma unroll
@@ -720,12 +736,32 @@ __global__ void __launch_bounds__(256) k_lj(uint32_t n_atoms, const double4* __r
     for (int u = 0; u < U; u++) rb[u] = r[j[u]];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      if (MIXED) lj_pair_mixed(lj, ra, rb[u], tx, ty, tz, te);
-      else lj_pair_fp64(lj, ra, rb[u], tx, ty, tz, te);
+      const Vec3d dr{rb[u].x - ra.x, rb[u].y - ra.y, rb[u].z - ra.z};
+      const double d2 = (dr.x * dr.x + dr.y * dr.y) + dr.z * dr.z;
+      if (d2 > 0.0 && d2 <= rc2) {
+        if constexpr (Tr::CentralType && Tr::CentralEnergy)
+          func(dr, d2, type_a, tx, ty, tz, te, cells, 0, j[u], 1.0);
+        else if constexpr (Tr::CentralType)
+          func(dr, d2, type_a, tx, ty, tz, cells, 0, j[u], 1.0);
+        else if constexpr (Tr::CentralEnergy)
+          func(dr, d2, tx, ty, tz, te, cells, 0, j[u], 1.0);
+        else
+          func(dr, d2, tx, ty, tz, cells, 0, j[u], 1.0);
+      }
     }
   }
   if (!live) return;
-  store_force<ACCUM>(s, tx, ty, tz, te, fx, fy, fz, ep);
+  if (BUFFERED) {
+    fx[s] = (ACCUM ? fx[s] : 0.0) + tx;
+    fy[s] = (ACCUM ? fy[s] : 0.0) + ty;
+    fz[s] = (ACCUM ? fz[s] : 0.0) + tz;
+    if (Tr::CentralEnergy) ep[s] = (ACCUM ? ep[s] : 0.0) + te;
+  } else {
+    fx[s] = tx;
+    fy[s] = ty;
+    fz[s] = tz;
+    if (Tr::CentralEnergy) ep[s] = te;
+  }
   if (PUSH) {
     vx[s] += fx[s] * dth;
     vy[s] += fy[s] * dth;
@@ -806,9 +842,12 @@ __global__ void __launch_bounds__(256) k_lj_multimat(uint32_t n_atoms, const dou
 }
 
 // pair statistics of the current list: list entries and in-cutoff pairs (FP64 test)
-__global__ void k_pair_counts(uint32_t n_atoms, const double4* __restrict__ r, const uint32_t* __restrict__ nbr,
+__global__ void __launch_bounds__(256) k_pair_counts(uint32_t n_atoms, const double4* __restrict__ r, const uint32_t* __restrict__ nbr,
                               const uint32_t* __restrict__ nbr_cnt, uint32_t cap, double rc2,
                               unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long acc[2];
+  if (threadIdx.x == 0) acc[0] = acc[1] = 0;
+  __syncthreads();
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t n = 0, m = 0;
   if (s < n_atoms) {
@@ -822,8 +861,13 @@ __global__ void k_pair_counts(uint32_t n_atoms, const double4* __restrict__ r, c
       if (d2 > 0.0 && d2 <= rc2) ++m;
     }
   }
-  atomicAdd(&out[0], (unsigned long long)n);
-  atomicAdd(&out[1], (unsigned long long)m);
+  atomicAdd(&acc[0], (unsigned long long)n);
+  atomicAdd(&acc[1], (unsigned long long)m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], acc[0]);
+    atomicAdd(&out[1], acc[1]);
+  }
 }
 
 // ----------------------------------------------------------------- stream export
@@ -1066,7 +1110,7 @@ void launch_backup_r(const Geom& g, uint32_t N, const double4* r, const uint32_t
 }
 void launch_displ_over(const Geom& g, uint32_t N, const double4* r, const uint32_t* slot_cell, const uint32_t* backup, double thr2,
                        unsigned long long* count, hipStream_t st) {
-  k_displ_over<<<nblk(N, 256), 256, 0, st>>>(g, N, r, slot_cell, backup, thr2, count);
+  k_displ_over<<<nblk(N, 1024), 1024, 0, st>>>(g, N, r, slot_cell, backup, thr2, count);
 }
 void launch_push_f_v_r(uint32_t N, const PState& A, double dt, double dt2, hipStream_t st) {
   k_push_f_v_r<<<nblk(N, 256), 256, 0, st>>>(N, A.r, A.vx, A.vy, A.vz, A.fx, A.fy, A.fz, dt, dt2);
@@ -1077,7 +1121,7 @@ void launch_push_f_v(uint32_t N, const PState& A, double dt, hipStream_t st) {
 void launch_verlet_first_half(const Geom& g, uint32_t N, const PState& A, double dt, double dt2, double dth,
                               const uint32_t* slot_cell, const uint32_t* backup, double thr2, unsigned long long* count,
                               hipStream_t st) {
-  k_verlet_first_half<<<nblk(N, 256), 256, 0, st>>>(g, N, A.r, A.vx, A.vy, A.vz, A.fx, A.fy, A.fz, dt, dt2, dth, slot_cell, backup,
+  k_verlet_first_half<<<nblk(N, 1024), 1024, 0, st>>>(g, N, A.r, A.vx, A.vy, A.vz, A.fx, A.fy, A.fz, dt, dt2, dth, slot_cell, backup,
                                                     thr2, count);
 }
 void launch_zero_force(uint32_t n, const PState& A, hipStream_t st) {
@@ -1096,8 +1140,8 @@ static constexpr int LJ_UNROLL = 8;
 template <bool MIXED, bool ACCUM, bool PUSH>
 static void lj_launch(uint32_t n_atoms, const PState& A, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap, const LJConst& lj,
                       double dth, hipStream_t st) {
-  k_lj<MIXED, ACCUM, PUSH, LJ_UNROLL><<<nblk(n_atoms, 256), 256, 0, st>>>(n_atoms, A.r, nbr, nbr_cnt, cap, lj, A.fx, A.fy, A.fz, A.ep,
-                                                                        A.vx, A.vy, A.vz, dth);
+  k_compute_pair_slots<LennardJonesForceFunctor<MIXED>, ACCUM, PUSH, LJ_UNROLL><<<nblk(n_atoms, 256), 256, 0, st>>>(
+      n_atoms, A.r, nbr, nbr_cnt, cap, LennardJonesForceFunctor<MIXED>{lj}, lj.rc2, A.fx, A.fy, A.fz, A.ep, A.vx, A.vy, A.vz, dth);
 }
 
 uint32_t lj_list_granule() { return LJ_UNROLL; }
@@ -1117,6 +1161,26 @@ void launch_lj(int precision, bool accum, bool push, uint32_t n_atoms, const PSt
     case 6: lj_launch<true, true, false>(n_atoms, A, nbr, nbr_cnt, cap, lj, dth, st); break;
     default: lj_launch<true, true, true>(n_atoms, A, nbr, nbr_cnt, cap, lj, dth, st); break;
   }
+}
+
+// gravitational_force on per-slot lists (f += pairs, in stream order, into the stored force)
+void launch_gravity_slots(uint32_t n_atoms, const PState& A, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap,
+                          const GravitationalForceFunctor& gf, double rc2, hipStream_t st) {
+  if (n_atoms == 0) return;
+  k_compute_pair_slots<GravitationalForceFunctor, true, false, LJ_UNROLL><<<nblk(n_atoms, 256), 256, 0, st>>>(
+      n_atoms, A.r, nbr, nbr_cnt, cap, gf, rc2, A.fx, A.fy, A.fz, A.ep, A.vx, A.vy, A.vz, 0.0);
+}
+
+// largest particle type (r.w) of slots [0, n) -> atomicMax into *out
+__global__ void __launch_bounds__(256) k_max_type(uint32_t n, const double4* __restrict__ r, uint32_t* __restrict__ out) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t t = s < n ? (uint32_t)r[s].w : 0u;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) t = max(t, (uint32_t)__shfl_xor((int)t, o, 64));
+  if ((threadIdx.x & 63) == 0 && t) atomicMax(out, t);
+}
+void launch_max_type(uint32_t n, const double4* r, uint32_t* out, hipStream_t st) {
+  if (n) k_max_type<<<nblk(n, 256), 256, 0, st>>>(n, r, out);
 }
 
 int launch_lj_multimat(int precision, uint32_t n_atoms, const PState& A, const uint32_t* nbr, const uint32_t* nbr_cnt, uint32_t cap,
@@ -1374,7 +1438,7 @@ __global__ void __launch_bounds__(kBatchSlotBlock) kb_first_half(const BoxFrame*
   if (s < f.N)
     over = verlet_first_half_item(f.g, X.A.r, X.A.vx, X.A.vy, X.A.vz, X.A.fx, X.A.fy, X.A.fz, f.dt, f.dt2, f.dth, X.scA, f.backup,
                                   f.thr2, s);
-  wave_count_add(over, f.counts + (gstep - f.gstep0));  // local step k = gstep - gstep0 + 1 -> [k - 1]
+  block_count_add(over, f.counts + (gstep - f.gstep0));  // local step k = gstep - gstep0 + 1 -> [k - 1]
 }
 
 __global__ void kb_zero_words(uint32_t* __restrict__ words, const uint32_t* __restrict__ boxes, uint32_t n) {

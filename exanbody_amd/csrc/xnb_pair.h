@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/xnb_compute_pair.h"
+
 namespace xnb {
 
 // Plain FP64 form: the reference traversal and functor arithmetic, neighbours in stream
@@ -219,6 +221,110 @@ __device__ inline void lj_batch_fp64(const LJConst& lj, const double4& ra, const
     }
   }
 }
+
+// ------------------------------------------------------------------ force functors
+// LennardJonesForceFunctor (lennard_jones.cu:59-125) for the pair traversal: the buffer-less
+// form (lennard_jones.cu:105-124, plus the per-atom energy field) and the batched form the
+// brick traversal uses.  MIXED selects the mixed form (FP32 pair coefficient, FP64
+// displacement, cutoff test and accumulation).
+template <bool MIXED>
+struct LennardJonesForceFunctor {
+  LJConst p;
+
+  __device__ inline void operator()(Vec3d dr, double d2, double& fx, double& fy, double& fz, double& ep, const NbhCells&, size_t,
+                                    size_t, double w) const {
+    if (MIXED) {
+      float s6;
+      const double c = (double)lj_coef_f32(__builtin_amdgcn_rcpf((float)d2), p.sigma6f, p.m24epsf, s6);
+      fx = fma(c, dr.x, fx);
+      fy = fma(c, dr.y, fy);
+      fz = fma(c, dr.z, fz);
+      ep += (double)lj_energy_f32(s6, p.eps4f);
+    } else {
+      // lj_compute_energy (lennard_jones.cu:46-56), then de *= w / r, f += de * dr
+      const double r = sqrt(d2);
+      const double inv_r = 1.0 / r;
+      const double ratio = p.sigma * inv_r;
+      const double ratio2 = ratio * ratio;
+      const double ratio6 = ratio2 * ratio2 * ratio2;
+      const double ratio12 = ratio6 * ratio6;
+      const double e = p.eps4 * (ratio12 - ratio6);
+      double de = (p.m24eps * (2. * ratio12 - ratio6)) * inv_r;
+      de *= w / r;
+      fx += de * dr.x;
+      fy += de * dr.y;
+      fz += de * dr.z;
+      ep += e;
+    }
+  }
+
+  // U list entries in list order (sentinel padding included); pairs outside 0 < d2 <= rc^2
+  // add +-0.  FP64: pairs by two, each pair's positions read just before its arithmetic and
+  // the scheduler kept from hoisting the next pair's reads (with the force kernel's register
+  // cap: 8.4M-atom FP64 force 1.91 -> 1.78 ms against eight pairs read up front).
+  template <int U, bool ENERGY>
+  __device__ __forceinline__ void batch(const double4& ra, const NbhBatch<U>& nb, double& fx, double& fy, double& fz,
+                                        double& ep) const {
+    if (MIXED) {
+      double4 rb[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) rb[u] = nb(u);
+#pragma unroll
+      for (int u = 0; u < U; u++) lj_pair_mixed<ENERGY>(p, ra, rb[u], fx, fy, fz, ep);
+    } else {
+#pragma unroll
+      for (int h = 0; h < U; h += 2) {
+        double4 rb[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) rb[u] = nb(h + u);
+        lj_batch_fp64<2, ENERGY>(p, ra, rb, fx, fy, fz, ep);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+};
+
+template <bool MIXED>
+struct ComputePairTraits<LennardJonesForceFunctor<MIXED>> {
+  static constexpr bool ComputeBufferCompatible = true;  // lennard_jones.cu:132-138
+  static constexpr bool BufferLessCompatible = true;
+  static constexpr bool CentralType = false;
+  static constexpr bool CentralEnergy = true;
+  static constexpr bool Batched = true;
+};
+
+// GravitationalForceFunctor (contribs/pi/gravitational_force.cu:44-82): Newtonian attraction
+// between point masses given per particle type (the particle_type_properties "mass" scalar),
+// e = -G m_a m_b / r, de = G m_a m_b / r^2, then de *= w / r and f += de * dr.  Types are at
+// most kGravityTypes; the operator checks every particle's type against the table first.
+static constexpr int kGravityTypes = 16;
+struct GravitationalForceFunctor {
+  double G;
+  double mass[kGravityTypes];
+
+  __device__ inline void operator()(Vec3d dr, double d2, int type_a, double& fx, double& fy, double& fz, const NbhCells& cells,
+                                    size_t neighbor_cell, size_t neighbor_particle, double w) const {
+    const double mass_a = mass[type_a];
+    const int type_b = cells[neighbor_cell][field::type][neighbor_particle];
+    const double mass_b = mass[type_b];
+    const double r = sqrt(d2);
+    const double inv_r = 1.0 / r;
+    double de = G * mass_a * mass_b * inv_r * inv_r;  // gravitational_compute_energy (e is not used)
+    de *= w / r;
+    fx += de * dr.x;
+    fy += de * dr.y;
+    fz += de * dr.z;
+  }
+};
+
+template <>
+struct ComputePairTraits<GravitationalForceFunctor> {
+  static constexpr bool ComputeBufferCompatible = false;  // gravitational_force.cu:136-143
+  static constexpr bool BufferLessCompatible = true;
+  static constexpr bool CentralType = true;
+  static constexpr bool CentralEnergy = false;
+  static constexpr bool Batched = false;
+};
 
 inline LJConst make_lj_const(double eps, double sigma, double rcut) {
   LJConst c;

@@ -66,3 +66,19 @@ def slab_droplets(n_atoms=4_194_304, ncell=128, n_droplets=64, radius=5.0, frac_
     v *= np.sqrt(temperature * (3 * n - 3) / np.sum(v * v))
     ids = np.arange(n, dtype=np.uint64)
     return r, v, ids, types, cubic_domain(L, min_cell)
+
+
+def batch_job_list(n_boxes=512, steps=500, seed=5, precision=None, rebuild_every=10):
+    """The batched ensemble: n_boxes periodic LJ boxes, atoms log-uniform in [500, 50000]
+    (FCC-filled in lattice order, jittered +-0.01), density U[0.70, 0.90], T* U[0.5, 1.5],
+    mixed precision unless given, as job dicts (serve.make_job)."""
+    from . import _lib
+    from .serve import make_job
+    prec = _lib.PRECISION_MIXED if precision is None else precision
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for q in range(n_boxes):
+        n = int(round(np.exp(rng.uniform(np.log(500), np.log(50000)))))
+        jobs.append(make_job(n, rng.uniform(0.70, 0.90), rng.uniform(0.5, 1.5), steps, prec, seed=7000 + q,
+                             rebuild_every=rebuild_every))
+    return jobs

@@ -139,6 +139,14 @@ int xnb_lennard_jones_force(xnb_ctx* ctx, const xnb_lj_params* params, double rc
  * epsilon, sigma, rcut arrays of ntypes*(ntypes+1)/2 entries. */
 int xnb_lennard_jones_force_multimat(xnb_ctx* ctx, int ntypes, const double* epsilon,
                                      const double* sigma, const double* rcut, int precision);
+/* gravitational_force (contribs/pi/gravitational_force.cu:163-225, functor :44-82): over the
+ * pairs within rcut, f += G m_a m_b (r_b - r_a) / r^3 with the masses given per particle type
+ * (the "mass" type property; ntypes 1..16, every particle's type below ntypes, else
+ * XNB_E_ARG), each pair added into the stored force in stream order (zero_particle_force
+ * first for the force alone); ghost: the ghost cells too (lists over all cells).  Per-atom
+ * energies are not touched.  Runs the library's pair traversal with the reference's
+ * buffer-less functor form (include/xnb_compute_pair.h). */
+int xnb_gravitational_force(xnb_ctx* ctx, double G, int ntypes, const double* type_mass, double rcut, int ghost);
 /* push_f_v_r (src/defbox/include/exanb/defbox/push_vec3_2nd_order.h:29-119) */
 int xnb_push_f_v_r(xnb_ctx* ctx, double dt, double dt_scale);
 /* push_f_v (src/defbox/include/exanb/defbox/push_vec3_1st_order.h:29-108) */
@@ -171,6 +179,10 @@ int xnb_run(xnb_ctx* ctx, xnb_run_params* params);
  * used; nbh_dist must cover the largest pair cutoff). */
 int xnb_run_multimat(xnb_ctx* ctx, xnb_run_params* params, int ntypes, const double* eps_pair, const double* sigma_pair,
                      const double* rcut_pair);
+/* particle_displ_over counts (particle_displ_over.cu:38-77) of every step of the last xnb_run /
+ * xnb_run_multimat: counts[k - 1] = atoms past max_displ after step k's first half (before its
+ * rebuild, if any).  n <= the steps of that run. */
+int xnb_run_step_counts(xnb_ctx* ctx, uint64_t* counts, size_t n);
 
 /* ---- measurement ----------------------------------------------------------- */
 #define XNB_TIMER_FORCE 0

@@ -61,6 +61,7 @@ def lib(path=None):
         L.op_zero_force.argtypes = [P, I]
         L.op_lj_force.argtypes = [P, D, D, D, I]
         L.op_lj_force_multimat.argtypes = [P, I, P, P, P]
+        L.op_gravitational_force.argtypes = [P, D, I, P, D, I]
         L.op_push_f_v_r.argtypes = [P, D, D]
         L.op_push_f_v.argtypes = [P, D, D]
         L.op_stream_size.argtypes = [P, SZ]; L.op_stream_size.restype = SZ
@@ -184,6 +185,11 @@ class Oracle:
         s = np.ascontiguousarray(sigma_pair, np.float64)
         rc = np.ascontiguousarray(rcut_pair, np.float64)
         _check(self.L.op_lj_force_multimat(self._h, int(ntypes), _ptr(e), _ptr(s), _ptr(rc)), "lj_force_multimat")
+
+    def gravitational_force(self, G, type_mass, rcut, ghost=False):
+        m = np.ascontiguousarray(type_mass, np.float64)
+        _check(self.L.op_gravitational_force(self._h, float(G), int(m.size), _ptr(m), float(rcut), int(bool(ghost))),
+               "gravitational_force")
 
     def push_f_v_r(self, dt, dt_scale=1.0):
         _check(self.L.op_push_f_v_r(self._h, float(dt), float(dt_scale)), "push_f_v_r")

@@ -939,6 +939,64 @@ int op_lj_force(op_sim* s, double epsilon, double sigma, double rcut, int ghost)
   return err ? OP_E_ARG : OP_OK;
 }
 
+/* gravitational_force: compute_cell_particle_pairs over inner cells (all cells if ghost) with
+ * the buffer-less GravitationalForceFunctor (contribs/pi/gravitational_force.cu:56-82): per
+ * particle, neighbours in stream order, accepted iff 0 < d2 <= rcut^2
+ * (compute_cell_particle_pairs_impl_default.h:183-186), each pair added straight into the
+ * particle's force field: gravitational_compute_energy (:44-50) gives de = G m_a m_b / r^2,
+ * then de *= w / r (w = 1) and f += de * dr.  A type without a mass is an error. */
+int op_gravitational_force(op_sim* s, double G, int ntypes, const double* type_mass, double rcut, int ghost) {
+  if (!s->streams_valid) return OP_E_STATE;
+  if (ntypes < 1 || !type_mass) return OP_E_ARG;
+  for (size_t c = 0; c < s->ncells; c++) {
+    if (!ghost && is_ghost_cell(s, c)) continue;
+    for (uint32_t p = 0; p < s->cells[c].n; p++)
+      if (s->cells[c].type[p] >= ntypes) return OP_E_ARG;
+  }
+  const double rcut2 = rcut * rcut;
+  const int64_t* d = s->dims;
+#pragma omp parallel for schedule(dynamic)
+  for (size_t ca = 0; ca < s->ncells; ca++) {
+    if (!ghost && is_ghost_cell(s, ca)) continue;
+    op_cell* A = &s->cells[ca];
+    const uint32_t na = A->n;
+    const uint32_t* off;
+    int32_t shift;
+    const uint16_t* base = stream_info(s->stream[ca], na, &off, &shift);
+    const uint16_t* st = base;
+    for (uint32_t pa = 0; pa < na; pa++) {
+      if (off) st = base + off[pa] + shift;
+      const double mass_a = type_mass[A->type[pa]];
+      unsigned cg = *(st++);
+      for (; cg > 0; cg--) {
+        uint16_t enc = *(st++);
+        unsigned nch = *(st++);
+        int64_t ri = (int64_t)(enc & 31) - 16, rj = (int64_t)((enc >> 5) & 31) - 16, rk = (int64_t)((enc >> 10) & 31) - 16;
+        int64_t cb = (int64_t)ca + ((rk * d[1]) + rj) * d[0] + ri;
+        const op_cell* B = &s->cells[cb];
+        for (; nch > 0; nch--) {
+          uint32_t pb = *(st++);
+          double dx = B->rx[pb] - A->rx[pa];
+          double dy = B->ry[pb] - A->ry[pa];
+          double dz = B->rz[pb] - A->rz[pa];
+          double d2 = (dx * dx + dy * dy) + dz * dz;
+          if (d2 > 0.0 && d2 <= rcut2) {
+            const double mass_b = type_mass[B->type[pb]];
+            const double r = sqrt(d2);
+            const double inv_r = 1.0 / r;
+            double de = G * mass_a * mass_b * inv_r * inv_r;
+            de *= 1.0 / r;
+            A->fx[pa] += de * dx;
+            A->fy[pa] += de * dy;
+            A->fz[pa] += de * dz;
+          }
+        }
+      }
+    }
+  }
+  return OP_OK;
+}
+
 /* unique_pair_id (src/core/include/exanb/core/particle_type_pair.h:39-55) */
 static inline unsigned unique_pair_id(unsigned a, unsigned b) {
   if (a > b) { unsigned t = a; a = b; b = t; }

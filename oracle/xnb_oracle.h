@@ -87,6 +87,9 @@ int op_push_f_v(op_sim* s, double dt, double dt_scale);   /* push_f_v */
 
 /* Multi-material LJ (compute_pair_multimat.h:100-316 restated): ntypes types,
  * per unique pair (unique_pair_id, particle_type_pair.h:39-55) epsilon, sigma, rcut. */
+/* gravitational_force (contribs/pi/gravitational_force.cu:163-225): buffer-less functor
+ * (:56-82), masses per type; pairs added into the stored force in stream order. */
+int op_gravitational_force(op_sim* s, double G, int ntypes, const double* type_mass, double rcut, int ghost);
 int op_lj_force_multimat(op_sim* s, int ntypes, const double* eps_pair,
                          const double* sigma_pair, const double* rcut_pair);
 
