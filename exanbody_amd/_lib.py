@@ -76,6 +76,7 @@ _SIGS = {
     "xnb_push_f_v_r": (C.c_int, [P, C.c_double, C.c_double]),
     "xnb_push_f_v": (C.c_int, [P, C.c_double, C.c_double]),
     "xnb_simulation_energies": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "xnb_simulation_energies_noalloc": (C.c_int, [P, P]),
     "xnb_init": (C.c_int, [P, C.POINTER(LJParams), C.c_double, C.c_double, C.c_int]),
     "xnb_run": (C.c_int, [P, C.POINTER(RunParams)]),
     "xnb_run_multimat": (C.c_int, [P, C.POINTER(RunParams), C.c_int, P, P, P]),

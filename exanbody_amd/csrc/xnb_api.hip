@@ -1150,6 +1150,52 @@ int xnb_simulation_energies(xnb_ctx* c, double* ke, double* pe) {
   return XNB_OK;
 }
 
+}  // extern "C"
+
+// ---- energies without allocation: the reduction partials live in a fixed device array
+namespace {
+constexpr uint32_t kEnergyParts = 1024;  // (ke, pe) partial pairs: one per reduction workgroup
+// Each GPU the library runs on holds its own copy (16 KiB) until the process ends.  Host code
+// never uses the name: kernels get the address hipGetSymbolAddress gives per device.
+static __device__ __align__(256) double scratch[2 * kEnergyParts];
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+double* scratch_dev[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device (c->device), looked up on the first
+// call on that device
+int scratch_for(xnb_ctx* c, double** out) {
+  if (c->device < 0 || c->device >= kMaxDevices) return fail(c, XNB_E_ARG, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (!scratch_dev[c->device]) {
+    void* ptr = nullptr;
+    const hipError_t he = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (he != hipSuccess || !ptr) return fail(c, XNB_E_HIP, std::string("scratch address: ") + hipGetErrorString(he));
+    scratch_dev[c->device] = (double*)ptr;
+  }
+  *out = scratch_dev[c->device];
+  return XNB_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int xnb_simulation_energies_noalloc(xnb_ctx* c, double* ke_pe) {
+  int e;
+  if ((e = check_ctx(c))) return e;  // makes the context's device current
+  if (!ke_pe) return fail(c, XNB_E_ARG, "null output");
+  // the same partial count as xnb_simulation_energies, so the sums are the same bit for bit;
+  // the reduction strides over the atoms, so every particle count fits kEnergyParts partials
+  const uint32_t nparts = std::max<uint32_t>(1, std::min<uint32_t>(1024, nb256(c->N)));
+  if (nparts > kEnergyParts) return fail(c, XNB_E_CAPACITY, "more reduction partials than the scratch array holds");
+  double* part = nullptr;
+  if ((e = scratch_for(c, &part))) return e;
+  // every partial read by the final sum is written first by this call: nothing to clear
+  launch_energies(c->N, c->A, part, nparts, ke_pe, c->st);
+  HIPCHK(c, hipGetLastError());
+  return XNB_OK;
+}
+
 int xnb_init(xnb_ctx* c, const xnb_lj_params* p, double rcut, double rcut_inc, int precision) {
   int e;
   if ((e = check_ctx(c))) return e;

@@ -153,6 +153,14 @@ int xnb_push_f_v_r(xnb_ctx* ctx, double dt, double dt_scale);
 int xnb_push_f_v(xnb_ctx* ctx, double dt, double dt_scale);
 /* kinetic energy (mass 1, src/compute/simulation_stats.cpp:82-101) and PE = 1/2 sum ep */
 int xnb_simulation_energies(xnb_ctx* ctx, double* ke, double* pe);
+/* No counterpart in the reference.  xnb_simulation_energies for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc):
+ * the same kernels and sums, bitwise, with the reduction partials in a fixed per-GPU device
+ * array of 1024 (ke, pe) pairs (16 KiB) instead of a library allocation; every particle count
+ * fits.  ke_pe is caller-owned DEVICE memory for 2 doubles (ke, pe), written asynchronously on
+ * the context's stream (xnb_synchronize before reading it from the host).  Calls on one GPU
+ * share the array: calls that overlap on one GPU (two contexts' streams) race. */
+int xnb_simulation_energies_noalloc(xnb_ctx* ctx, double* ke_pe);
 
 /* ---- composite step loop --------------------------------------------------- */
 /* first_iteration (data/config/compute-loop.msp:1-7): move_particles, backup_r, ghosts,
