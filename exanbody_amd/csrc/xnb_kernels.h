@@ -30,6 +30,7 @@ struct BrickParams {
   uint32_t cap;   // list rows per brick slab
   uint32_t* status;  // device word: bit 0 set when a region exceeds Rmax, bit 1 when a brick exceeds T
   uint32_t off_rcnt, off_rpref, off_roff, off_bpref, off_pos, off_type;  // LDS byte offsets
+  uint32_t off_rdel, off_crow;  // staging tables: per region row, per 64 region atoms
 };
 
 // Particle arrays of one storage buffer (slots [0, N) inner, [N, N+G) ghosts).
