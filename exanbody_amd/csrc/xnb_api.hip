@@ -347,7 +347,7 @@ int choose_bricks(xnb_ctx* c, uint32_t cap, BrickParams& out) {
   return NOT_APPLICABLE;
 }
 
-int op_nbh_brick(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& cfg) {
+int op_nbh_brick(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& cfg, const PairCut& pc) {
   const double max2 = nbh_dist * nbh_dist;
   int e;
   uint32_t cap = (c->nbr_cap + list_granule() - 1) / list_granule() * list_granule();
@@ -359,7 +359,7 @@ int op_nbh_brick(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& 
     HIPCHK(c, hipMemsetAsync(c->u32s.p + 2, 0, sizeof(uint32_t), c->st));
     HIPCHK(c, hipMemsetAsync(c->u32s.p + 6, 0, sizeof(uint32_t), c->st));
     tstart(c, XNB_TIMER_NBH);
-    hipError_t he = launch_nbh_brick(c->g, bp, c->A.r, c->cell_off.p, c->cell_n.p, max2, cfg.half_symmetric, cfg.skip_ghosts,
+    hipError_t he = launch_nbh_brick(c->g, bp, c->A.r, c->cell_off.p, c->cell_n.p, max2, pc, cfg.half_symmetric, cfg.skip_ghosts,
                                      c->blist.p, c->bcnt.p, c->u32s.p + 2, c->st);
     tstop(c);
     if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick neighbour build: ") + hipGetErrorString(he));
@@ -376,6 +376,7 @@ int op_nbh_brick(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& 
       c->list_atoms = c->N;
       c->list_cfg = cfg;
       c->lists_valid = true;
+      c->lists_pruned = pc.nt > 0;
       return XNB_OK;
     }
     cap = grow_cap(mx);
@@ -383,12 +384,14 @@ int op_nbh_brick(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& 
   return NOT_APPLICABLE;
 }
 
-int op_nbh(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& cfg, int all_cells) {
+int op_nbh(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& cfg, int all_cells, const PairCut* pc) {
   const int lay = (int)std::ceil(nbh_dist / c->g.cs);  // cell_layers (amr_grid_algorithm.h:451)
   if (lay > 15) return fail(c, XNB_E_ARG, "neighbour distance spans more than 15 cells");
   int e;
+  c->lists_pruned = false;
   if (!all_cells && lay == 1 && c->g.gl >= 1 && c->use_bricks && c->N > 0) {
-    e = op_nbh_brick(c, nbh_dist, cfg);
+    const PairCut none{};
+    e = op_nbh_brick(c, nbh_dist, cfg, (pc && !cfg.half_symmetric && !cfg.skip_ghosts) ? *pc : none);
     if (e != NOT_APPLICABLE) return e;
   }
   const uint32_t n_atoms = all_cells ? c->N + c->G : c->N;
@@ -474,14 +477,23 @@ int launch_force_multimat(xnb_ctx* c, int precision, const LJMulti& mm, int ntyp
   return XNB_OK;
 }
 
-int full_update(xnb_ctx* c, bool forces) {
+int full_update(xnb_ctx* c, bool forces, const PairCut* pc) {
   int e;
   if ((e = op_move(c, forces, false))) return e;
   if ((e = op_amr(c, kSubGridDensity, forces))) return e;
   if ((e = op_backup(c))) return e;
   if ((e = op_ghost_all(c, true, forces))) return e;
   xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
-  return op_nbh(c, c->nbh_dist_lab, cfg, 0);
+  return op_nbh(c, c->nbh_dist_lab, cfg, 0, pc);
+}
+
+// Lists a multi-material run built with per type-pair radii hold only the pairs of their
+// pair's radius; any other consumer gets full lists (nbh_dist_lab, inner cells) rebuilt on the
+// same positions first.
+int ensure_full_lists(xnb_ctx* c) {
+  if (!c->lists_valid || !c->lists_pruned) return XNB_OK;
+  xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
+  return op_nbh(c, c->nbh_dist_lab, cfg, 0, nullptr);
 }
 
 int check_ctx(xnb_ctx* c) {
@@ -1004,7 +1016,7 @@ int xnb_chunk_neighbors_stream(xnb_ctx* c, uint16_t* out, size_t cap, uint64_t* 
   int e;
   if ((e = check_ctx(c))) return e;
   if (!c->lists_valid) return fail(c, XNB_E_STATE, "no neighbour lists (chunk_neighbors)");
-  if ((e = ensure_slot_lists(c))) return e;
+  if ((e = ensure_full_lists(c)) || (e = ensure_slot_lists(c))) return e;
   const Geom& g = c->g;
   const int ntables = c->list_cfg.build_particle_offset ? (c->list_cfg.dual_particle_offset ? 2 : 1) : 0;
   launch_stream_plen(c->list_atoms, c->slot_cell_A.p, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, c->list_cfg.random_access, c->plen.p,
@@ -1046,6 +1058,7 @@ int xnb_lennard_jones_force(xnb_ctx* c, const xnb_lj_params* p, double rcut, int
   if ((e = check_ctx(c))) return e;
   if (!p || (precision != XNB_PRECISION_FP64 && precision != XNB_PRECISION_MIXED)) return fail(c, XNB_E_ARG, "bad LJ arguments");
   if (!c->lists_valid) return fail(c, XNB_E_STATE, "no neighbour lists (chunk_neighbors)");
+  if ((e = ensure_full_lists(c))) return e;
   c->last_rcut = rcut;
   tstart(c, XNB_TIMER_FORCE);
   e = launch_force(c, precision, true, false, ghost != 0, p->epsilon, p->sigma, rcut, 0.0);
@@ -1061,6 +1074,7 @@ int xnb_lennard_jones_force_multimat(xnb_ctx* c, int ntypes, const double* eps, 
   if ((e = check_ctx(c))) return e;
   if (ntypes <= 0 || ntypes > 4 || !eps || !sigma || !rcut) return fail(c, XNB_E_ARG, "bad multimat arguments (1..4 types)");
   if (!c->lists_valid) return fail(c, XNB_E_STATE, "no neighbour lists");
+  if ((e = ensure_full_lists(c))) return e;
   LJMulti mm;
   if (make_lj_multi(ntypes, eps, sigma, rcut, mm)) return fail(c, XNB_E_ARG, "too many type pairs");
   tstart(c, XNB_TIMER_FORCE);
@@ -1085,6 +1099,7 @@ int xnb_gravitational_force(xnb_ctx* c, double G, int ntypes, const double* type
   if (ntypes < 1 || ntypes > kGravityTypes || !type_mass || !(rcut > 0.0))
     return fail(c, XNB_E_ARG, "bad gravitational_force arguments (1..16 types, rcut > 0)");
   if (!c->lists_valid) return fail(c, XNB_E_STATE, "no neighbour lists (chunk_neighbors)");
+  if ((e = ensure_full_lists(c))) return e;
   const uint32_t n_atoms = ghost ? c->N + c->G : c->N;
   // every type indexes the mass table (the reference reads m_type_mass[type] unchecked)
   HIPCHK(c, hipMemsetAsync(c->u32s.p + 9, 0, sizeof(uint32_t), c->st));
@@ -1223,11 +1238,27 @@ int xnb_init(xnb_ctx* c, const xnb_lj_params* p, double rcut, double rcut_inc, i
 namespace xnb_impl {
 
 // The step loop of xnb_run / xnb_run_multimat (mm == nullptr: single-material LJ).
-int run_loop(xnb_ctx* c, xnb_run_params* p, const LJMulti* mm, int ntypes) {
+int run_loop(xnb_ctx* c, xnb_run_params* p, const LJMulti* mm, int ntypes, const double* rcut_pair) {
   int e;
   if ((e = check_ctx(c))) return e;
   if (!p || p->steps < 0) return fail(c, XNB_E_ARG, "bad run parameters");
   if (!c->lists_valid || !c->ghosts_valid || !c->backup_valid) return fail(c, XNB_E_STATE, "xnb_init must run first");
+  // A multi-material run rebuilds its lists with each type pair's radius rc_ij + 2 max_displ
+  // (at most nbh_dist): pairs beyond rc_ij add +-0 to the sums, so the forces are bitwise those
+  // of full lists, and the skin argument holds pair by pair.
+  PairCut pc{};
+  if (mm) {
+    pc.nt = ntypes;
+    for (int a = 0; a < ntypes; a++)
+      for (int b = 0; b < ntypes; b++) {
+        const double rl = std::min(c->nbh_dist_lab, rcut_pair[unique_pair_id((unsigned)a, (unsigned)b)] + 2.0 * c->max_displ);
+        pc.thr2[a][b] = rl * rl;
+      }
+  }
+  // lists pruned by another run's radii: start from full lists
+  if (c->lists_pruned && (!mm || std::memcmp(&pc, &c->pc_used, sizeof(PairCut)) != 0))
+    if ((e = ensure_full_lists(c))) return e;
+  c->pc_used = pc;
   const double dt = p->dt * 1.0, dt2 = dt * dt * 0.5, dth = p->dt * 0.5;
   const double thr2 = c->max_displ * c->max_displ;
   const bool interval = p->rebuild_every > 0;
@@ -1257,7 +1288,7 @@ int run_loop(xnb_ctx* c, xnb_run_params* p, const LJMulti* mm, int ntypes) {
       trigger = c->h_u64[0] > 0;
     }
     if (trigger) {
-      if ((e = full_update(c, false))) return e;  // f, ep: overwritten by this step's force kernel
+      if ((e = full_update(c, false, mm ? &pc : nullptr))) return e;  // f, ep: overwritten by this step's force kernel
       p->n_rebuilds++;
       if (interval) rebuilt[(size_t)step - 1] = 1;
     } else {
@@ -1299,7 +1330,7 @@ int run_loop(xnb_ctx* c, xnb_run_params* p, const LJMulti* mm, int ntypes) {
 
 extern "C" {
 
-int xnb_run(xnb_ctx* c, xnb_run_params* p) { return run_loop(c, p, nullptr, 0); }
+int xnb_run(xnb_ctx* c, xnb_run_params* p) { return run_loop(c, p, nullptr, 0, nullptr); }
 
 int xnb_run_step_counts(xnb_ctx* c, uint64_t* counts, size_t n) {
   int e;
@@ -1317,7 +1348,7 @@ int xnb_run_multimat(xnb_ctx* c, xnb_run_params* p, int ntypes, const double* ep
   if (ntypes <= 0 || ntypes > 4 || !eps || !sigma || !rcut) return fail(c, XNB_E_ARG, "bad multimat arguments (1..4 types)");
   LJMulti mm;
   if (make_lj_multi(ntypes, eps, sigma, rcut, mm)) return fail(c, XNB_E_ARG, "too many type pairs");
-  return run_loop(c, p, &mm, ntypes);
+  return run_loop(c, p, &mm, ntypes, rcut);
 }
 
 // ---- autotuner: brick shape (cells per workgroup tile) by measurement, cached per class
@@ -1341,6 +1372,7 @@ int xnb_autotune(xnb_ctx* c, const xnb_lj_params* p, double rcut, int precision,
   if ((e = check_ctx(c))) return e;
   if (!p) return fail(c, XNB_E_ARG, "null parameters");
   if (!c->lists_valid || !c->ghosts_valid) return fail(c, XNB_E_STATE, "xnb_init must run first");
+  if ((e = ensure_full_lists(c))) return e;
   if (c->list_kind != 1) {  // not on the brick path: nothing to tune
     if (dims_out) dims_out[0] = dims_out[1] = dims_out[2] = 0;
     return XNB_OK;
@@ -1453,7 +1485,7 @@ int xnb_pair_counts(xnb_ctx* c, uint64_t* list_entries, uint64_t* cutoff_pairs) 
   int e;
   if ((e = check_ctx(c))) return e;
   if (!c->lists_valid) return fail(c, XNB_E_STATE, "no neighbour lists");
-  if ((e = ensure_slot_lists(c))) return e;
+  if ((e = ensure_full_lists(c)) || (e = ensure_slot_lists(c))) return e;
   HIPCHK(c, hipMemsetAsync(c->u64s.p + 1, 0, 2 * sizeof(unsigned long long), c->st));
   launch_pair_counts(c->N, c->A.r, c->nbr.p, c->nbr_cnt.p, c->nbr_cap, c->last_rcut * c->last_rcut, c->u64s.p + 1, c->st);
   HIPCHK(c, hipGetLastError());

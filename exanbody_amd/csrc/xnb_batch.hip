@@ -420,6 +420,7 @@ int xnb_batch_attach(xnb_batch* B, xnb_ctx* c, const xnb_run_params* p, int* slo
   if ((e = check_ctx(c))) return e;
   if (c->device != B->device) return bfail(B, XNB_E_ARG, "box on another device");
   if (!c->lists_valid || !c->ghosts_valid || !c->backup_valid) return bfail(B, XNB_E_STATE, "xnb_init must run first");
+  if (ensure_full_lists(c)) return bfail(B, XNB_E_STATE, std::string("full neighbour lists: ") + c->err);
   // work and rebuild_amr list entries hold a cell or block index in kWorkBoxShift bits
   if (c->g.ncells > kWorkBlockMask) return bfail(B, XNB_E_CAPACITY, "box has too many cells for a batch");
   int i = -1;

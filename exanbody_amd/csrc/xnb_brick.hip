@@ -305,11 +305,14 @@ __global__ void k_brick_stats(Geom g, BrickParams bp, const uint32_t* __restrict
 // (a per-cell loop; the unfiltered build walks rows of three cells).
 // Brick b of one box; the workgroup may be wider than bp.T (batched launches), lanes from
 // bp.T on hold no atom and touch no list.
-template <bool FILTER>
+// PAIRWISE (multi-material runs): a candidate within nbh_dist is stored only when it is also
+// within its type pair's list radius (PairCut); its d2 is recomputed as in the test.
+template <bool FILTER, bool PAIRWISE = false>
 __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams& bp, uint32_t b, const double4* __restrict__ r,
                                                const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
                                                double max2, int half_symmetric, int skip_ghosts, uint16_t* __restrict__ blist,
-                                               uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt) {
+                                               uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt,
+                                               const PairCut* pc = nullptr) {
   extern __shared__ __align__(16) char lds[];
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
@@ -330,6 +333,12 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
   }
   const uint32_t last = bp.cap - 1;
   uint32_t cnt = 0;
+  double thr[4] = {max2, max2, max2, max2};  // this atom's list radius^2 per neighbour type
+  if (PAIRWISE) {
+    const unsigned ta = min((unsigned)L.pt[A.ia], 3u);
+#pragma unroll
+    for (int q = 0; q < 4; q++) thr[q] = q < pc->nt ? pc->thr2[ta][q] : max2;
+  }
   if constexpr (!FILTER) {
     // Rows: the cells di = -1, 0, 1 of one (dk, dj) are consecutive in LDS, so a row is one
     // candidate range [rbase, rbase + rlen).  A chunk tests 4-aligned candidates p0 + k
@@ -375,6 +384,18 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
           while (m) {
             const uint32_t k = (uint32_t)__builtin_clz(m);
             m ^= 0x80000000u >> k;
+            if (PAIRWISE) {
+              const uint32_t ib = lb + k;
+              const double dx = ax - L.px[3 * ib];
+              const double dy = ay - L.py[3 * ib];
+              const double dz = az - L.pz[3 * ib];
+              const double d2 = (dx * dx + dy * dy) + dz * dz;
+              const unsigned tb = L.pt[ib];
+              double lim = thr[0];
+#pragma unroll
+              for (int q = 1; q < 4; q++) lim = tb == (unsigned)q ? thr[q] : lim;
+              if (!(d2 <= lim)) continue;
+            }
             *reinterpret_cast<uint16_t*>(ob + wo) = (uint16_t)(lb + k);
             ++cnt;
             wo += (cnt & 3u) ? 2u : wstep;
@@ -438,14 +459,14 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
   for (uint32_t q = min(cnt, bp.cap); q < pad_end; q++) out[lpos(q, t, T)] = sentinel;
 }
 
-template <bool FILTER>
+template <bool FILTER, bool PAIRWISE>
 __global__ void __launch_bounds__(1024) k_nbh_brick_mask(Geom g, BrickParams bp, const double4* __restrict__ r,
                                                          const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
                                                          double max2, int half_symmetric, int skip_ghosts, uint16_t* __restrict__ blist,
-                                                         uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt) {
+                                                         uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt, PairCut pc) {
   const uint32_t b = brick_of_block<true>(bp);
   if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
-  nbh_brick_body<FILTER>(g, bp, b, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist, bcnt, max_cnt);
+  nbh_brick_body<FILTER, PAIRWISE>(g, bp, b, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist, bcnt, max_cnt, &pc);
 }
 
 // ------------------------------------------------------------------ pair traversal
@@ -932,20 +953,28 @@ static hipError_t set_lds(const void* fn, uint32_t bytes) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+// pc.nt > 0: per type-pair list radii (not with the half_symmetric / skip_ghosts filters)
 hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n,
-                            double max2, int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt, uint32_t* max_cnt,
-                            hipStream_t st) {
+                            double max2, const PairCut& pc, int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt,
+                            uint32_t* max_cnt, hipStream_t st) {
   const uint32_t lds = brick_layout(bp);
   hipError_t e;
   if (half_symmetric || skip_ghosts) {
-    e = set_lds((const void*)k_nbh_brick_mask<true>, lds);
+    if (pc.nt > 0) return hipErrorInvalidValue;
+    e = set_lds((const void*)k_nbh_brick_mask<true, false>, lds);
     if (e != hipSuccess) return e;
-    k_nbh_brick_mask<true><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist, bcnt,
-                                                          max_cnt);
+    k_nbh_brick_mask<true, false><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts,
+                                                                        blist, bcnt, max_cnt, pc);
+  } else if (pc.nt > 0) {
+    e = set_lds((const void*)k_nbh_brick_mask<false, true>, lds);
+    if (e != hipSuccess) return e;
+    k_nbh_brick_mask<false, true><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, 0, 0, blist, bcnt, max_cnt,
+                                                                        pc);
   } else {
-    e = set_lds((const void*)k_nbh_brick_mask<false>, lds);
+    e = set_lds((const void*)k_nbh_brick_mask<false, false>, lds);
     if (e != hipSuccess) return e;
-    k_nbh_brick_mask<false><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, 0, 0, blist, bcnt, max_cnt);
+    k_nbh_brick_mask<false, false><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, 0, 0, blist, bcnt, max_cnt,
+                                                                         pc);
   }
   return hipGetLastError();
 }

@@ -58,6 +58,8 @@ struct xnb_ctx {
   uint32_t max_neighbors = 0;
   xnb_chunk_neighbors_config list_cfg{1, 0, 0, 0, 0};
   bool lists_valid = false, ghosts_valid = false, backup_valid = false;
+  bool lists_pruned = false;  // built with per type-pair radii inside xnb_run_multimat
+  PairCut pc_used{};          // the radii of the last run (pc_used.nt = 0: single material)
   DArr<uint16_t> stream_buf;
 
   // scan / scalar scratch
@@ -139,8 +141,10 @@ int check_ctx(xnb_ctx* c);
 int op_ghost_all(xnb_ctx* c, bool move_check = false, bool forces = true);
 int op_backup(xnb_ctx* c);
 int op_amr(xnb_ctx* c, double density, bool forces);
-int op_nbh(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& cfg, int all_cells);
-int full_update(xnb_ctx* c, bool forces = true);  // forces false: f and ep neither carried by the move nor copied to ghosts
+int op_nbh(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& cfg, int all_cells, const PairCut* pc = nullptr);
+// forces false: f and ep neither carried by the move nor copied to ghosts; pc: per type-pair list radii
+int full_update(xnb_ctx* c, bool forces = true, const PairCut* pc = nullptr);
+int ensure_full_lists(xnb_ctx* c);
 void tstart(xnb_ctx* c, int id);
 void tstop(xnb_ctx* c);
 

@@ -33,6 +33,13 @@ struct BrickParams {
   uint32_t off_rdel, off_crow;  // staging tables: per region row, per 64 region atoms
 };
 
+// Per type-pair list radii of a multi-material run: a candidate b of an atom a is kept when
+// d2 <= thr2[type a][type b] (each at most the build's nbh_dist^2); nt = 0: one radius for all.
+struct PairCut {
+  int nt;
+  double thr2[4][4];
+};
+
 // Particle arrays of one storage buffer (slots [0, N) inner, [N, N+G) ghosts).
 // Force-kernel epilogues (see k_lj_brick)
 enum { EPI_NONE = 0, EPI_KICK = 1, EPI_NEXT = 2 };
@@ -194,6 +201,7 @@ uint32_t brick_layout(BrickParams& bp);
 uint32_t brick_list_granule();
 void launch_brick_stats(const Geom& g, const BrickParams& bp, const uint32_t* cell_n, uint32_t* out, hipStream_t st);
 hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, double max2,
+                            const PairCut& pc,
                             int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt, uint32_t* max_cnt, hipStream_t st);
 hipError_t launch_lj_brick(int precision, bool accum, int epi, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
                            const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, double eps, double sigma, double rcut,

@@ -72,3 +72,48 @@ def test_multimat_run_bitwise():
     assert_same_layout(oe, ge, "after multimat run")
     for k in ("v", "f", "ep"):
         np.testing.assert_array_equal(oe[k].view(np.uint64), ge[k].view(np.uint64), err_msg=k)
+
+
+def test_multimat_run_pair_radii_then_full_lists():
+    """xnb_run_multimat keeps lists of each type pair's radius; the operators that follow
+    (stream export, the multimat force) see full lists again: the CS=1 streams of every inner
+    cell equal the oracle's built on the same positions, and the forces stay bitwise."""
+    o, b = _pair(seed=33)
+    b.lennard_jones_force_multimat(2, EPS, SIG, RC, precision=PRECISION_FP64)
+    dt, steps, every = 0.002, 22, 10
+    nbh = RC.max() + SKIN
+    for k in range(1, steps + 1):
+        o.push_f_v_r(dt, 1.0)
+        o.push_f_v(dt, 0.5)
+        if k % every == 0:
+            o.move_particles()
+            o.rebuild_amr()
+            o.backup_r()
+            o.ghost_update_all()
+            o.chunk_neighbors(nbh)
+        else:
+            o.ghost_update_r()
+        o.zero_force()
+        o.lj_force_multimat(2, EPS, SIG, RC)
+        o.push_f_v(dt, 0.5)
+    b.run_multimat(steps, dt, 2, EPS, SIG, RC, SKIN, precision=PRECISION_FP64, rebuild_every=every)
+    oe, ge = o.export(), b.export()
+    assert_same_layout(oe, ge, "after multimat run")
+    np.testing.assert_array_equal(oe["f"].view(np.uint64), ge["f"].view(np.uint64))
+    o.chunk_neighbors(nbh)  # full lists on the current positions, as the device rebuilds them
+    gs, gmax = b.streams()
+    os_ = o.streams()
+    dims, gl, nc = b.grid_info()
+    inner = 0
+    for c in range(nc):
+        k, rem = divmod(c, dims[0] * dims[1])
+        j, i = divmod(rem, dims[0])
+        if all(gl <= x < d - gl for x, d in zip((i, j, k), dims)):
+            np.testing.assert_array_equal(gs[c], os_[c], err_msg=f"stream of cell {c}")
+            inner += 1
+    assert inner > 0
+    o.zero_force()
+    b.zero_particle_force()
+    o.lj_force_multimat(2, EPS, SIG, RC)
+    b.lennard_jones_force_multimat(2, EPS, SIG, RC, precision=PRECISION_FP64)
+    np.testing.assert_array_equal(o.export()["f"].view(np.uint64), b.export()["f"].view(np.uint64))
