@@ -57,6 +57,9 @@ _SIGS = {
     "xnb_nbh_dist": (C.c_int, [P, C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                C.POINTER(C.c_double)]),
     "xnb_set_particles": (C.c_int, [P, C.c_size_t, P, P, P, P]),
+    "xnb_set_particles_with_fields": (C.c_int, [P, C.c_size_t, P, P, P, P, C.c_int, P]),
+    "xnb_get_fields": (C.c_int, [P, C.c_int, C.c_int, P]),
+    "xnb_field_count": (C.c_int, [P, C.POINTER(C.c_int)]),
     "xnb_num_particles": (C.c_int, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "xnb_grid_info": (C.c_int, [P, P, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]),
     "xnb_get_particles": (C.c_int, [P, C.c_int, P, P, P, P, P, P, P]),

@@ -84,13 +84,16 @@ class Box:
         self._check(self.L.xnb_nbh_dist(self._h, rcut_max, rcut_inc, C.byref(a), C.byref(b), C.byref(c)), "nbh_dist")
         return dict(nbh_dist_lab=a.value, max_displ=b.value, ghost_dist=c.value)
 
-    def set_particles(self, r, v=None, ids=None, types=None):
+    def set_particles(self, r, v=None, ids=None, types=None, extra=None):
+        """extra: a list of up to 4 arrays of n values (extra per-particle fields)."""
         r = np.ascontiguousarray(r, np.float64)
         v = None if v is None else np.ascontiguousarray(v, np.float64)
         ids = None if ids is None else np.ascontiguousarray(ids, np.uint64)
         types = None if types is None else np.ascontiguousarray(types, np.uint8)
-        self._check(self.L.xnb_set_particles(self._h, r.shape[0], _ptr(r), _ptr(v), _ptr(ids), _ptr(types)),
-                    "set_particles")
+        xs = [np.ascontiguousarray(x, np.float64) for x in (extra or [])]
+        ptrs = (C.c_void_p * max(len(xs), 1))(*[x.ctypes.data for x in xs])
+        self._check(self.L.xnb_set_particles_with_fields(self._h, r.shape[0], _ptr(r), _ptr(v), _ptr(ids), _ptr(types),
+                                                         len(xs), ptrs), "set_particles")
 
     def num_particles(self):
         a, b = C.c_size_t(), C.c_size_t()
@@ -117,6 +120,14 @@ class Box:
         self._check(self.L.xnb_get_particles(self._h, int(ghosts), _ptr(out["cell"]), _ptr(out["r"]), _ptr(out["v"]),
                                              _ptr(out["f"]), _ptr(out["ep"]), _ptr(out["id"]), _ptr(out["type"])),
                     "get_particles")
+        nxc = C.c_int()
+        self._check(self.L.xnb_field_count(self._h, C.byref(nxc)), "field_count")
+        nx = nxc.value
+        if nx:
+            xs = [np.zeros(n) for _ in range(nx)]
+            ptrs = (C.c_void_p * nx)(*[x.ctypes.data for x in xs])
+            self._check(self.L.xnb_get_fields(self._h, int(ghosts), nx, ptrs), "get_fields")
+            out["extra"] = xs
         return out
 
     # ---- operators

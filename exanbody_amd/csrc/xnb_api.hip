@@ -111,8 +111,13 @@ int set_geometry(xnb_ctx* c, double ghost_dist) {
 }
 
 void bind_states(xnb_ctx* c) {
-  c->A = PState{c->rA.p, c->vA[0].p, c->vA[1].p, c->vA[2].p, c->fA[0].p, c->fA[1].p, c->fA[2].p, c->eA.p, c->idA.p};
-  c->B = PState{c->rB.p, c->vB[0].p, c->vB[1].p, c->vB[2].p, c->fB[0].p, c->fB[1].p, c->fB[2].p, c->eB.p, c->idB.p};
+  c->A = PState{c->rA.p, c->vA[0].p, c->vA[1].p, c->vA[2].p, c->fA[0].p, c->fA[1].p, c->fA[2].p, c->eA.p, c->idA.p, {}, 0};
+  c->B = PState{c->rB.p, c->vB[0].p, c->vB[1].p, c->vB[2].p, c->fB[0].p, c->fB[1].p, c->fB[2].p, c->eB.p, c->idB.p, {}, 0};
+  for (int k = 0; k < c->n_extra; k++) {
+    c->A.x[k] = c->xA[k].p;
+    c->B.x[k] = c->xB[k].p;
+  }
+  c->A.nx = c->B.nx = (uint32_t)c->n_extra;
 }
 
 // (Re)allocate particle buffers to hold ncap slots; the first keep slots of buffer A are kept.
@@ -145,6 +150,8 @@ int ensure_capacity(xnb_ctx* c, size_t ncap, size_t keep) {
   }
   if ((e = grow(c->eA, newcap))) return e;
   if ((e = grow(c->idA, newcap))) return e;
+  for (int k = 0; k < c->n_extra; k++)
+    if ((e = grow(c->xA[k], newcap))) return e;
   if ((e = grow(c->slot_cell_A, newcap))) return e;
   // B and per-slot scratch need no content
   dfree(c->rB);
@@ -160,6 +167,10 @@ int ensure_capacity(xnb_ctx* c, size_t ncap, size_t keep) {
   dfree(c->slot_cell_B);
   if ((e = dalloc(c, c->eB, newcap))) return e;
   if ((e = dalloc(c, c->idB, newcap))) return e;
+  for (int k = 0; k < c->n_extra; k++) {
+    dfree(c->xB[k]);
+    if ((e = dalloc(c, c->xB[k], newcap))) return e;
+  }
   if ((e = dalloc(c, c->slot_cell_B, newcap))) return e;
   for (DArr<uint32_t>* a : {&c->mv_dst, &c->pk, &c->newloc, &c->in_list, &c->plen, &c->pstart}) {
     dfree(*a);
@@ -197,6 +208,7 @@ void swap_buffers(xnb_ctx* c) {
   }
   std::swap(c->eA, c->eB);
   std::swap(c->idA, c->idB);
+  for (int k = 0; k < c->n_extra; k++) std::swap(c->xA[k], c->xB[k]);
   std::swap(c->slot_cell_A, c->slot_cell_B);
   bind_states(c);
 }
@@ -602,6 +614,7 @@ int xnb_destroy(xnb_ctx* c) {
   dfree(c->slot_cell_A); dfree(c->slot_cell_B); dfree(c->rA); dfree(c->rB);
   for (int a = 0; a < 3; a++) { dfree(c->vA[a]); dfree(c->vB[a]); dfree(c->fA[a]); dfree(c->fB[a]); }
   dfree(c->eA); dfree(c->eB); dfree(c->idA); dfree(c->idB);
+  for (int k = 0; k < kMaxExtraFields; k++) { dfree(c->xA[k]); dfree(c->xB[k]); }
   for (DArr<uint32_t>* a : {&c->cell_off, &c->cell_n, &c->cnt_out, &c->cnt_in, &c->new_n, &c->new_off, &c->in_cur, &c->gcnt, &c->goff,
                             &c->clen, &c->coff, &c->mv_dst, &c->pk, &c->newloc, &c->in_list, &c->plen, &c->pstart, &c->ghost_src,
                             &c->ghost_shift, &c->backup, &c->nbr, &c->nbr_cnt, &c->bsum, &c->u32s, &c->amr_list})
@@ -652,10 +665,18 @@ int xnb_nbh_dist(xnb_ctx* c, double rcut_max, double rcut_inc, double* nbh_dist_
 }
 
 int xnb_set_particles(xnb_ctx* c, size_t n, const double* r, const double* v, const uint64_t* id, const uint8_t* type) {
+  return xnb_set_particles_with_fields(c, n, r, v, id, type, 0, nullptr);
+}
+
+int xnb_set_particles_with_fields(xnb_ctx* c, size_t n, const double* r, const double* v, const uint64_t* id,
+                                  const uint8_t* type, int n_extra, const double* const* extra) {
   int e;
   if ((e = check_ctx(c))) return e;
   if (n > 0 && !r) return fail(c, XNB_E_ARG, "null positions");
   if (n >= 0x7FFFFFFFu) return fail(c, XNB_E_CAPACITY, "too many particles");
+  if (n_extra < 0 || n_extra > kMaxExtraFields) return fail(c, XNB_E_ARG, "0 to 4 extra fields");
+  for (int k = 0; k < n_extra; k++)
+    if (n > 0 && (!extra || !extra[k])) return fail(c, XNB_E_ARG, "null extra field");
   const Geom& g = c->g;
   // host binning in input order (domain_periodic_location, domain.h:160-191)
   std::vector<uint32_t> cell(n), count(g.ncells, 0);
@@ -691,17 +712,26 @@ int xnb_set_particles(xnb_ctx* c, size_t n, const double* r, const double* v, co
   for (int a = 0; a < 3; a++) hv[a].assign(n, 0.0);
   std::vector<uint64_t> hid(n);
   std::vector<uint32_t> hsc(n);
+  std::vector<double> hx[kMaxExtraFields];
+  for (int k = 0; k < n_extra; k++) hx[k].resize(n);
   for (size_t p = 0; p < n; p++) {
     const uint32_t t = cur[cell[p]]++;
     hr[t] = make_double4(wr[3 * p], wr[3 * p + 1], wr[3 * p + 2], type ? (double)type[p] : 0.0);
     for (int a = 0; a < 3; a++) hv[a][t] = v ? v[3 * p + a] : 0.0;
     hid[t] = id ? id[p] : (uint64_t)p;
     hsc[t] = cell[p];
+    for (int k = 0; k < n_extra; k++) hx[k][t] = extra[k][p];
   }
   c->N = (uint32_t)n;
   c->G = 0;
   const size_t est = n + std::max<size_t>(n / 2, 4096);
+  c->n_extra = n_extra;
   if ((e = ensure_capacity(c, est, 0))) return e;
+  for (int k = 0; k < n_extra; k++)  // fields declared on a context that already had capacity
+    if ((e = dalloc(c, c->xA[k], c->cap)) || (e = dalloc(c, c->xB[k], c->cap))) return e;
+  bind_states(c);
+  for (int k = 0; k < n_extra; k++)
+    if (n > 0) HIPCHK(c, hipMemcpyAsync(c->xA[k].p, hx[k].data(), n * sizeof(double), hipMemcpyHostToDevice, c->st));
   if (n > 0) {
     HIPCHK(c, hipMemcpyAsync(c->A.r, hr.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->A.vx, hv[0].data(), n * sizeof(double), hipMemcpyHostToDevice, c->st));
@@ -730,7 +760,7 @@ int xnb_set_particles(xnb_ctx* c, size_t n, const double* r, const double* v, co
 namespace {
 struct SnapHeader {
   char magic[8];  // "XNBSNAP1"
-  uint32_t version, n, ncells, pad;
+  uint32_t version, n, ncells, n_extra;  // n_extra: extra fields after the backup block
   double nbh_dist_lab, max_displ, ghost_dist;
   xnb_domain domain;
 };
@@ -744,8 +774,8 @@ bool same_domain(const xnb_domain& a, const xnb_domain& b) {
   return a.cell_size == b.cell_size;
 }
 
-size_t snap_bytes(uint32_t n, uint32_t ncells) {
-  return sizeof(SnapHeader) + (size_t)ncells * 4 + (size_t)n * (32 + 24 + 24 + 8 + 8 + 12);
+size_t snap_bytes(uint32_t n, uint32_t ncells, uint32_t n_extra) {
+  return sizeof(SnapHeader) + (size_t)ncells * 4 + (size_t)n * (32 + 24 + 24 + 8 + 8 + 12 + 8 * (size_t)n_extra);
 }
 }  // namespace
 
@@ -753,14 +783,14 @@ int xnb_snapshot_size(xnb_ctx* c, size_t* bytes) {
   int e;
   if ((e = check_ctx(c))) return e;
   if (!bytes) return fail(c, XNB_E_ARG, "null argument");
-  *bytes = snap_bytes(c->N, c->g.ncells);
+  *bytes = snap_bytes(c->N, c->g.ncells, (uint32_t)c->n_extra);
   return XNB_OK;
 }
 
 int xnb_snapshot_save(xnb_ctx* c, void* buf, size_t bytes) {
   int e;
   if ((e = check_ctx(c))) return e;
-  if (!buf || bytes < snap_bytes(c->N, c->g.ncells)) return fail(c, XNB_E_CAPACITY, "snapshot buffer too small");
+  if (!buf || bytes < snap_bytes(c->N, c->g.ncells, (uint32_t)c->n_extra)) return fail(c, XNB_E_CAPACITY, "snapshot buffer too small");
   if (!c->backup_valid || !c->nbh_set) return fail(c, XNB_E_STATE, "snapshot needs an initialised box (xnb_init)");
   const uint32_t n = c->N, nc = c->g.ncells;
   SnapHeader h;
@@ -769,6 +799,7 @@ int xnb_snapshot_save(xnb_ctx* c, void* buf, size_t bytes) {
   h.version = kSnapVersion;
   h.n = n;
   h.ncells = nc;
+  h.n_extra = (uint32_t)c->n_extra;
   h.nbh_dist_lab = c->nbh_dist_lab;
   h.max_displ = c->max_displ;
   h.ghost_dist = c->ghost_dist;
@@ -796,6 +827,8 @@ int xnb_snapshot_save(xnb_ctx* c, void* buf, size_t bytes) {
       (e = d2h(c->A.fz, (size_t)n * 8)) || (e = d2h(c->A.ep, (size_t)n * 8)) || (e = d2h(c->A.id, (size_t)n * 8)) ||
       (e = d2h(c->backup.p, (size_t)n * 12)))
     return e;
+  for (int k = 0; k < c->n_extra; k++)
+    if ((e = d2h(c->xA[k].p, (size_t)n * 8))) return e;
   HIPCHK(c, hipStreamSynchronize(c->st));
   return XNB_OK;
 }
@@ -810,7 +843,8 @@ int xnb_snapshot_load(xnb_ctx* c, const void* buf, size_t bytes) {
   std::memcpy(&h, buf, sizeof(h));
   if (std::memcmp(h.magic, "XNBSNAP1", 8) != 0 || h.version != kSnapVersion) return fail(c, XNB_E_ARG, "not a snapshot");
   if (!same_domain(h.domain, c->dom)) return fail(c, XNB_E_DOMAIN, "snapshot of another domain");
-  if (bytes < snap_bytes(h.n, h.ncells)) return fail(c, XNB_E_ARG, "truncated snapshot");
+  if (h.n_extra > (uint32_t)kMaxExtraFields) return fail(c, XNB_E_ARG, "snapshot with more extra fields than supported");
+  if (bytes < snap_bytes(h.n, h.ncells, h.n_extra)) return fail(c, XNB_E_ARG, "truncated snapshot");
   if ((e = set_geometry(c, h.ghost_dist))) return e;
   if (h.ncells != c->g.ncells) return fail(c, XNB_E_DOMAIN, "snapshot grid differs");
   c->nbh_set = true;
@@ -836,7 +870,11 @@ int xnb_snapshot_load(xnb_ctx* c, const void* buf, size_t bytes) {
   }
   c->N = n;
   c->G = 0;
+  c->n_extra = (int)h.n_extra;
   if ((e = ensure_capacity(c, (size_t)n + std::max<size_t>(n / 2, 4096), 0))) return e;
+  for (int k = 0; k < c->n_extra; k++)
+    if ((e = dalloc(c, c->xA[k], c->cap)) || (e = dalloc(c, c->xB[k], c->cap))) return e;
+  bind_states(c);
   auto h2d = [&](void* dst, size_t sz) -> int {
     HIPCHK(c, hipMemcpyAsync(dst, o, sz, hipMemcpyHostToDevice, c->st));
     o += sz;
@@ -847,6 +885,8 @@ int xnb_snapshot_load(xnb_ctx* c, const void* buf, size_t bytes) {
       (e = h2d(c->A.fz, (size_t)n * 8)) || (e = h2d(c->A.ep, (size_t)n * 8)) || (e = h2d(c->A.id, (size_t)n * 8)) ||
       (e = h2d(c->backup.p, (size_t)n * 12)))
     return e;
+  for (int k = 0; k < c->n_extra; k++)
+    if ((e = h2d(c->xA[k].p, (size_t)n * 8))) return e;
   HIPCHK(c, hipMemcpyAsync(c->slot_cell_A.p, sc.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->cell_n.p, cn.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->cell_off.p, off.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice, c->st));
@@ -885,6 +925,39 @@ int xnb_get_cell_counts(xnb_ctx* c, uint32_t* counts) {
   if (!counts) return fail(c, XNB_E_ARG, "null output");
   HIPCHK(c, hipMemcpyAsync(counts, c->cell_n.p, c->g.ncells * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
+
+int xnb_field_count(const xnb_ctx* c, int* n_extra) {
+  if (!c || !n_extra) return XNB_E_ARG;
+  *n_extra = c->n_extra;
+  return XNB_OK;
+}
+
+int xnb_get_fields(xnb_ctx* c, int include_ghosts, int n_extra, double* const* out) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (n_extra < 0 || n_extra > c->n_extra || (n_extra && !out)) return fail(c, XNB_E_ARG, "more fields than the grid holds");
+  const size_t tot = (size_t)c->N + c->G;
+  const Geom& g = c->g;
+  std::vector<double> hx[kMaxExtraFields];
+  std::vector<uint32_t> off(g.ncells), cn(g.ncells);
+  for (int k = 0; k < n_extra; k++) {
+    hx[k].resize(tot);
+    if (tot) HIPCHK(c, hipMemcpyAsync(hx[k].data(), c->xA[k].p, tot * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  }
+  HIPCHK(c, hipMemcpyAsync(off.data(), c->cell_off.p, g.ncells * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(cn.data(), c->cell_n.p, g.ncells * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  size_t q2 = 0;
+  for (uint32_t q = 0; q < g.ncells; q++) {  // the order of xnb_get_particles
+    int i, j, kk;
+    cell_ijk(g, q, i, j, kk);
+    if (!include_ghosts && is_ghost(g, i, j, kk)) continue;
+    for (uint32_t p = 0; p < cn[q]; p++, q2++)
+      for (int k = 0; k < n_extra; k++)
+        if (out[k]) out[k][q2] = hx[k][(size_t)off[q] + p];
+  }
   return XNB_OK;
 }
 

@@ -175,6 +175,7 @@ void ctx_from_frame(xnb_batch* B, int i, uint32_t par) {
     }
     std::swap(c->eA, c->eB);
     std::swap(c->idA, c->idB);
+    for (int k = 0; k < c->n_extra; k++) std::swap(c->xA[k], c->xB[k]);
     std::swap(c->slot_cell_A, c->slot_cell_B);
   }
   bind_states(c);

@@ -45,6 +45,8 @@ struct xnb_ctx {
   DArr<double4> rA, rB;
   DArr<double> vA[3], vB[3], fA[3], fB[3], eA, eB;
   DArr<uint64_t> idA, idB;
+  int n_extra = 0;  // extra per-particle fields (xnb_set_particles_with_fields)
+  DArr<double> xA[kMaxExtraFields], xB[kMaxExtraFields];
 
   // per cell
   DArr<uint32_t> cell_off, cell_n, cnt_out, cnt_in, new_n, new_off, in_cur, gcnt, goff, clen, coff;

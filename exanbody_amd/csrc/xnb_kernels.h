@@ -53,12 +53,19 @@ struct NextStep {
   unsigned long long* count;
 };
 
+// Extra per-particle FP64 fields a grid may carry beside the fixed ones (a field set with more
+// fields, make_grid_variant_operator.h:104-164): moved, sorted and copied to ghost images with
+// their particle, never read by the force kernels.
+constexpr int kMaxExtraFields = 4;
+
 struct PState {
   double4* r;  // x, y, z, type
   double *vx, *vy, *vz;
   double *fx, *fy, *fz;
   double* ep;  // per-atom potential energy (sum of pair energies over the full list)
   uint64_t* id;
+  double* x[kMaxExtraFields];  // extra fields [0, nx)
+  uint32_t nx;
 };
 
 // ------------------------------------------------------------------ batched boxes

@@ -89,6 +89,19 @@ int xnb_nbh_dist(xnb_ctx* ctx, double rcut_max, double rcut_inc, double* nbh_dis
  * v, id, type may be NULL (zero velocity, id = index, type 0). */
 int xnb_set_particles(xnb_ctx* ctx, size_t n, const double* r, const double* v,
                       const uint64_t* id, const uint8_t* type);
+/* A grid whose field set holds extra per-particle FP64 fields (a field set beyond
+ * rx..type, as grid variants of make_grid_variant_operator.h:104-164 / grid_fields.h:114-142
+ * carry them): xnb_set_particles plus n_extra (0..4) host arrays of n values in input order.
+ * Every operator that moves, re-orders or images particles carries them (move_particles,
+ * rebuild_amr, ghost_update_all, the batch engine, snapshots); no force operator reads them.
+ * xnb_set_particles is this call with n_extra = 0 (r, v, id, type as there: type NULL is a
+ * grid without the type field, every particle type 0; id NULL numbers them 0..n-1). */
+int xnb_set_particles_with_fields(xnb_ctx* ctx, size_t n, const double* r, const double* v, const uint64_t* id,
+                                  const uint8_t* type, int n_extra, const double* const* extra);
+/* The first n_extra extra fields, in the order of xnb_get_particles (out[k] may be NULL). */
+int xnb_get_fields(xnb_ctx* ctx, int include_ghosts, int n_extra, double* const* out);
+/* Extra fields the grid holds (also after xnb_snapshot_load). */
+int xnb_field_count(const xnb_ctx* ctx, int* n_extra);
 int xnb_num_particles(const xnb_ctx* ctx, size_t* n_inner, size_t* n_ghost);
 int xnb_grid_info(const xnb_ctx* ctx, int64_t dims[3], int* ghost_layers, size_t* n_cells);
 /* Particles in cell order (ghost cells too if include_ghosts); any pointer may be NULL. */
