@@ -89,6 +89,7 @@ _SIGS = {
     "xnb_timer_read": (C.c_int, [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "xnb_pair_counts": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "xnb_synchronize": (C.c_int, [P]),
+    "xnb_device_bytes": (C.c_int, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "xnb_copy_probe": (C.c_int, [P, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "xnb_fp64_core": (C.c_int, [P, P, C.c_size_t, P, P]),
     "xnb_autotune": (C.c_int, [P, C.POINTER(LJParams), C.c_double, C.c_int, P]),
@@ -105,6 +106,7 @@ _SIGS = {
     "xnb_batch_graph_stats": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "xnb_batch_detach": (C.c_int, [P, C.c_int, C.POINTER(RunParams)]),
     "xnb_job_bytes": (C.c_size_t, [C.c_size_t]),
+    "xnb_job_bytes_for": (C.c_size_t, [P]),
     "xnb_serve_run": (C.c_int, [P, C.c_int, P, P, P]),
     "xnb_deck_create": (C.c_int, [C.POINTER(P)]),
     "xnb_deck_destroy": (C.c_int, [P]),

@@ -282,5 +282,11 @@ class Box:
         self._check(self.L.xnb_fp64_core(self._h, _ptr(x), x.size, _ptr(sq), _ptr(rc)), "fp64_core")
         return sq, rc
 
+    def device_bytes(self):
+        """(device bytes held now, most held at once) by this context."""
+        a, b = C.c_size_t(), C.c_size_t()
+        self._check(self.L.xnb_device_bytes(self._h, C.byref(a), C.byref(b)), "device_bytes")
+        return a.value, b.value
+
     def synchronize(self):
         self._check(self.L.xnb_synchronize(self._h), "synchronize")

@@ -135,10 +135,10 @@ int ensure_capacity(xnb_ctx* c, size_t ncap, size_t keep) {
     if (arr.p && keep) he = hipMemcpyAsync(na.p, arr.p, keep * sizeof(T), hipMemcpyDeviceToDevice, c->st);
     if (he == hipSuccess) he = hipStreamSynchronize(c->st);
     if (he != hipSuccess) {
-      dfree(na);
+      dfree(c, na);
       return fail(c, XNB_E_HIP, std::string("capacity copy failed: ") + hipGetErrorString(he));
     }
-    dfree(arr);
+    dfree(c, arr);
     arr = na;
     return XNB_OK;
   };
@@ -154,26 +154,26 @@ int ensure_capacity(xnb_ctx* c, size_t ncap, size_t keep) {
     if ((e = grow(c->xA[k], newcap))) return e;
   if ((e = grow(c->slot_cell_A, newcap))) return e;
   // B and per-slot scratch need no content
-  dfree(c->rB);
+  dfree(c, c->rB);
   if ((e = dalloc(c, c->rB, newcap))) return e;
   for (int a = 0; a < 3; a++) {
-    dfree(c->vB[a]);
-    dfree(c->fB[a]);
+    dfree(c, c->vB[a]);
+    dfree(c, c->fB[a]);
     if ((e = dalloc(c, c->vB[a], newcap))) return e;
     if ((e = dalloc(c, c->fB[a], newcap))) return e;
   }
-  dfree(c->eB);
-  dfree(c->idB);
-  dfree(c->slot_cell_B);
+  dfree(c, c->eB);
+  dfree(c, c->idB);
+  dfree(c, c->slot_cell_B);
   if ((e = dalloc(c, c->eB, newcap))) return e;
   if ((e = dalloc(c, c->idB, newcap))) return e;
   for (int k = 0; k < c->n_extra; k++) {
-    dfree(c->xB[k]);
+    dfree(c, c->xB[k]);
     if ((e = dalloc(c, c->xB[k], newcap))) return e;
   }
   if ((e = dalloc(c, c->slot_cell_B, newcap))) return e;
   for (DArr<uint32_t>* a : {&c->mv_dst, &c->pk, &c->newloc, &c->in_list, &c->plen, &c->pstart}) {
-    dfree(*a);
+    dfree(c, *a);
     if ((e = dalloc(c, *a, newcap))) return e;
   }
   // the ghost scheme (one entry per ghost) stays valid across a growth: ghost_update_r reads it
@@ -183,7 +183,7 @@ int ensure_capacity(xnb_ctx* c, size_t ncap, size_t keep) {
     const size_t ng = std::min<size_t>(c->G, a->n);
     if (a->p && ng) HIPCHK(c, hipMemcpyAsync(na.p, a->p, ng * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    dfree(*a);
+    dfree(c, *a);
     *a = na;
   }
   {  // backup_r of the kept (inner) slots survives: a ghost rebuild may grow the buffers after it
@@ -192,7 +192,7 @@ int ensure_capacity(xnb_ctx* c, size_t ncap, size_t keep) {
     if (c->backup.p && keep)
       HIPCHK(c, hipMemcpyAsync(nb.p, c->backup.p, 3 * keep * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    dfree(c->backup);
+    dfree(c, c->backup);
     c->backup = nb;
   }
   c->cap = newcap;
@@ -611,16 +611,16 @@ int xnb_destroy(xnb_ctx* c) {
     (void)hipEventDestroy(t.b);
   }
   for (auto ev : c->pool) (void)hipEventDestroy(ev);
-  dfree(c->slot_cell_A); dfree(c->slot_cell_B); dfree(c->rA); dfree(c->rB);
-  for (int a = 0; a < 3; a++) { dfree(c->vA[a]); dfree(c->vB[a]); dfree(c->fA[a]); dfree(c->fB[a]); }
-  dfree(c->eA); dfree(c->eB); dfree(c->idA); dfree(c->idB);
-  for (int k = 0; k < kMaxExtraFields; k++) { dfree(c->xA[k]); dfree(c->xB[k]); }
+  dfree(c, c->slot_cell_A); dfree(c, c->slot_cell_B); dfree(c, c->rA); dfree(c, c->rB);
+  for (int a = 0; a < 3; a++) { dfree(c, c->vA[a]); dfree(c, c->vB[a]); dfree(c, c->fA[a]); dfree(c, c->fB[a]); }
+  dfree(c, c->eA); dfree(c, c->eB); dfree(c, c->idA); dfree(c, c->idB);
+  for (int k = 0; k < kMaxExtraFields; k++) { dfree(c, c->xA[k]); dfree(c, c->xB[k]); }
   for (DArr<uint32_t>* a : {&c->cell_off, &c->cell_n, &c->cnt_out, &c->cnt_in, &c->new_n, &c->new_off, &c->in_cur, &c->gcnt, &c->goff,
                             &c->clen, &c->coff, &c->mv_dst, &c->pk, &c->newloc, &c->in_list, &c->plen, &c->pstart, &c->ghost_src,
                             &c->ghost_shift, &c->backup, &c->nbr, &c->nbr_cnt, &c->bsum, &c->u32s, &c->amr_list})
-    dfree(*a);
-  dfree(c->blist); dfree(c->bcnt);
-  dfree(c->stream_buf); dfree(c->u64s); dfree(c->step_counts); dfree(c->red_part); dfree(c->red_out);
+    dfree(c, *a);
+  dfree(c, c->blist); dfree(c, c->bcnt);
+  dfree(c, c->stream_buf); dfree(c, c->u64s); dfree(c, c->step_counts); dfree(c, c->red_part); dfree(c, c->red_out);
   if (c->h_u32) (void)hipHostFree(c->h_u32);
   if (c->h_u64) (void)hipHostFree(c->h_u64);
   if (c->st) (void)hipStreamDestroy(c->st);
@@ -1618,6 +1618,13 @@ int xnb_fp64_core(xnb_ctx* c, const double* x, size_t n, double* sqrt_out, doubl
   if (he == hipSuccess) he = hipStreamSynchronize(c->st);
   if (d) (void)hipFree(d);
   if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("fp64 core: ") + hipGetErrorString(he));
+  return XNB_OK;
+}
+
+int xnb_device_bytes(const xnb_ctx* c, size_t* current, size_t* peak) {
+  if (!c) return XNB_E_ARG;
+  if (current) *current = c->dev_bytes;
+  if (peak) *peak = c->dev_peak;
   return XNB_OK;
 }
 

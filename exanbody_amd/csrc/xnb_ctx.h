@@ -97,6 +97,7 @@ struct xnb_ctx {
   double brick_margin = 1.0;        // brick T / Rmax head room (batched boxes rebuild without re-choosing)
 
   double last_rcut = 0.;
+  size_t dev_bytes = 0, dev_peak = 0;  // device bytes the context holds now / at most so far
   std::string err;
 };
 
@@ -112,12 +113,22 @@ int fail(xnb_ctx* c, int code, const std::string& msg);
       return ::xnb_impl::fail((ctx), XNB_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));  \
   } while (0)
 
+// Device arrays of a context: every allocation and free is counted in c->dev_bytes (and its
+// running maximum c->dev_peak), the memory the serving loop's per-job estimate must bound.
+template <class T>
+void dfree(xnb_ctx* c, DArr<T>& a) {
+  if (a.p) {
+    (void)hipFree(a.p);  // hipFree waits for work still using the buffer
+    c->dev_bytes -= a.n * sizeof(T);
+  }
+  a.p = nullptr;
+  a.n = 0;
+}
+
 template <class T>
 int dalloc(xnb_ctx* c, DArr<T>& a, size_t n) {
   if (a.n >= n && a.p) return XNB_OK;
-  if (a.p) (void)hipFree(a.p);  // hipFree waits for work still using the buffer
-  a.p = nullptr;
-  a.n = 0;
+  dfree(c, a);
   size_t nn = n ? n : 1;
   hipError_t e = hipMalloc((void**)&a.p, nn * sizeof(T));
   if (e != hipSuccess) {
@@ -125,14 +136,9 @@ int dalloc(xnb_ctx* c, DArr<T>& a, size_t n) {
     return fail(c, XNB_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
   }
   a.n = nn;
+  c->dev_bytes += nn * sizeof(T);
+  if (c->dev_bytes > c->dev_peak) c->dev_peak = c->dev_bytes;
   return XNB_OK;
-}
-
-template <class T>
-void dfree(DArr<T>& a) {
-  if (a.p) (void)hipFree(a.p);
-  a.p = nullptr;
-  a.n = 0;
 }
 
 

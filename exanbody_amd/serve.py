@@ -16,7 +16,7 @@ class Job(C.Structure):
 
 class JobResult(C.Structure):
     _fields_ = [("start_s", C.c_double), ("finish_s", C.c_double), ("latency_s", C.c_double), ("status", C.c_int),
-                ("n_rebuilds", C.c_int64), ("n_violations", C.c_int64)]
+                ("n_rebuilds", C.c_int64), ("n_violations", C.c_int64), ("peak_device_bytes", C.c_size_t)]
 
 
 class ServeConfig(C.Structure):
@@ -28,7 +28,7 @@ class ServeStats(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("atom_steps", C.c_double), ("atom_steps_per_s", C.c_double),
                 ("p50_latency_s", C.c_double), ("p99_latency_s", C.c_double), ("max_latency_s", C.c_double),
                 ("mean_active", C.c_double), ("completed", C.c_int), ("failed", C.c_int), ("peak_active", C.c_int),
-                ("peak_bytes", C.c_size_t)]
+                ("peak_bytes", C.c_size_t), ("peak_device_bytes", C.c_size_t)]
 
 
 def _ptr(a):
@@ -59,6 +59,32 @@ def job_mix(n_jobs, rate, atoms=(1000, 200000), rho=(0.70, 0.90), temperature=(0
     return jobs
 
 
+def _job_struct(j, out=None):
+    """The xnb_job of a job dict, and the arrays it points into (keep them alive)."""
+    r = np.ascontiguousarray(j["r"], np.float64)
+    v = np.ascontiguousarray(j["v"], np.float64)
+    ids = np.ascontiguousarray(j["id"], np.uint64)
+    d = Domain()
+    dom = j["domain"]
+    for a in range(3):
+        d.bounds_min[a] = dom["bounds_min"][a]
+        d.bounds_max[a] = dom["bounds_max"][a]
+        d.grid_dims[a] = int(dom["grid_dims"][a])
+        d.periodic[a] = int(dom["periodic"][a])
+    d.cell_size = dom["cell_size"]
+    rp = RunParams(j["dt"], LJParams(j["epsilon"], j["sigma"]), j["rcut"], j["skin"], j["precision"], j["rebuild_every"],
+                   j["steps"], 0, 0)
+    job = Job(j["arrival"], r.shape[0], _ptr(r), _ptr(v), _ptr(ids), d, rp, _ptr(out["r"]) if out else None,
+              _ptr(out["v"]) if out else None, _ptr(out["id"]) if out else None)
+    return job, [r, v, ids]
+
+
+def job_bytes(j):
+    """Device bytes the loop budgets for a job dict (xnb_job_bytes_for)."""
+    job, keep = _job_struct(j)
+    return int(_lib.load().xnb_job_bytes_for(C.byref(job)))
+
+
 def serve(jobs, device=0, hbm_budget_bytes=0, max_active=0, steps_per_iter=1, outputs=False, autotune=False):
     """Run the loop over job dicts (arrival-ordered).  Returns (results, stats); with outputs,
     each result carries the final r, v, id in cell order."""
@@ -68,26 +94,12 @@ def serve(jobs, device=0, hbm_budget_bytes=0, max_active=0, steps_per_iter=1, ou
     keep = []
     outs = []
     for q, j in enumerate(jobs):
-        r = np.ascontiguousarray(j["r"], np.float64)
-        v = np.ascontiguousarray(j["v"], np.float64)
-        ids = np.ascontiguousarray(j["id"], np.uint64)
-        keep += [r, v, ids]
-        d = Domain()
-        dom = j["domain"]
-        for a in range(3):
-            d.bounds_min[a] = dom["bounds_min"][a]
-            d.bounds_max[a] = dom["bounds_max"][a]
-            d.grid_dims[a] = int(dom["grid_dims"][a])
-            d.periodic[a] = int(dom["periodic"][a])
-        d.cell_size = dom["cell_size"]
-        rp = RunParams(j["dt"], LJParams(j["epsilon"], j["sigma"]), j["rcut"], j["skin"], j["precision"], j["rebuild_every"],
-                       j["steps"], 0, 0)
         o = None
         if outputs:
-            o = dict(r=np.zeros_like(r), v=np.zeros_like(v), id=np.zeros_like(ids))
+            o = {k: np.zeros_like(np.ascontiguousarray(j[k], np.uint64 if k == "id" else np.float64)) for k in ("r", "v", "id")}
         outs.append(o)
-        arr[q] = Job(j["arrival"], r.shape[0], _ptr(r), _ptr(v), _ptr(ids), d, rp, _ptr(o["r"]) if o else None,
-                     _ptr(o["v"]) if o else None, _ptr(o["id"]) if o else None)
+        arr[q], held = _job_struct(j, o)
+        keep += held
     res = (JobResult * max(n, 1))()
     st = ServeStats()
     cfg = ServeConfig(int(device), int(hbm_budget_bytes), int(max_active), int(steps_per_iter), int(bool(autotune)))
@@ -98,7 +110,7 @@ def serve(jobs, device=0, hbm_budget_bytes=0, max_active=0, steps_per_iter=1, ou
     for q in range(n):
         x = res[q]
         d = dict(start=x.start_s, finish=x.finish_s, latency=x.latency_s, status=x.status, rebuilds=x.n_rebuilds,
-                 violations=x.n_violations)
+                 violations=x.n_violations, peak_device_bytes=x.peak_device_bytes)
         if outputs:
             d.update(outs[q])
         results.append(d)

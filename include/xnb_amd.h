@@ -220,6 +220,8 @@ int xnb_timer_read(xnb_ctx* ctx, int timer, double* total_ms, int64_t* launches)
 /* Pairs seen by the last force call: list entries and in-cutoff pairs. */
 int xnb_pair_counts(xnb_ctx* ctx, uint64_t* list_entries, uint64_t* cutoff_pairs);
 int xnb_synchronize(xnb_ctx* ctx);
+/* Device memory the context holds now and the most it has held at once (its own arrays). */
+int xnb_device_bytes(const xnb_ctx* ctx, size_t* current, size_t* peak);
 /* Device copy of `bytes` (read + write), iters times, timed with HIP events: achieved GB/s
  * counting read and write bytes.  Also the known-size kernel that calibrates HBM counters. */
 int xnb_copy_probe(xnb_ctx* ctx, size_t bytes, int iters, double* gbps);
@@ -283,9 +285,10 @@ int xnb_batch_detach(xnb_batch* batch, int slot, xnb_run_params* p);
 /* ---- serving loop ---------------------------------------------------------------- */
 /* No counterpart in the reference.  Continuous batching of MD jobs: jobs become visible at
  * their arrival time, are admitted in arrival order while the estimated device bytes of the
- * admitted jobs stay within the budget (and at most max_active run), advance together in one
- * xnb_batch, and leave when their steps are done; a job admitted while others run overlaps
- * with them on the device.  max_active = 1 serves one job at a time on the same path. */
+ * admitted jobs stay within the budget (and at most max_active run), are set up on a worker
+ * thread (context, binning, first forces on the job's own stream) while the batch keeps
+ * stepping, join the one xnb_batch when ready, and leave when their steps are done.
+ * max_active = 1 serves one job at a time on the same path. */
 typedef struct {
   double arrival_s;                    /* seconds after the loop starts */
   size_t n;                            /* atoms */
@@ -303,6 +306,7 @@ typedef struct {
   double start_s, finish_s, latency_s; /* admission, completion, completion - arrival */
   int status;                          /* XNB_OK or the error of the job */
   int64_t n_rebuilds, n_violations;
+  size_t peak_device_bytes;            /* most device memory the job's context held at once */
 } xnb_job_result;
 
 typedef struct {
@@ -317,10 +321,19 @@ typedef struct {
   double wall_s, atom_steps, atom_steps_per_s;
   double p50_latency_s, p99_latency_s, max_latency_s, mean_active;
   int completed, failed, peak_active;
-  size_t peak_bytes;
+  size_t peak_bytes;                   /* most budgeted bytes (xnb_job_bytes_for) of admitted jobs at once */
+  size_t peak_device_bytes;            /* most device bytes held by the contexts in the batch, summed
+                                        * (sampled after each batch step) */
 } xnb_serve_stats;
 
-/* Device bytes the loop budgets for a job of n atoms. */
+/* Device bytes the loop budgets for a job: an upper bound of what the job's context
+ * allocates while it runs in the loop with the default batch margins (checked by the tests
+ * against the contexts' counted allocations, xnb_job_result.peak_device_bytes).  It grows
+ * with the atom and periodic-ghost slots, with the neighbours an atom has within
+ * rcut + rcut_inc at the job's mean density, and with the cells. */
+size_t xnb_job_bytes_for(const xnb_job* job);
+/* The same for n atoms with ghosts at most 0.4 n, at most 110 neighbours each within
+ * rcut + rcut_inc (density 1.2 at rcut + rcut_inc = 2.8) and at most n cells. */
 size_t xnb_job_bytes(size_t n);
 int xnb_serve_run(const xnb_job* jobs, int njobs, const xnb_serve_config* cfg, xnb_job_result* results,
                   xnb_serve_stats* stats);

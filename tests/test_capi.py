@@ -136,3 +136,26 @@ def test_header_compiles_as_c_and_ctypes_layouts_match(tmp_path):
                 if m:
                     names.append(m.group(1))
         assert names == [f[0] for f in py._fields_], f"{cname}: {names}"
+
+
+def test_job_bytes_estimate():
+    """xnb_job_bytes_for grows with the atoms, with the neighbours per atom and with the ghost
+    share of a small periodic box; xnb_job_bytes(n) is the value at 0.4 n ghosts, 110
+    neighbours and n cells."""
+    from exanbody_amd.serve import job_bytes, make_job
+    L = _lib.load()
+    a = make_job(4000, 0.6, 1.0, 10, _lib.PRECISION_FP64, seed=1)
+    b = make_job(4000, 1.2, 1.0, 10, _lib.PRECISION_FP64, seed=1)
+    c = make_job(8000, 1.2, 1.0, 10, _lib.PRECISION_FP64, seed=1)
+    assert job_bytes(a) < job_bytes(b) < job_bytes(c)
+    dom = b["domain"]
+    D = np.asarray(dom["grid_dims"], float)
+    vol = np.prod(np.asarray(dom["bounds_max"]) - np.asarray(dom["bounds_min"]))
+    nb = 4000 / vol * 4.0 / 3.0 * np.pi * 2.8 ** 3
+    ghost = np.prod((D + 2) / D)
+    want = int(4000 * ghost * 500 + 4000 * 4 * nb + np.prod(D + 2) * 64) + (16 << 20)
+    assert abs(job_bytes(b) - want) <= 2  # last-bit rounding
+    # a non-periodic box has no ghost slots
+    d = dict(b, domain=dict(dom, periodic=(0, 0, 0)))
+    assert job_bytes(d) < job_bytes(b)
+    assert L.xnb_job_bytes(4000) == int(1.4 * 4000 * 500 + 4000 * 4 * 110.0 + 4000 * 64) + (16 << 20)
