@@ -52,6 +52,14 @@ struct BatchBox {
 constexpr double kBrickMargin = 1.12;
 constexpr double kSlotMargin = 1.3;    // slot capacity head room for ghosts
 
+// One launch of brick work: its table range, workgroup lanes and dynamic LDS.
+struct BrickLaunch {
+  int kind = kBatchNbh;  // which batched brick kernel
+  size_t o = 0, n = 0;
+  uint32_t T = 64, lds = 0;
+  bool operator==(const BrickLaunch& b) const { return kind == b.kind && o == b.o && n == b.n && T == b.T && lds == b.lds; }
+};
+
 }  // namespace
 
 struct xnb_batch {
@@ -74,16 +82,24 @@ struct xnb_batch {
   DArr<uint32_t> dclock;  // {par, gstep} on the device, read by the quiet-step kernels
   DArr<uint32_t> amr_list, amr_count;  // rebuild_amr: cells of rebuilding boxes to sort, their number
   uint32_t lds_limit = 65536;
+  BatchKernelInfo kinfo[3] = {};  // per batched brick kernel (kBatchLjFp64, kBatchLjMixed, kBatchNbh)
   // HIP graph of a quiet step (no box fresh, rebuilt on displacement or rebuilding): ghost
   // refresh, force, clock advance, replayed while the step's work table stays the same
   bool use_graph = true;
   // head room of attached boxes (XNB_BATCH_MARGINS="brick,slot" at create; 1,1 leaves none, so
   // most rebuilds that grow a brick or the ghosts take the one-box recovery)
   double brick_margin = kBrickMargin, slot_margin = kSlotMargin;
+  // LDS bytes a box's brick workgroup may take: half the CU's by default, so that two brick
+  // workgroups share a CU (512-box batch, 100 steps: 3.15e9 -> 3.54e9 atom-steps/s against no
+  // cap; a third of the CU is slower again); XNB_BATCH_LDS="fraction" at create
+  uint32_t brick_lds_cap = 0;
+  // brick work launched per class of boxes (XNB_BATCH_CLASSES=0: one launch sized for the
+  // widest box)
+  bool use_classes = true;
   hipGraphExec_t gexec = nullptr;
   std::vector<uint32_t> gtab, last_quiet;  // table of the captured graph / of the last quiet step
-  size_t g_og = 0, g_ng = 0, g_of = 0, g_nf = 0;
-  uint32_t g_Tf = 0, g_lds = 0;
+  size_t g_og = 0, g_ng = 0;
+  std::vector<BrickLaunch> g_lf;
   DArr<uint32_t> dgwork[2];  // device copies of graph tables (the current graph's and the next)
   int gcur = 0;
   std::vector<hipGraphExec_t> retired;  // destroyed after the next stream synchronisation
@@ -216,6 +232,49 @@ std::vector<uint32_t> bricks_table(const std::vector<int>& boxes, const std::vec
   return t;
 }
 
+uint32_t brick_lds_bytes(BrickParams bp) { return brick_layout(bp); }
+
+// Workgroups of a box's bricks that fit one CU at once (LDS bytes and the waves the
+// kernel's registers allow), 1..4.
+uint32_t brick_class(uint32_t T, uint32_t lds, uint32_t cu_lds, uint32_t waves_per_cu) {
+  const uint32_t by_lds = lds ? cu_lds / lds : 4u;
+  const uint32_t by_waves = waves_per_cu / ((T + 63u) / 64u);
+  return std::max(1u, std::min({by_lds, by_waves, 4u}));
+}
+
+// Bricks of the listed boxes for the batched kernel `kind` (for the force: the boxes of that
+// precision), one launch per class of boxes whose workgroups fit the same number of times on
+// a CU, each sized for its own widest box, so that boxes with small bricks do not run in the
+// lanes and LDS of the largest one.  Results do not depend on the class: a brick's lanes
+// beyond its box's T take no atoms.
+void brick_launches(const xnb_batch* B, Tables& T, int kind, const std::vector<int>& boxes, const std::vector<uint32_t>& tag,
+                    std::vector<BrickLaunch>& out) {
+  const BatchKernelInfo& ki = B->kinfo[kind];
+  for (uint32_t k = 1; k <= 4; k++) {
+    std::vector<int> sub;
+    std::vector<uint32_t> nbr, tg;
+    BrickLaunch L;
+    L.kind = kind;
+    for (size_t q = 0; q < boxes.size(); q++) {
+      const BatchBox& bb = B->box[boxes[q]];
+      if (kind != kBatchNbh && (kind == kBatchLjMixed) != (bb.f.mixed != 0)) continue;
+      const uint32_t lds = brick_lds_bytes(bb.f.bp);
+      if (B->use_classes && brick_class(bb.f.bp.T, lds + ki.static_lds, B->lds_limit, ki.waves_per_cu) != k) continue;
+      sub.push_back(boxes[q]);
+      nbr.push_back(bb.f.bp.nbricks);
+      tg.push_back(tag[q]);
+      L.T = std::max(L.T, bb.f.bp.T);
+      L.lds = std::max(L.lds, lds);
+    }
+    if (!sub.empty()) {
+      L.o = T.add(bricks_table(sub, nbr, tg));
+      L.n = T.v.size() - L.o;
+      out.push_back(L);
+    }
+    if (!B->use_classes) break;
+  }
+}
+
 // pinned staging (two buffers) and B->dwork hold at least n words
 int ensure_staging(xnb_batch* B, size_t n) {
   if (n > B->hwork_cap) {
@@ -263,7 +322,7 @@ void destroy_retired(xnb_batch* B) {  // after a stream synchronisation
 // Capture the quiet step of table T2 (force table at o_f, ghost refresh at o_g) into a new
 // graph.  Its table goes to the graph buffer not used by the current graph: graphs launched
 // earlier that read that buffer run before the copy (stream order).
-int capture_quiet(xnb_batch* B, const Tables& T2, size_t o_g, size_t n_g, size_t o_f, size_t n_f, uint32_t Tf, uint32_t lds) {
+int capture_quiet(xnb_batch* B, const Tables& T2, size_t o_g, size_t n_g, const std::vector<BrickLaunch>& lf) {
   const int j = B->gcur ^ 1;
   const size_t n = T2.v.size();
   int e;
@@ -278,11 +337,19 @@ int capture_quiet(xnb_batch* B, const Tables& T2, size_t o_g, size_t n_g, size_t
     B->dgwork[j].n = B->hwork_cap;
   }
   if ((e = stage_copy(B, T2, B->dgwork[j].p))) return e;
-  BHIP(B, batch_lj_set_lds(lds));  // attribute set outside the capture
+  for (int kind : {kBatchLjFp64, kBatchLjMixed}) {  // attributes set outside the capture
+    uint32_t lds = 0;
+    for (const BrickLaunch& L : lf)
+      if (L.kind == kind) lds = std::max(lds, L.lds);
+    if (lds) BHIP(B, batch_set_lds(kind, lds));
+  }
   const uint32_t* W = B->dgwork[j].p;
   BHIP(B, hipStreamBeginCapture(B->st, hipStreamCaptureModeRelaxed));
   launch_batch_ghost_update_r(B->dF.p, W + o_g, (uint32_t)n_g, B->dclock.p, B->st);
-  const hipError_t le = launch_batch_lj(B->dF.p, W + o_f, (uint32_t)n_f, Tf, lds, B->dclock.p, B->st, false);
+  hipError_t le = hipSuccess;
+  for (const BrickLaunch& L : lf)
+    if (le == hipSuccess)
+      le = launch_batch_lj(L.kind == kBatchLjMixed, B->dF.p, W + L.o, (uint32_t)L.n, L.T, L.lds, B->dclock.p, B->st, false);
   launch_batch_advance(B->dclock.p, B->st);
   hipGraph_t g = nullptr;
   const hipError_t ce = hipStreamEndCapture(B->st, &g);
@@ -300,17 +367,32 @@ int capture_quiet(xnb_batch* B, const Tables& T2, size_t o_g, size_t n_g, size_t
   B->gtab = T2.v;
   B->g_og = o_g;
   B->g_ng = n_g;
-  B->g_of = o_f;
-  B->g_nf = n_f;
-  B->g_Tf = Tf;
-  B->g_lds = lds;
+  B->g_lf = lf;
   B->n_captures++;
   return XNB_OK;
 }
 
-uint32_t brick_lds_bytes(BrickParams bp) { return brick_layout(bp); }
-
 bool active(const BatchBox& bb) { return bb.used && !bb.finished && !bb.failed; }
+
+// chunk_neighbors of a box for the batch: brick head room and the batch's LDS cap per brick
+// workgroup, or the context's own limit when no brick fits the cap (restored afterwards)
+int batch_box_nbh(xnb_batch* B, xnb_ctx* c) {
+  c->brick_margin = B->brick_margin;
+  const uint32_t keep = c->lds_limit;
+  xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
+  int e = XNB_OK;
+  bool done = false;
+  c->batch_shapes = true;
+  if (B->brick_lds_cap < keep) {
+    c->lds_limit = B->brick_lds_cap;
+    e = op_nbh(c, c->nbh_dist_lab, cfg, 0);
+    c->lds_limit = keep;
+    done = e || c->list_kind == 1;
+  }
+  if (!done) e = op_nbh(c, c->nbh_dist_lab, cfg, 0);
+  c->batch_shapes = false;
+  return e;
+}
 
 // One-box recovery after a batched rebuild overflowed the box's frame sizes: ghosts and
 // lists again with the one-box operators (which re-size), then a new frame.
@@ -322,8 +404,7 @@ int recover_box(xnb_batch* B, int i, const xnb_run_params& p) {
   c->brick_margin = B->brick_margin;
   if ((e = ensure_capacity(c, (size_t)std::ceil((c->N + c->G) * B->slot_margin) + 256, c->N))) return e;
   if ((e = op_ghost_all(c, false, false))) return e;  // f and ep of the run's boxes are not carried
-  xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
-  if ((e = op_nbh(c, c->nbh_dist_lab, cfg, 0))) return e;
+  if ((e = batch_box_nbh(B, c))) return e;
   if (c->list_kind != 1) return fail(c, XNB_E_STATE, "box no longer fits the brick path");
   if (c->bp.nbricks > kWorkBrickMask + 1) return fail(c, XNB_E_CAPACITY, "box has too many bricks for a batch");
   BHIP(B, hipStreamSynchronize(c->st));
@@ -347,6 +428,7 @@ int xnb_batch_create(xnb_batch** out, int device) {
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
     B->lds_limit = (uint32_t)lds;
+  for (int k = 0; k < 3; k++) B->kinfo[k] = batch_kernel_info(k);
   bool ok = hipStreamCreate(&B->st) == hipSuccess;
   ok = ok && hipMalloc((void**)&B->dF.p, kMaxBatchBoxes * sizeof(BoxFrame)) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->hF, kMaxBatchBoxes * sizeof(BoxFrame), hipHostMallocDefault) == hipSuccess;
@@ -363,6 +445,15 @@ int xnb_batch_create(xnb_batch** out, int device) {
     ok = hipMemcpy(B->dclock.p, clock0, sizeof(clock0), hipMemcpyHostToDevice) == hipSuccess;
   }
   if (const char* s = std::getenv("XNB_BATCH_GRAPH")) B->use_graph = std::atoi(s) != 0;
+  if (const char* s = std::getenv("XNB_BATCH_CLASSES")) B->use_classes = std::atoi(s) != 0;
+  double lds_frac = 0.5;
+  if (const char* s = std::getenv("XNB_BATCH_LDS")) {
+    const double f = std::atof(s);
+    if (f > 0.0 && f <= 1.0) lds_frac = f;
+  }
+  uint32_t st_lds = 0;
+  for (int k = 0; k < 3; k++) st_lds = std::max(st_lds, B->kinfo[k].static_lds);
+  B->brick_lds_cap = (uint32_t)(lds_frac * B->lds_limit) - std::min<uint32_t>(st_lds + 64u, B->lds_limit / 8);
   if (const char* s = std::getenv("XNB_BATCH_MARGINS")) {
     double bm = 0, sm = 0;
     if (std::sscanf(s, "%lf,%lf", &bm, &sm) != 2 || !(bm >= 1.0 && bm <= 4.0) || !(sm >= 1.0 && sm <= 4.0)) {
@@ -432,10 +523,8 @@ int xnb_batch_attach(xnb_batch* B, xnb_ctx* c, const xnb_run_params* p, int* slo
     }
   if (i < 0) return bfail(B, XNB_E_CAPACITY, "batch is full");
   // head room: ghost slots, and bricks / lists rebuilt with margins
-  c->brick_margin = B->brick_margin;
   if ((e = ensure_capacity(c, (size_t)std::ceil((c->N + c->G) * B->slot_margin) + 256, (size_t)c->N + c->G))) return e;
-  xnb_chunk_neighbors_config cfg{1, 0, 0, 0, 0};
-  if ((e = op_nbh(c, c->nbh_dist_lab, cfg, 0))) return e;
+  if ((e = batch_box_nbh(B, c))) return e;
   if (c->list_kind != 1) return bfail(B, XNB_E_STATE, "box geometry is not on the brick path (cell edge < nbh_dist)");
   if (brick_lds_bytes(c->bp) > B->lds_limit) return bfail(B, XNB_E_CAPACITY, "box bricks exceed LDS");
   // force-table entries hold the brick index in kWorkEpiShift bits
@@ -507,18 +596,15 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
       else quiet.push_back(i);
     }
     Tables T1;
-    std::vector<uint32_t> nN, nC, nGh, nBr;
-    size_t o_rs = 0, n_rs = 0, o_rc = 0, n_rc = 0, o_rg = 0, n_rg = 0, o_rb = 0, n_rb = 0, o_list = 0;
-    uint32_t Treb = 64, ldsreb = 0;
+    std::vector<uint32_t> nN, nC, nGh;
+    size_t o_rs = 0, n_rs = 0, o_rc = 0, n_rc = 0, o_rg = 0, n_rg = 0, o_list = 0;
+    std::vector<BrickLaunch> lb;
     if (!reb.empty()) {
       nN.clear();
       for (int i : reb) {
         nN.push_back(B->box[i].f.N);
         nC.push_back(B->box[i].f.g.ncells);
         nGh.push_back(ghost_cell_count(B->box[i].f.g) * 64u);  // one wave per ghost cell
-        nBr.push_back(B->box[i].f.bp.nbricks);
-        Treb = std::max(Treb, B->box[i].f.bp.T);
-        ldsreb = std::max(ldsreb, brick_lds_bytes(B->box[i].f.bp));
       }
       o_rs = T1.add(blocks_table(reb, nN));
       n_rs = T1.v.size() - o_rs;
@@ -526,9 +612,7 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
       n_rc = T1.v.size() - o_rc;
       o_rg = T1.add(blocks_table(reb, nGh));
       n_rg = T1.v.size() - o_rg;
-      std::vector<uint32_t> zero(reb.size(), 0u);
-      o_rb = T1.add(bricks_table(reb, nBr, zero));
-      n_rb = T1.v.size() - o_rb;
+      brick_launches(B, T1, kBatchNbh, reb, std::vector<uint32_t>(reb.size(), 0u), lb);
       std::vector<uint32_t> lst(reb.begin(), reb.end());
       o_list = T1.add(lst);
     }
@@ -555,8 +639,10 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
       BHIP(B, hipMemsetAsync(B->amr_count.p, 0, sizeof(uint32_t), B->st));
       launch_batch_ghosts(B->dF.p, W + o_rs, (uint32_t)n_rs, W + o_rc, (uint32_t)n_rc, W + o_rg, (uint32_t)n_rg, W + o_list,
                           (uint32_t)reb.size(), B->amr_list.p, B->amr_count.p, par, B->st);
-      hipError_t he = launch_batch_nbh(B->dF.p, W + o_rb, (uint32_t)n_rb, Treb, ldsreb, par, B->st);
-      if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched neighbour build: ") + hipGetErrorString(he));
+      for (const BrickLaunch& L : lb) {
+        hipError_t he = launch_batch_nbh(B->dF.p, W + L.o, (uint32_t)L.n, L.T, L.lds, par, B->st);
+        if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched neighbour build: ") + hipGetErrorString(he));
+      }
       BHIP(B, hipMemcpyAsync(B->hwords, B->dwords.p, 4 * kMaxBatchBoxes * sizeof(uint32_t), hipMemcpyDeviceToHost, B->st));
       BHIP(B, hipStreamSynchronize(B->st));
       destroy_retired(B);
@@ -606,8 +692,7 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
     const size_t o_g = T2.add(blocks_table(q2, nG));
     const size_t n_g = T2.v.size() - o_g;
     std::vector<int> frc;
-    std::vector<uint32_t> nbr, tag;
-    uint32_t Tf = 64, ldsf = 0;
+    std::vector<uint32_t> tag;
     for (int i : act) {
       BatchBox& bb = B->box[i];
       if (bb.failed) continue;
@@ -615,22 +700,18 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
       // a fused step stores no forces or energies: only the last step's are read (see run_loop)
       const bool fuse = k < bb.steps;
       frc.push_back(i);
-      nbr.push_back(bb.f.bp.nbricks);
       tag.push_back(fuse ? (uint32_t)EPI_NEXT : (uint32_t)EPI_KICK);
-      Tf = std::max(Tf, bb.f.bp.T);
-      ldsf = std::max(ldsf, brick_lds_bytes(bb.f.bp));
     }
-    const size_t o_f = T2.add(bricks_table(frc, nbr, tag));
-    const size_t n_f = T2.v.size() - o_f;
+    std::vector<BrickLaunch> lf;
+    for (int kind : {kBatchLjMixed, kBatchLjFp64}) brick_launches(B, T2, kind, frc, tag, lf);
     const bool quiet_step = fresh.empty() && disp.empty() && reb.empty() && !T2.v.empty();
     bool graphed = false;
     if (quiet_step && B->use_graph) {
       auto same = [&]() {
-        return B->gexec && T2.v == B->gtab && o_g == B->g_og && n_g == B->g_ng && o_f == B->g_of && n_f == B->g_nf &&
-               Tf == B->g_Tf && ldsf == B->g_lds;
+        return B->gexec && T2.v == B->gtab && o_g == B->g_og && n_g == B->g_ng && lf == B->g_lf;
       };
       // capture a table seen on two quiet steps in a row (not one-off tables such as a box's last step)
-      if (!same() && T2.v == B->last_quiet && (e = capture_quiet(B, T2, o_g, n_g, o_f, n_f, Tf, ldsf))) return e;
+      if (!same() && T2.v == B->last_quiet && (e = capture_quiet(B, T2, o_g, n_g, lf))) return e;
       if (same()) {
         BHIP(B, hipGraphLaunch(B->gexec, B->st));
         graphed = true;
@@ -648,8 +729,10 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
       if ((e = upload_tables(B, T2))) return e;
       W = B->dwork.p;
       launch_batch_ghost_update_r(B->dF.p, W + o_g, (uint32_t)n_g, B->dclock.p, B->st);
-      hipError_t he = launch_batch_lj(B->dF.p, W + o_f, (uint32_t)n_f, Tf, ldsf, B->dclock.p, B->st);
-      if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched force: ") + hipGetErrorString(he));
+      for (const BrickLaunch& L : lf) {
+        hipError_t he = launch_batch_lj(L.kind == kBatchLjMixed, B->dF.p, W + L.o, (uint32_t)L.n, L.T, L.lds, B->dclock.p, B->st);
+        if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched force: ") + hipGetErrorString(he));
+      }
       launch_batch_advance(B->dclock.p, B->st);
     }
     for (size_t q = 0; q < frc.size(); q++) {

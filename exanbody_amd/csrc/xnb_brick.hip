@@ -1105,8 +1105,10 @@ __device__ __forceinline__ void lj_batch_case(const BoxFrame& f, uint32_t b, uin
       f.g, f.bp, b, A.r, f.cell_off, f.cell_n, f.blist, f.bcnt, LennardJonesForceFunctor<MIXED>{f.lj}, f.lj.rc2, A, f.dth, nx);
 }
 
-// Force of every brick of every box; precision and epilogue are per box (uniform per
-// workgroup).  EPI_KICK closes a box's run; EPI_NEXT fuses its next step.
+// Force of every brick of the listed boxes, all of one precision (one kernel per precision,
+// so the mixed form runs with its own, smaller register budget); the epilogue is per box
+// (uniform per workgroup).  EPI_KICK closes a box's run; EPI_NEXT fuses its next step.
+template <bool MIXED>
 __global__ void __launch_bounds__(1024) kb_lj_brick(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
                                                     const uint32_t* __restrict__ clock) {
   uint32_t box, b;
@@ -1115,13 +1117,15 @@ __global__ void __launch_bounds__(1024) kb_lj_brick(const BoxFrame* __restrict__
   const uint32_t epi = work_entry_epi(b);
   b = work_entry_brick(b);
   const BoxFrame& f = F[box];
-  const uint32_t key = (f.mixed ? 4u : 0u) | epi;
-  switch (key) {
-    case EPI_KICK: lj_batch_case<false, EPI_KICK>(f, b, par, gstep); break;
-    case EPI_NEXT: lj_batch_case<false, EPI_NEXT>(f, b, par, gstep); break;
-    case 4u | EPI_KICK: lj_batch_case<true, EPI_KICK>(f, b, par, gstep); break;
-    case 4u | EPI_NEXT: lj_batch_case<true, EPI_NEXT>(f, b, par, gstep); break;
-    default: break;
+  if (epi == EPI_NEXT) lj_batch_case<MIXED, EPI_NEXT>(f, b, par, gstep);
+  else if (epi == EPI_KICK) lj_batch_case<MIXED, EPI_KICK>(f, b, par, gstep);
+}
+
+static const void* batch_kernel(int kind) {
+  switch (kind) {
+    case kBatchLjFp64: return (const void*)kb_lj_brick<false>;
+    case kBatchLjMixed: return (const void*)kb_lj_brick<true>;
+    default: return (const void*)kb_nbh_brick;
   }
 }
 
@@ -1134,16 +1138,32 @@ hipError_t launch_batch_nbh(const BoxFrame* F, const uint32_t* work, uint32_t nw
   return hipGetLastError();
 }
 
-hipError_t batch_lj_set_lds(uint32_t lds) { return set_lds((const void*)kb_lj_brick, lds); }
+hipError_t batch_set_lds(int kind, uint32_t lds) { return set_lds(batch_kernel(kind), lds); }
 
-hipError_t launch_batch_lj(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds, const uint32_t* clock,
-                           hipStream_t st, bool set) {
+// Static LDS (wave counters) and the waves per CU its registers allow, for counting how
+// many workgroups of a batched brick kernel fit one CU.
+BatchKernelInfo batch_kernel_info(int kind) {
+  BatchKernelInfo info{256u, 16u};
+  hipFuncAttributes a{};
+  if (hipFuncGetAttributes(&a, batch_kernel(kind)) == hipSuccess) {
+    info.static_lds = (uint32_t)a.sharedSizeBytes;
+    if (a.numRegs > 0) {
+      const uint32_t vg = ((uint32_t)a.numRegs + 7u) & ~7u;  // allocation granule of 8 VGPRs
+      info.waves_per_cu = 4u * std::min(8u, std::max(1u, 512u / vg));
+    }
+  }
+  return info;
+}
+
+hipError_t launch_batch_lj(bool mixed, const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds,
+                           const uint32_t* clock, hipStream_t st, bool set) {
   if (!nwork) return hipSuccess;
   if (set) {
-    hipError_t e = set_lds((const void*)kb_lj_brick, lds);
+    hipError_t e = batch_set_lds(mixed ? kBatchLjMixed : kBatchLjFp64, lds);
     if (e != hipSuccess) return e;
   }
-  kb_lj_brick<<<nwork, T, lds, st>>>(F, work, clock);
+  if (mixed) kb_lj_brick<true><<<nwork, T, lds, st>>>(F, work, clock);
+  else kb_lj_brick<false><<<nwork, T, lds, st>>>(F, work, clock);
   return hipGetLastError();
 }
 

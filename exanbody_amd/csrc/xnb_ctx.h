@@ -94,6 +94,7 @@ struct xnb_ctx {
   bool slot_lists_current = false;  // nbr/nbr_cnt mirror the brick lists
   bool fuse_push = true;            // force kernel performs the next step's first half (brick lists)
   uint32_t lds_limit = 65536;
+  bool batch_shapes = false;  // brick shapes offered to batched boxes only
   double brick_margin = 1.0;        // brick T / Rmax head room (batched boxes rebuild without re-choosing)
 
   double last_rcut = 0.;

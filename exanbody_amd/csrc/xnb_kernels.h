@@ -131,10 +131,17 @@ void launch_batch_first_half(const BoxFrame* F, const uint32_t* work, uint32_t n
 void launch_batch_trigger(const BoxFrame* F, const uint32_t* boxes, uint32_t n, uint32_t gstep, uint32_t* out, hipStream_t st);
 hipError_t launch_batch_nbh(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds, uint32_t par,
                             hipStream_t st);
+// batched brick kernels: the force per precision and the neighbour build
+enum { kBatchLjFp64 = 0, kBatchLjMixed = 1, kBatchNbh = 2 };
+struct BatchKernelInfo {
+  uint32_t static_lds;    // bytes of static LDS
+  uint32_t waves_per_cu;  // waves one CU holds at the kernel's register use
+};
+BatchKernelInfo batch_kernel_info(int kind);
 // set_lds false: the workgroup LDS attribute was already raised to lds (launches under capture)
-hipError_t launch_batch_lj(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds, const uint32_t* clock,
-                           hipStream_t st, bool set_lds = true);
-hipError_t batch_lj_set_lds(uint32_t lds);
+hipError_t launch_batch_lj(bool mixed, const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds,
+                           const uint32_t* clock, hipStream_t st, bool set_lds = true);
+hipError_t batch_set_lds(int kind, uint32_t lds);
 constexpr uint32_t kBatchSlotBlock = 256;  // threads per workgroup of the per-slot / per-cell batched kernels
 
 void scan_exclusive(const uint32_t* in, uint32_t n, uint32_t add, uint32_t* out, uint32_t* bsum, uint32_t* total, hipStream_t st);

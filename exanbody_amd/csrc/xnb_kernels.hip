@@ -206,7 +206,11 @@ __device__ inline void copy_particle(const PState& A, uint32_t s, const PState& 
     B.ep[t] = A.ep[s];
   }
   B.id[t] = A.id[s];
-  for (uint32_t k = 0; k < A.nx; k++) B.x[k][t] = A.x[k][s];
+  // constant indices keep the pointer arrays in registers (a runtime index would move the
+  // whole state to scratch)
+#pragma unroll
+  for (uint32_t k = 0; k < kMaxExtraFields; k++)
+    if (k < A.nx) B.x[k][t] = A.x[k][s];
 }
 
 __device__ inline void move_scatter_item(uint32_t N, PState A, PState B, const uint32_t* __restrict__ slot_cell,
@@ -497,7 +501,9 @@ __device__ inline void ghost_fill_wave(const Geom& g, uint32_t N, const PState& 
             B.fx[t] = 0.; B.fy[t] = 0.; B.fz[t] = 0.; B.ep[t] = 0.;
           }
           B.id[t] = B.id[s];
-          for (uint32_t k = 0; k < B.nx; k++) B.x[k][t] = B.x[k][s];
+#pragma unroll
+          for (uint32_t k = 0; k < kMaxExtraFields; k++)
+            if (k < B.nx) B.x[k][t] = B.x[k][s];
           slot_cell[t] = c;
           ghost_src[t - N] = s;
           ghost_shift[t - N] = code;
