@@ -271,9 +271,11 @@ def batched_bench(n_boxes, steps, alone_sample, drift_sample, seed=5):
     left = len(slots)
     while left:
         left = batch.step(steps)
+    recoveries = sum(batch.recoveries(s) for s in slots)
     for s in slots:
         batch.detach(s)  # synchronises
     wall = time.perf_counter() - t0
+    graph_steps, captures = batch.graph_stats()
     atoms = np.array([j["r"].shape[0] for j in jobs])
     # mixed vs FP64 force on every box's final positions
     f64s, fmxs = [], []
@@ -302,6 +304,7 @@ def batched_bench(n_boxes, steps, alone_sample, drift_sample, seed=5):
     return dict(workload=f"{n_boxes} periodic LJ boxes, atoms log-uniform [500, 50000] (total {int(atoms.sum())}), "
                          f"rho* U[0.70, 0.90], T* U[0.5, 1.5], {steps} steps each, mixed, rebuild every {EVERY}",
                 atom_steps_per_s=batched, ms_per_step=1e3 * wall / steps, wall_s=wall, setup_s=setup,
+                recoveries=int(recoveries), graph_steps=int(graph_steps), graph_captures=int(captures),
                 one_at_a_time_atom_steps_per_s=alone, one_at_a_time_sample=int(len(pick)),
                 speedup_vs_one_at_a_time=batched / alone,
                 force_error_end=dict(err, note="mixed vs FP64 force on every box's final positions (all boxes pooled)"),

@@ -47,9 +47,11 @@ struct BatchBox {
   std::string err;
 };
 
-// brick T / Rmax head room of attached boxes.  512-box batch, 500 steps: 1.25 -> 1.12 gives
-// +4 % throughput for 9 recoveries instead of 1; 1.0 recovers thousands of times.
-constexpr double kBrickMargin = 1.12;
+// brick T / Rmax head room of attached boxes; under the batch's LDS cap it also decides
+// which brick shape a box gets.  512-box batch, 300 steps, atom-steps/s: 1.06 3.83e9, 1.12
+// 3.92e9, 1.15 4.27e9, 1.18 4.21e9, 1.21 3.94e9, 1.35 3.88e9, 1.5 3.45e9; 1.0 recovers
+// thousands of times.
+constexpr double kBrickMargin = 1.15;
 constexpr double kSlotMargin = 1.3;    // slot capacity head room for ghosts
 
 // One launch of brick work: its table range, workgroup lanes and dynamic LDS.
