@@ -14,13 +14,18 @@
 // forces are summed in that order.
 //
 // Requires cell_layers == 1 (cell edge >= nbh_dist) and ghost layers >= 1.
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
 #include "xnb_kernels.h"
 
 namespace xnb {
 
-// inclusive-exclusive scan across the whole workgroup (blockDim.x multiple of 64, <= 1024)
-__device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total) {
-  __shared__ uint32_t wsum[16];
+// inclusive-exclusive scan across the whole workgroup (blockDim.x multiple of 64, <= 1024);
+// wsum: 16 words of LDS.  The brick kernels keep every LDS word in their dynamic block, so
+// it starts at address 0 and a staged atom's address is one multiply of its index.
+__device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t* wsum) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   uint32_t x = v;
 #pragma unroll
@@ -69,6 +74,8 @@ struct BrickLds {
   uint16_t* crow;   // per 64 region atoms: the row holding the first of them
   double *px, *py, *pz;
   uint8_t* pt;  // particle type (r.w) of each region atom
+  uint32_t* wsum;  // scan scratch (16 words)
+  uint32_t* wcnt;  // wave_group_count_add's two words
 };
 
 __device__ inline BrickLds brick_lds(const BrickParams& bp, char* lds) {
@@ -83,6 +90,8 @@ __device__ inline BrickLds brick_lds(const BrickParams& bp, char* lds) {
   L.py = L.px + 1;
   L.pz = L.px + 2;
   L.pt = (uint8_t*)(lds + bp.off_type);
+  L.wsum = (uint32_t*)(lds + bp.off_wsum);
+  L.wcnt = L.wsum + 16;
   return L;
 }
 
@@ -158,7 +167,7 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
     L.roff[t] = off;
   }
   uint32_t tot;
-  const uint32_t ex = block_scan_excl(cnt, &tot);
+  const uint32_t ex = block_scan_excl(cnt, &tot, L.wsum);
   if (t < (uint32_t)bp.nrc) L.rpref[t] = ex;
   if (t == 0) L.rpref[bp.nrc] = tot;
   __syncthreads();
@@ -169,7 +178,7 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
               ck = (int)(t / ((uint32_t)bp.Bi * (uint32_t)bp.Bj));
     if (ci < P.ni && cj < P.nj && ck < P.nk) bcnt = L.rcnt[((ck + 1) * bp.RY + cj + 1) * bp.RX + ci + 1];
   }
-  const uint32_t bex = block_scan_excl(bcnt, &tot);
+  const uint32_t bex = block_scan_excl(bcnt, &tot, L.wsum);
   if (t < (uint32_t)bp.nbc) L.bpref[t] = bex;
   if (t == 0) L.bpref[bp.nbc] = tot;
   __syncthreads();
@@ -506,8 +515,8 @@ __device__ __forceinline__ void compute_pair_brick_body(const Geom& g, const Bri
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
   if (EPI == EPI_NEXT && threadIdx.x == 0) {  // ordered before every wave's count by brick_setup's barriers
-    wave_group_counter()[0] = 0;
-    wave_group_counter()[1] = 0;
+    L.wcnt[0] = 0;
+    L.wcnt[1] = 0;
   }
   const uint32_t t = threadIdx.x, T = bp.T;
   // the list length is issued before the staging so its latency overlaps it (the build wrote
@@ -559,6 +568,11 @@ __device__ __forceinline__ void compute_pair_brick_body(const Geom& g, const Bri
       nb.j[4 * w + 1] = wc[w].x >> 16;
       nb.j[4 * w + 2] = wc[w].y & 0xffffu;
       nb.j[4 * w + 3] = wc[w].y >> 16;
+      // 24-bit multiplies of the 16-bit halves (one instruction each with a word select)
+      nb.off[4 * w + 0] = __umul24(wc[w].x & 0xffffu, 24u);
+      nb.off[4 * w + 1] = __umul24(wc[w].x >> 16, 24u);
+      nb.off[4 * w + 2] = __umul24(wc[w].y & 0xffffu, 24u);
+      nb.off[4 * w + 3] = __umul24(wc[w].y >> 16, 24u);
     }
     // entries past a lane's count are the sentinel (rejected by the cutoff)
     if constexpr (Tr::Batched) {
@@ -627,7 +641,7 @@ __device__ __forceinline__ void compute_pair_brick_body(const Geom& g, const Bri
       A.vz[s] = uz;
     }
   }
-  if (NEXT) wave_group_count_add(over, nx.count);
+  if (NEXT) wave_group_count_add(over, nx.count, L.wcnt);
 }
 
 template <bool MIXED, bool ACCUM, int EPI, int U>
@@ -674,8 +688,8 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
   if (EPI == EPI_NEXT && threadIdx.x == 0) {  // ordered before every wave's count by brick_setup's barriers
-    wave_group_counter()[0] = 0;
-    wave_group_counter()[1] = 0;
+    L.wcnt[0] = 0;
+    L.wcnt[1] = 0;
   }
   if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, true)) return;
   const LaneAtom At = lane_atom(bp, L);
@@ -861,7 +875,7 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
       A.vz[s] = uz;
     }
   }
-  if (NEXT) wave_group_count_add(over, nx.count);
+  if (NEXT) wave_group_count_add(over, nx.count, L.wcnt);
 }
 
 template <bool MIXED, int NT, bool ACCUM, int EPI>
@@ -942,6 +956,8 @@ uint32_t brick_layout(BrickParams& bp) {
   o = align8(o + 4u * (uint32_t)(bp.nrc / bp.RX));
   bp.off_crow = o;
   o = align8(o + 2u * ((bp.Rmax + kSentinels + 63u) / 64u + 1u));
+  bp.off_wsum = o;
+  o += 4u * (16u + 2u);
   return align8(o);
 }
 
@@ -949,7 +965,25 @@ void launch_brick_stats(const Geom& g, const BrickParams& bp, const uint32_t* ce
   k_brick_stats<<<(bp.nbricks + 255) / 256, 256, 0, st>>>(g, bp, cell_n, out);
 }
 
+// Brick kernels keep all their LDS in the dynamic block, which then starts at LDS address 0:
+// the force traversal addresses staged atoms by absolute LDS address (NbhBatch::off).  Checked
+// once per kernel before its first launch.
+static hipError_t check_no_static_lds(const void* fn) {
+  static std::mutex m;
+  static std::vector<const void*> ok;
+  std::lock_guard<std::mutex> g(m);
+  if (std::find(ok.begin(), ok.end(), fn) != ok.end()) return hipSuccess;
+  hipFuncAttributes a{};
+  const hipError_t e = hipFuncGetAttributes(&a, fn);
+  if (e != hipSuccess) return e;
+  if (a.sharedSizeBytes != 0) return hipErrorInvalidConfiguration;
+  ok.push_back(fn);
+  return hipSuccess;
+}
+
 static hipError_t set_lds(const void* fn, uint32_t bytes) {
+  const hipError_t e = check_no_static_lds(fn);
+  if (e != hipSuccess) return e;
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 

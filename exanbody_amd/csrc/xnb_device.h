@@ -183,21 +183,14 @@ __device__ inline void block_count_add(bool pred, unsigned long long* counter) {
   if (threadIdx.x == 0 && wg_count) atomicAdd(counter, (unsigned long long)wg_count);
 }
 
-// The workgroup's two LDS words of wave_group_count_add: [0] threads counted, [1] waves done.
-// One thread zeroes them ahead of a barrier that precedes every wave's call.
-__device__ inline uint32_t* wave_group_counter() {
-  __shared__ uint32_t w[2];
-  return w;
-}
-
 // block_count_add without a barrier, for kernels whose waves should leave as soon as they
 // finish (an end-of-kernel barrier kept every wave of an 11-wave workgroup resident until the
 // slowest one finished, and the next workgroup could not start in their place: the fused FP64
 // force launch went from 1.78 to 2.1 ms).  Each wave adds its count into LDS; the last wave to
 // finish adds the workgroup's total to the device counter.  Every wave of the workgroup calls
-// it exactly once, after wave_group_counter() was zeroed.
-__device__ inline void wave_group_count_add(bool pred, unsigned long long* counter) {
-  uint32_t* w = wave_group_counter();
+// it exactly once.  w: two LDS words of the workgroup, [0] threads counted, [1] waves done,
+// zeroed by one thread ahead of a barrier that precedes every wave's call.
+__device__ inline void wave_group_count_add(bool pred, unsigned long long* counter, uint32_t* w) {
   const unsigned long long m = __ballot(pred);
   if ((threadIdx.x & 63) == 0) {
     if (m) __hip_atomic_fetch_add(&w[0], (uint32_t)__popcll(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);

@@ -92,13 +92,19 @@ struct NbhCells {
 template <int U>
 struct NbhBatch {
   uint32_t j[U];       /* staged-region index of each neighbour */
+  uint32_t off[U];     /* LDS byte address of neighbour u's x (24 j[u]: the staged positions
+                          start at LDS address 0, the traversal kernels hold no static LDS) */
   const double* pos;   /* x, y, z of region atom i at pos[3 i], pos[3 i + 1], pos[3 i + 2] */
   const uint8_t* type; /* type of region atom i at type[i] */
   __device__ double4 operator()(int u) const {
+    /* read through an LDS-space pointer made from the address, so that forming it is one
+       24-bit multiply of the list entry's half-word */
+    typedef const double __attribute__((address_space(3))) lds_f64;
+    const lds_f64* q = (const lds_f64*)(uintptr_t)off[u];
     double4 p;
-    p.x = pos[3 * j[u]];
-    p.y = pos[3 * j[u] + 1];
-    p.z = pos[3 * j[u] + 2];
+    p.x = q[0];
+    p.y = q[1];
+    p.z = q[2];
     p.w = 0.0;
     return p;
   }
