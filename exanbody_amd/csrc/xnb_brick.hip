@@ -343,10 +343,18 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
   const uint32_t last = bp.cap - 1;
   uint32_t cnt = 0;
   double thr[4] = {max2, max2, max2, max2};  // this atom's list radius^2 per neighbour type
+  // the chunk test's radius^2: the largest this atom needs (its pair radii never exceed
+  // nbh_dist), so candidates beyond every radius of its type are not expanded and re-tested
+  double cut2 = max2;
   if (PAIRWISE) {
     const unsigned ta = min((unsigned)L.pt[A.ia], 3u);
+    cut2 = 0.0;
 #pragma unroll
-    for (int q = 0; q < 4; q++) thr[q] = q < pc->nt ? pc->thr2[ta][q] : max2;
+    for (int q = 0; q < 4; q++) {
+      thr[q] = q < pc->nt ? pc->thr2[ta][q] : max2;
+      if (q < pc->nt) cut2 = fmax(cut2, thr[q]);
+    }
+    cut2 = fmin(cut2, max2);
   }
   if constexpr (!FILTER) {
     // Rows: the cells di = -1, 0, 1 of one (dk, dj) are consecutive in LDS, so a row is one
@@ -379,7 +387,7 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
               const double dy = ay - pp[3 * (q + u) + 1];
               const double dz = az - pp[3 * (q + u) + 2];
               const double d2 = (dx * dx + dy * dy) + dz * dz;
-              m = m + m + ((d2 > 0.0 && d2 <= max2) ? 1u : 0u);
+              m = m + m + ((d2 > 0.0 && d2 <= (PAIRWISE ? cut2 : max2)) ? 1u : 0u);
             }
           }
           if (np < 32) m <<= 32 - np;
