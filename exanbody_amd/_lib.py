@@ -73,6 +73,7 @@ _SIGS = {
     "xnb_chunk_neighbors": (C.c_int, [P, C.c_double, C.POINTER(ChunkNeighborsConfig), C.c_int]),
     "xnb_chunk_neighbors_stream": (C.c_int, [P, P, C.c_size_t, P, C.POINTER(C.c_size_t), C.POINTER(C.c_uint32)]),
     "xnb_zero_particle_force": (C.c_int, [P, C.c_int]),
+    "xnb_divide_force_by_type_scalar": (C.c_int, [P, C.c_int, P, C.c_int]),
     "xnb_lennard_jones_force": (C.c_int, [P, C.POINTER(LJParams), C.c_double, C.c_int, C.c_int]),
     "xnb_lennard_jones_force_multimat": (C.c_int, [P, C.c_int, P, P, P, C.c_int]),
     "xnb_gravitational_force": (C.c_int, [P, C.c_double, C.c_int, P, C.c_double, C.c_int]),

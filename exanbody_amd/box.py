@@ -174,6 +174,12 @@ class Box:
     def zero_particle_force(self, ghost=False):
         self._check(self.L.xnb_zero_particle_force(self._h, int(ghost)), "zero_particle_force")
 
+    def divide_force_by_type_scalar(self, values, ghost=False):
+        """f /= values[type] (divide_force_by_type_scalar, e.g. with the type masses)."""
+        v = np.ascontiguousarray(values, np.float64)
+        self._check(self.L.xnb_divide_force_by_type_scalar(self._h, int(v.size), v.ctypes.data, int(ghost)),
+                    "divide_force_by_type_scalar")
+
     def lennard_jones_force(self, epsilon, sigma, rcut, precision=_lib.PRECISION_FP64, ghost=False):
         p = _lib.LJParams(epsilon, sigma)
         self._check(self.L.xnb_lennard_jones_force(self._h, C.byref(p), rcut, int(precision), int(ghost)),

@@ -1130,6 +1130,25 @@ int xnb_zero_particle_force(xnb_ctx* c, int ghost) {
   return XNB_OK;
 }
 
+int xnb_divide_force_by_type_scalar(xnb_ctx* c, int ntypes, const double* values, int ghost) {
+  int e;
+  if ((e = check_ctx(c))) return e;
+  if (ntypes < 1 || ntypes > 256 || !values) return fail(c, XNB_E_ARG, "divide_force_by_type_scalar: 1..256 type values required");
+  const uint32_t n = ghost ? c->N + c->G : c->N;
+  if (n > 0) {  // every particle's type indexes the table
+    HIPCHK(c, hipMemsetAsync(c->u32s.p + 9, 0, sizeof(uint32_t), c->st));
+    launch_max_type(n, c->A.r, c->u32s.p + 9, c->st);
+    HIPCHK(c, hipGetLastError());
+    uint32_t tmax = 0;
+    if ((e = read_u32(c, c->u32s.p + 9, &tmax))) return e;
+    if ((int)tmax >= ntypes) return fail(c, XNB_E_ARG, "divide_force_by_type_scalar: a particle type has no value");
+  }
+  launch_divide_force_by_type(n, c->A, values, ntypes, c->st);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return XNB_OK;
+}
+
 int xnb_lennard_jones_force(xnb_ctx* c, const xnb_lj_params* p, double rcut, int precision, int ghost) {
   int e;
   if ((e = check_ctx(c))) return e;

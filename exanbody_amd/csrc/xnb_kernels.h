@@ -185,6 +185,7 @@ void launch_verlet_first_half(const Geom& g, uint32_t N, const PState& A, double
                               const uint32_t* slot_cell, const uint32_t* backup, double thr2, unsigned long long* count,
                               hipStream_t st);
 void launch_zero_force(uint32_t n, const PState& A, hipStream_t st);
+void launch_divide_force_by_type(uint32_t n, const PState& A, const double* values, int ntypes, hipStream_t st);
 void launch_nbh_build(const Geom& g, uint32_t n_atoms, const double4* r, const uint32_t* slot_cell, const uint32_t* cell_off,
                       const uint32_t* cell_n, double max2, int lay, int half_symmetric, int skip_ghosts, uint32_t* nbr,
                       uint32_t* nbr_cnt, uint32_t cap, uint32_t* max_cnt, hipStream_t st);

@@ -1132,6 +1132,27 @@ void launch_verlet_first_half(const Geom& g, uint32_t N, const PState& A, double
   k_verlet_first_half<<<nblk(N, 1024), 1024, 0, st>>>(g, N, A.r, A.vx, A.vy, A.vz, A.fx, A.fy, A.fz, dt, dt2, dth, slot_cell, backup,
                                                     thr2, count);
 }
+// divide_force_by_type_scalar (vec3_typescalar_op.cu:52-120, 161): f /= value[type] per
+// particle, the type taken from r.w; values of up to 256 types in a kernel argument
+struct TypeScalars {
+  double v[256];
+};
+__global__ void k_divide_force_by_type(uint32_t n, const double4* __restrict__ r, double* __restrict__ fx, double* __restrict__ fy,
+                                       double* __restrict__ fz, TypeScalars ts) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const double m = ts.v[(uint32_t)r[s].w & 255u];
+  fx[s] = fx[s] / m;
+  fy[s] = fy[s] / m;
+  fz[s] = fz[s] / m;
+}
+
+void launch_divide_force_by_type(uint32_t n, const PState& A, const double* values, int ntypes, hipStream_t st) {
+  TypeScalars ts{};
+  for (int t = 0; t < 256; t++) ts.v[t] = t < ntypes ? values[t] : 1.0;
+  if (n) k_divide_force_by_type<<<nblk(n, 256), 256, 0, st>>>(n, A.r, A.fx, A.fy, A.fz, ts);
+}
+
 void launch_zero_force(uint32_t n, const PState& A, hipStream_t st) {
   k_zero_force<<<nblk(n, 256), 256, 0, st>>>(n, A.fx, A.fy, A.fz, A.ep);
 }

@@ -6,13 +6,16 @@
 // Operator names and slot names follow the reference (data/config/numerical-scheme.msp,
 // update-particles.msp, compute-loop.msp; operator sources cited per operator below).
 #include <algorithm>
+#include <array>
 #include <cctype>
 #include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <functional>
 #include <map>
+#include <random>
 #include <memory>
 #include <sstream>
 #include <stdexcept>
@@ -268,12 +271,64 @@ Node parse_yaml(const std::string& text) {
 }
 
 // ------------------------------------------------------------------ values
+// Quantities: a number, optionally followed by a unit expression ("3.48 ang", "2e-3 ps",
+// "0.3729 eV", "58.693 Da", "ang/ps^2"), converted to the internal units of the reference's
+// physics library: length angstrom, time picosecond, mass Dalton, energy Da ang^2 / ps^2,
+// charge e, temperature K.  A bare number is taken as already internal (reduced-unit decks).
+// Constants: those of the reference's unit system, fixed by its golden check_values_lj_Ni.dat
+// (elementary charge 1.6021892e-19 C, Dalton 1.660539040e-27 kg).
+static constexpr double kElemCharge = 1.6021892e-19;
+static constexpr double kDaltonKg = 1.660539040e-27;
+static constexpr double kEnergyJ = kDaltonKg * 1.0e-20 / 1.0e-24;  // internal energy unit in J
+
+static bool unit_factor(const std::string& u, double& f) {
+  static const std::map<std::string, double> table = {
+      {"m", 1e10}, {"cm", 1e8}, {"mm", 1e7}, {"um", 1e4}, {"nm", 10.0}, {"ang", 1.0}, {"angstrom", 1.0},
+      {"s", 1e12}, {"ms", 1e9}, {"us", 1e6}, {"ns", 1e3}, {"ps", 1.0}, {"fs", 1e-3},
+      {"kg", 1.0 / kDaltonKg}, {"g", 1e-3 / kDaltonKg}, {"Da", 1.0}, {"Dalton", 1.0}, {"amu", 1.0},
+      {"J", 1.0 / kEnergyJ}, {"eV", kElemCharge / kEnergyJ},
+      {"C", 1.0 / kElemCharge}, {"elementary_charge", 1.0},
+      {"K", 1.0}, {"rad", 1.0}, {"radian", 1.0}, {"degree", 3.14159265358979323846 / 180.0}};
+  auto it = table.find(u);
+  if (it == table.end()) return false;
+  f = it->second;
+  return true;
+}
+
+// "a.b/c^2": products ('.', '*') and quotients ('/') of units with integer powers ('^')
+static bool unit_expression(const std::string& e, double& f) {
+  f = 1.0;
+  size_t i = 0;
+  bool divide = false;
+  while (i < e.size()) {
+    size_t j = i;
+    while (j < e.size() && e[j] != '.' && e[j] != '*' && e[j] != '/' && e[j] != '^') j++;
+    double uf;
+    if (!unit_factor(e.substr(i, j - i), uf)) return false;
+    int pw = 1;
+    if (j < e.size() && e[j] == '^') {
+      char* end = nullptr;
+      pw = (int)std::strtol(e.c_str() + j + 1, &end, 10);
+      j = (size_t)(end - e.c_str());
+    }
+    const double x = std::pow(uf, pw);
+    f = divide ? f / x : f * x;
+    if (j < e.size()) divide = e[j] == '/';
+    i = j + 1;
+  }
+  return true;
+}
+
 static double as_double(const Node& n, const std::string& what) {
   if (n.kind != Node::Scalar) throw DeckError(what + ": expected a number");
   char* end = nullptr;
   const double v = std::strtod(n.s.c_str(), &end);
   if (end == n.s.c_str()) throw DeckError(what + ": not a number: " + n.s);
-  return v;
+  const std::string unit = trim(std::string(end));
+  if (unit.empty()) return v;
+  double f;
+  if (!unit_expression(unit, f)) throw DeckError(what + ": unknown unit '" + unit + "'");
+  return v * f;
 }
 static bool as_bool(const Node& n, const std::string& what) {
   if (n.kind != Node::Scalar) throw DeckError(what + ": expected a boolean");
@@ -376,6 +431,8 @@ struct Runner {
       if (def && def->kind == Node::Map) params = *def;
       if (inline_params && inline_params->kind == Node::Map)
         for (auto& kv : inline_params->m) params.set(kv.first, kv.second);
+      else if (inline_params && inline_params->kind == Node::Scalar)
+        params.set("__scalar", *inline_params);  // "- op: value" sets the operator's main slot
       op->second(*this, params);
     } else if (name == "nop") {
     } else {
@@ -419,6 +476,296 @@ struct Runner {
   }
 };
 
+// ------------------------------------------------------------------ input and checks
+// Operators of the reference's sample decks around the path: particle types and their
+// properties, the lattice generator, deterministic position noise, the mass division of the
+// forces and the golden-file check.  Single rank, identity xform.
+
+// particle_type_map slot -> type index of a name (particle_types, particle_type_map.h)
+static int type_index(Runner& r, const std::string& name) {
+  auto it = r.slots.find("particle_type_map");
+  if (it != r.slots.end() && it->second.kind == Node::Map) {
+    if (const Node* v = it->second.get(name)) return (int)as_double(*v, "particle_type_map");
+    throw DeckError("particle type '" + name + "' is not in particle_type_map");
+  }
+  Node& order = r.slots["__type_order"];  // no map: types numbered in order of first use
+  order.kind = Node::Seq;
+  for (size_t q = 0; q < order.q.size(); q++)
+    if (order.q[q].s == name) return (int)q;
+  order.q.push_back(Node::scalar(name));
+  return (int)order.q.size() - 1;
+}
+
+struct DomainBox {
+  double lo[3], hi[3], cs;
+  int64_t dims[3];
+};
+static DomainBox domain_box(Runner& r, const char* what) {
+  auto it = r.slots.find("domain");
+  if (it == r.slots.end()) throw DeckError(std::string(what) + ": no domain");
+  const Node& p = it->second;
+  DomainBox b{};
+  b.cs = as_double(*p.get("cell_size"), "cell_size");
+  for (int a = 0; a < 3; a++) {
+    b.lo[a] = as_double(p.get("bounds")->q.at(0).q.at(a), "bounds");
+    b.hi[a] = as_double(p.get("bounds")->q.at(1).q.at(a), "bounds");
+    b.dims[a] = (int64_t)as_double(p.get("grid_dims")->q.at(a), "grid_dims");
+  }
+  return b;
+}
+
+static void register_input_ops(Runner& R) {
+  // particle_types (src/core/particle_type_map.cpp): name -> index map
+  R.ops["particle_types"] = [](Runner& r, const Node& p) {
+    const Node* m = p.get("particle_type_map");
+    if (!m || m->kind != Node::Map) throw DeckError("particle_types: particle_type_map {name: index} is required");
+    r.slots["particle_type_map"] = *m;
+  };
+  // particle_type_add_properties: per type name, scalar properties (mass, charge, ...)
+  R.ops["particle_type_add_properties"] = [](Runner& r, const Node& p) {
+    Node& props = r.slots["particle_type_properties"];
+    props.kind = Node::Map;
+    for (auto& kv : p.m) {
+      if (kv.second.kind != Node::Map) continue;
+      Node* t = props.get(kv.first);
+      if (!t) {
+        Node e;
+        e.kind = Node::Map;
+        props.set(kv.first, e);
+        t = props.get(kv.first);
+      }
+      for (auto& q : kv.second.m) t->set(q.first, Node::scalar(fmt(as_double(q.second, q.first))));
+    }
+  };
+  // lattice (lattice_generator.h:100-245, generate_particle_lattice.h:157, 262-383 with
+  // deterministic numbering): one thread walks k, j, i from floor(lo / a) - 1 to ceil(hi / a) + 1
+  // and the unit cell's sites in order; a site inside the domain bounds (ends included) joins
+  // the cell floor((x - lo) / cell_size) if that is an inner cell; ids start after the largest
+  // existing id and run through the cells in index order, within a cell in insertion order.
+  R.ops["lattice"] = [](Runner& r, const Node& p) {
+    xnb_ctx* c = r.need_ctx("lattice");
+    size_t n0 = 0, g0 = 0;
+    r.check(xnb_num_particles(c, &n0, &g0), "lattice");
+    if (n0) throw DeckError("lattice: the grid already holds particles (only an empty grid is filled)");
+    const Node* st = p.get("structure");
+    const Node* ty = p.get("types");
+    const Node* sz = p.get("size");
+    if (!st || !ty || !sz || ty->kind != Node::Seq || sz->kind != Node::Seq) throw DeckError("lattice: structure, types and size are required");
+    std::vector<std::array<double, 3>> sites;
+    if (st->s == "SC") sites = {{0., 0., 0.}};
+    else if (st->s == "BCC") sites = {{0., 0., 0.}, {.5, .5, .5}};
+    else if (st->s == "FCC") sites = {{0., 0., 0.}, {0., .5, .5}, {.5, 0., .5}, {.5, .5, 0.}};
+    else if (st->s == "2BCT") sites = {{0., 0., 0.}, {0., .5, .25}, {.5, .5, .5}, {.5, 0., .75}};
+    else throw DeckError("lattice: structure '" + st->s + "' is not supported (SC, BCC, FCC, 2BCT)");
+    if (ty->q.size() != sites.size()) throw DeckError("lattice: structure " + st->s + " needs " + std::to_string(sites.size()) + " types");
+    std::vector<int> tix;
+    for (auto& t : ty->q) tix.push_back(type_index(r, t.s));
+    double a[3], shift[3] = {0., 0., 0.};
+    for (int d = 0; d < 3; d++) a[d] = as_double(sz->q.at(d), "size");
+    if (const Node* sh = p.get("shift"))
+      for (int d = 0; d < 3; d++) shift[d] = as_double(sh->q.at(d), "shift");
+    const DomainBox b = domain_box(r, "lattice");
+    int64_t lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+      lo[d] = (int64_t)std::floor(b.lo[d] / a[d]) - 1;
+      hi[d] = (int64_t)std::ceil(b.hi[d] / a[d]) + 1;
+    }
+    const int64_t nc = b.dims[0] * b.dims[1] * b.dims[2];
+    std::vector<std::vector<std::array<double, 4>>> cells((size_t)nc);
+    for (int64_t k = lo[2]; k <= hi[2]; k++)
+      for (int64_t j = lo[1]; j <= hi[1]; j++)
+        for (int64_t i = lo[0]; i <= hi[0]; i++)
+          for (size_t l = 0; l < sites.size(); l++) {
+            const double x[3] = {(i + sites[l][0]) * a[0] + shift[0], (j + sites[l][1]) * a[1] + shift[1],
+                                 (k + sites[l][2]) * a[2] + shift[2]};
+            int64_t ci[3];
+            bool in = true;
+            for (int d = 0; d < 3; d++) {
+              in = in && x[d] >= b.lo[d] && x[d] <= b.hi[d];
+              ci[d] = (int64_t)std::floor((x[d] - b.lo[d]) / b.cs);
+              in = in && ci[d] >= 0 && ci[d] < b.dims[d];
+            }
+            if (!in) continue;
+            cells[(size_t)(ci[0] + b.dims[0] * (ci[1] + b.dims[1] * ci[2]))].push_back({x[0], x[1], x[2], (double)tix[l]});
+          }
+    std::vector<double> rr, vv;
+    std::vector<uint64_t> ids;
+    std::vector<uint8_t> tt;
+    uint64_t next = 1;
+    for (auto& cl : cells)
+      for (auto& s4 : cl) {
+        rr.insert(rr.end(), {s4[0], s4[1], s4[2]});
+        vv.insert(vv.end(), {0., 0., 0.});
+        ids.push_back(next++);
+        tt.push_back((uint8_t)s4[3]);
+      }
+    r.check(xnb_set_particles(c, ids.size(), rr.data(), vv.data(), ids.data(), tt.data()), "lattice");
+    r.log << "lattice " << st->s << ": " << ids.size() << " particles\n";
+  };
+  // gaussian_noise_r (gaussian_noise.h:62-79, 122-165, gaussian_noise.cpp:32): normal deviates
+  // of standard deviation sigma added to rx, ry, rz of every inner particle, cell by cell in
+  // particle order; with deterministic_noise each cell seeds its own std::mt19937_64 with
+  // (domain cell index) * 1023 and uses a fresh std::normal_distribution.  The particles are
+  // then re-binned by their new positions.
+  R.ops["gaussian_noise_r"] = [](Runner& r, const Node& p) {
+    xnb_ctx* c = r.need_ctx("gaussian_noise_r");
+    const double sigma = r.num(p, "sigma", 1.0);
+    if (p.get("sigma_cut") || p.get("region")) throw DeckError("gaussian_noise_r: sigma_cut and region are not supported");
+    const bool det = r.flag(p, "deterministic_noise", false);
+    size_t n = 0, ng = 0;
+    r.check(xnb_num_particles(c, &n, &ng), "gaussian_noise_r");
+    int64_t dims[3];
+    int gl = 0;
+    size_t ncells = 0;
+    r.check(xnb_grid_info(c, dims, &gl, &ncells), "gaussian_noise_r");
+    std::vector<uint32_t> cell(n);
+    std::vector<double> rr(3 * n), vv(3 * n);
+    std::vector<uint64_t> ids(n);
+    std::vector<uint8_t> tt(n);
+    r.check(xnb_get_particles(c, 0, cell.data(), rr.data(), vv.data(), nullptr, nullptr, ids.data(), tt.data()), "gaussian_noise_r");
+    std::random_device rd;
+    std::mt19937_64 free_re(rd());
+    std::mt19937_64 det_re;
+    const int64_t Di = dims[0] - 2 * gl, Dj = dims[1] - 2 * gl;
+    size_t q = 0;
+    while (q < n) {
+      size_t e = q;
+      while (e < n && cell[e] == cell[q]) e++;
+      const int64_t ci = (int64_t)cell[q] % dims[0], cj = ((int64_t)cell[q] / dims[0]) % dims[1],
+                    ck = (int64_t)cell[q] / (dims[0] * dims[1]);
+      const uint64_t dom_idx = (uint64_t)((ci - gl) + Di * ((cj - gl) + Dj * (ck - gl)));
+      std::mt19937_64& re = det ? det_re : free_re;
+      if (det) det_re.seed(dom_idx * 1023);
+      std::normal_distribution<double> gaussian(0.0, sigma);
+      for (size_t k = q; k < e; k++)
+        for (int d = 0; d < 3; d++) rr[3 * k + d] += gaussian(re);
+      q = e;
+    }
+    r.check(xnb_set_particles(c, n, rr.data(), vv.data(), ids.data(), tt.data()), "gaussian_noise_r");
+  };
+  // replicate_domain (src/grid_cell_particles/replicate_domain.cpp): only repeat [1, 1, 1]
+  R.ops["replicate_domain"] = [](Runner&, const Node& p) {
+    if (const Node* rep = p.get("repeat"))
+      for (auto& x : rep->q)
+        if (as_double(x, "repeat") != 1.0) throw DeckError("replicate_domain: only repeat [1, 1, 1] is supported");
+  };
+  // divide_force_by_type_scalar (vec3_typescalar_op.cu:52-120, 161): property slot (the scalar
+  // form "- divide_force_by_type_scalar: mass" sets it), values from particle_type_properties
+  R.ops["divide_force_by_type_scalar"] = [](Runner& r, const Node& p) {
+    const Node* pr = p.get("property");
+    if (!pr) pr = p.get("__scalar");
+    if (!pr) throw DeckError("divide_force_by_type_scalar: property is required");
+    auto it = r.slots.find("particle_type_properties");
+    if (it == r.slots.end()) throw DeckError("divide_force_by_type_scalar: no particle_type_properties");
+    std::vector<double> vals;
+    for (auto& kv : it->second.m) {
+      const Node* v = kv.second.get(pr->s);
+      if (!v) throw DeckError("particle type property '" + pr->s + "' does not exist for type " + kv.first);
+      const int t = type_index(r, kv.first);
+      if (t < 0 || t > 255) throw DeckError("divide_force_by_type_scalar: type index out of range");
+      if ((int)vals.size() <= t) vals.resize((size_t)t + 1, 1.0);
+      vals[(size_t)t] = as_double(*v, pr->s);
+    }
+    if (vals.empty()) throw DeckError("divide_force_by_type_scalar: no types");
+    r.check(xnb_divide_force_by_type_scalar(r.need_ctx("divide_force_by_type_scalar"), (int)vals.size(), vals.data(), 0),
+            "divide_force_by_type_scalar");
+  };
+  // check_values (src/debug/check_values.cpp:48-400): compare sampled particles (id, position,
+  // acceleration = force field, velocity; hex floats) with a reference file; positions are
+  // compared through the periodic image closest to the particle.  A missing file is created
+  // (samples particles at evenly spaced indices of the id-sorted set, the reference picks
+  // them at random).  Thresholds: pos / acc default `threshold` (1e-16), vel unchecked.
+  R.ops["check_values"] = [](Runner& r, const Node& p) {
+    xnb_ctx* c = r.need_ctx("check_values");
+    const Node* fnode = p.get("file");
+    if (!fnode) throw DeckError("check_values: file is required");
+    const double thr = r.num(p, "threshold", 1e-16);
+    const double pos_thr = p.get("pos_threshold") ? r.num(p, "pos_threshold", thr) : thr;
+    const double acc_thr = p.get("acc_threshold") ? r.num(p, "acc_threshold", thr) : thr;
+    const double vel_thr = p.get("vel_threshold") ? r.num(p, "vel_threshold", thr) : 1.0e40;
+    const bool fatal = r.flag(p, "fatal", true);
+    size_t n = 0, ng = 0;
+    r.check(xnb_num_particles(c, &n, &ng), "check_values");
+    std::vector<double> rr(3 * n), vv(3 * n), ff(3 * n);
+    std::vector<uint64_t> ids(n);
+    r.check(xnb_get_particles(c, 0, nullptr, rr.data(), vv.data(), ff.data(), nullptr, ids.data(), nullptr), "check_values");
+    std::map<uint64_t, size_t> at;
+    for (size_t q = 0; q < n; q++) at[ids[q]] = q;
+    const DomainBox b = domain_box(r, "check_values");
+    std::ifstream fin(fnode->s);
+    if (!fin) {
+      const long want = std::min<long>((long)r.num(p, "samples", 128), (long)n);
+      std::ofstream fout(fnode->s);
+      if (!fout) throw DeckError("check_values: cannot create " + fnode->s);
+      fout << "length_unit: 1.0 ang\nvalues:\n";
+      std::vector<uint64_t> sorted(ids);
+      std::sort(sorted.begin(), sorted.end());
+      for (long s = 0; s < want; s++) {
+        const size_t q = at[sorted[(size_t)((double)s * (double)n / (double)want)]];
+        char line[1024];
+        std::snprintf(line, sizeof line, "- [%llu, %a, %a, %a, %a, %a, %a, %a, %a, %a]\n", (unsigned long long)ids[q], rr[3 * q],
+                      rr[3 * q + 1], rr[3 * q + 2], ff[3 * q], ff[3 * q + 1], ff[3 * q + 2], vv[3 * q], vv[3 * q + 1], vv[3 * q + 2]);
+        fout << line;
+      }
+      r.log << "check_values: created " << fnode->s << " with " << want << " samples\n";
+      return;
+    }
+    std::stringstream ss;
+    ss << fin.rdbuf();
+    const Node doc = parse_yaml(ss.str());
+    const Node* vals = doc.get("values");
+    if (!vals || vals->kind != Node::Seq) throw DeckError("check_values: no values in " + fnode->s);
+    const double ls = doc.get("length_unit") ? as_double(*doc.get("length_unit"), "length_unit") : 1.0;
+    double mr = 0., ma = 0., mv = 0.;
+    long bad = 0;
+    std::string first_bad;
+    for (const Node& row : vals->q) {
+      if (row.kind != Node::Seq || row.q.size() < 7) throw DeckError("check_values: bad value row");
+      const uint64_t id = (uint64_t)std::strtoull(row.q[0].s.c_str(), nullptr, 10);
+      auto f = at.find(id);
+      if (f == at.end()) throw DeckError("check_values: particle #" + row.q[0].s + " is not in the grid");
+      const size_t q = f->second;
+      double er = 0., ea = 0., ev = 0.;
+      for (int d = 0; d < 3; d++) {
+        const double L = b.hi[d] - b.lo[d];
+        double dr = as_double(row.q[1 + d], "r") * ls - rr[3 * q + d];
+        dr -= L * std::nearbyint(dr / L);
+        er += dr * dr;
+        const double da = as_double(row.q[4 + d], "a") * ls - ff[3 * q + d];
+        ea += da * da;
+        if (row.q.size() >= 10) {
+          const double dv = as_double(row.q[7 + d], "v") - vv[3 * q + d];
+          ev += dv * dv;
+        }
+      }
+      er = std::sqrt(er);
+      ea = std::sqrt(ea);
+      ev = std::sqrt(ev);
+      mr = std::max(mr, er);
+      ma = std::max(ma, ea);
+      mv = std::max(mv, ev);
+      if (er > pos_thr || ea > acc_thr || ev > vel_thr) {
+        if (bad++ == 0)
+          first_bad = "particle #" + row.q[0].s + " : re=" + fmt(er) + "/" + fmt(pos_thr) + " ae=" + fmt(ea) + "/" + fmt(acc_thr) +
+                      " ve=" + fmt(ev) + "/" + fmt(vel_thr);
+      }
+    }
+    r.out_num("check_values_max_dr", mr);
+    r.out_num("check_values_max_da", ma);
+    r.out_num("check_values_max_dv", mv);
+    r.log << "check_values: " << vals->q.size() << " particles, max |dr| " << fmt(mr) << ", |da| " << fmt(ma) << ", |dv| " << fmt(mv)
+          << (bad ? ", FAILED" : ", ok") << "\n";
+    if (bad && fatal) throw DeckError("check_values: " + std::to_string(bad) + " particles beyond the thresholds, first " + first_bad);
+  };
+  // single-rank / host-side no-ops of the reference's default_simulation (main-config.msp:33-49);
+  // update_force_from_ghost adds nothing here: forces are computed for inner particles from
+  // full lists, ghost particles never receive any
+  for (const char* nm : {"logo_banner", "mpi_comm_world", "init_cuda", "update_ghost_config", "unit_system", "print_domain",
+                         "memory_stats", "finalize_cuda", "update_force_from_ghost", "trigger_dump_data", "trigger_print_log",
+                         "dump_data_if_triggered", "print_log_if_triggered", "global"})
+    R.ops[nm] = [](Runner&, const Node&) {};
+}
+
 // ------------------------------------------------------------------ operators
 static void register_ops(Runner& R) {
   // domain (src/io/domain_init.cpp; Domain, src/core/include/exanb/core/domain.h:36-129) + init_rcb_grid
@@ -429,6 +776,10 @@ static void register_ops(Runner& R) {
     const Node* bd = p.get("bounds");
     const Node* pr = p.get("periodic");
     if (!cs || !gd || !bd) throw DeckError("domain: cell_size, grid_dims and bounds are required");
+    if (const Node* xf = p.get("xform"))  // only the identity transform
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+          if (as_double(xf->q.at(a).q.at(b), "xform") != (a == b ? 1.0 : 0.0)) throw DeckError("domain: only the identity xform is supported");
     d.cell_size = as_double(*cs, "cell_size");
     for (int a = 0; a < 3; a++) {
       d.grid_dims[a] = (int64_t)as_double(gd->q.at(a), "grid_dims");
@@ -544,8 +895,11 @@ static void register_ops(Runner& R) {
     const bool first = r.flag(p, "first", true);
     r.out_bool("result", fq > 0 && ts % fq == 0 && (first || ts > 0));
   };
+  // loop continues while timestep <= end_at: with compute-loop.msp's order (first_iteration
+  // advances the timestep to 1, each iteration advances it after its step) end_at steps are
+  // done, which the reference's golden check_values_lj_Ni.dat (100 steps) fixes
   R.ops["sim_continue"] = [](Runner& r, const Node& p) {
-    r.out_bool("result", r.num(p, "timestep", 0.0) < r.num(p, "end_at", 0, true));
+    r.out_bool("result", r.num(p, "timestep", 0.0) <= r.num(p, "end_at", 0, true));
   };
   R.ops["next_time_step"] = [](Runner& r, const Node& p) {
     r.out_num("timestep", r.num(p, "timestep", 0.0) + 1.0);
@@ -564,6 +918,7 @@ static void register_ops(Runner& R) {
   R.ops["message"] = [](Runner& r, const Node& p) {
     if (const Node* m = p.get("text")) r.log << m->s << "\n";
   };
+  register_input_ops(R);
 }
 
 // merge a parsed deck into the runner: "name:" replaces, "+name:" appends to a sequence

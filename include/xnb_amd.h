@@ -142,6 +142,10 @@ int xnb_chunk_neighbors_stream(xnb_ctx* ctx, uint16_t* out, size_t cap, uint64_t
                                size_t* total, uint32_t* max_neighbors);
 /* zero_particle_force (src/compute/zero_particle_force.cu:19-55) */
 int xnb_zero_particle_force(xnb_ctx* ctx, int ghost);
+/* divide_force_by_type_scalar (src/compute/vec3_typescalar_op.cu:52-120, 161): every inner
+ * particle's force (ghosts too if ghost) divided by values[type], e.g. the type masses to turn
+ * forces into accelerations.  1..256 values; a particle type without a value is XNB_E_ARG. */
+int xnb_divide_force_by_type_scalar(xnb_ctx* ctx, int ntypes, const double* values, int ghost);
 /* lennard_jones_force (contribs/md/lennard_jones/lennard_jones.cu:171-223) over the
  * compute_cell_particle_pairs traversal (src/compute/include/exanb/compute/
  * compute_cell_particle_pairs.h:119-243); adds to f and to per-atom energy ep. */

@@ -58,7 +58,7 @@ trigger_move_particles:
              "global:\n  simulation_end_iteration: 40\n  rebuild_freq: 10\n" + extra)
     d.set_particles(r, v, ids)
     d.run()
-    assert d.slot("timestep") == "40"
+    assert d.slot("timestep") == "41"  # advanced after each of the 40 steps, from 1
     o = Oracle(dom["bounds_min"], dom["bounds_max"], dom["grid_dims"], dom["cell_size"], ghost_dist=2.8)
     o.set_particles(r, v, ids)
     o.init(1.0, 1.0, 2.5, 0.3)
