@@ -22,28 +22,36 @@
 
 namespace xnb {
 
-// inclusive-exclusive scan across the whole workgroup (blockDim.x multiple of 64, <= 1024);
-// wsum: 16 words of LDS.  The brick kernels keep every LDS word in their dynamic block, so
-// it starts at address 0 and a staged atom's address is one multiply of its index.
-__device__ inline uint32_t block_scan_excl(uint32_t v, uint32_t* total, uint32_t* wsum) {
+// The brick kernels keep every LDS word in their dynamic block, so it starts at address 0 and
+// a staged atom's address is one multiply of its index.
+
+// Two exclusive scans in one pass (a in the low, b in the high 32 bits of a 64-bit sum; the
+// low sums stay below 2^32): ea / eb the exclusive prefixes, ta / tb the workgroup totals.
+// wsum: 32 words of LDS, not used again by the caller (so no barrier after its last read).
+__device__ inline void block_scan_excl2(uint32_t a, uint32_t b, uint32_t& ea, uint32_t& eb, uint32_t& ta, uint32_t& tb,
+                                        uint32_t* wsum) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  uint32_t x = v;
+  const unsigned long long v = (unsigned long long)a | ((unsigned long long)b << 32);
+  unsigned long long x = v;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    uint32_t y = __shfl_up(x, o, 64);
+    const unsigned long long y = __shfl_up(x, o, 64);
     if (lane >= o) x += y;
   }
-  if (lane == 63) wsum[w] = x;
+  unsigned long long* ws = reinterpret_cast<unsigned long long*>(wsum);
+  if (lane == 63) ws[w] = x;
   __syncthreads();
-  uint32_t woff = 0, tot = 0;
+  unsigned long long woff = 0, tot = 0;
   for (int i = 0; i < nw; i++) {
-    const uint32_t s = wsum[i];
+    const unsigned long long s = ws[i];
     if (i < w) woff += s;
     tot += s;
   }
-  __syncthreads();
-  *total = tot;
-  return x - v + woff;
+  const unsigned long long ex = x - v + woff;
+  ea = (uint32_t)ex;
+  eb = (uint32_t)(ex >> 32);
+  ta = (uint32_t)tot;
+  tb = (uint32_t)(tot >> 32);
 }
 
 __device__ inline uint32_t wave_max_b(uint32_t v) { return wave_max_u32(v); }
@@ -74,7 +82,7 @@ struct BrickLds {
   uint16_t* crow;   // per 64 region atoms: the row holding the first of them
   double *px, *py, *pz;
   uint8_t* pt;  // particle type (r.w) of each region atom
-  uint32_t* wsum;  // scan scratch (16 words)
+  uint32_t* wsum;  // scan scratch (32 words)
   uint32_t* wcnt;  // wave_group_count_add's two words
 };
 
@@ -91,7 +99,7 @@ __device__ inline BrickLds brick_lds(const BrickParams& bp, char* lds) {
   L.pz = L.px + 2;
   L.pt = (uint8_t*)(lds + bp.off_type);
   L.wsum = (uint32_t*)(lds + bp.off_wsum);
-  L.wcnt = L.wsum + 16;
+  L.wcnt = L.wsum + 32;
   return L;
 }
 
@@ -166,23 +174,24 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
     L.rcnt[t] = cnt;
     L.roff[t] = off;
   }
-  uint32_t tot;
-  const uint32_t ex = block_scan_excl(cnt, &tot, L.wsum);
-  if (t < (uint32_t)bp.nrc) L.rpref[t] = ex;
-  if (t == 0) L.rpref[bp.nrc] = tot;
-  __syncthreads();
-  // brick cells: bc = (ck*Bj + cj)*Bi + ci, region cell ((ck+1)*RY + cj+1)*RX + ci+1
+  // brick cells bc = (ck*Bj + cj)*Bi + ci (region cell ((ck+1)*RY + cj+1)*RX + ci+1), read in
+  // the same memory round, and both prefix sums in one scan
   uint32_t bcnt = 0;
   if (t < (uint32_t)bp.nbc) {
     const int ci = (int)(t % (uint32_t)bp.Bi), cj = (int)((t / (uint32_t)bp.Bi) % (uint32_t)bp.Bj),
               ck = (int)(t / ((uint32_t)bp.Bi * (uint32_t)bp.Bj));
-    if (ci < P.ni && cj < P.nj && ck < P.nk) bcnt = L.rcnt[((ck + 1) * bp.RY + cj + 1) * bp.RX + ci + 1];
+    if (ci < P.ni && cj < P.nj && ck < P.nk) bcnt = cell_n[cell_index(g, P.gi0 + ci, P.gj0 + cj, P.gk0 + ck)];
   }
-  const uint32_t bex = block_scan_excl(bcnt, &tot, L.wsum);
+  uint32_t ex, bex, rtot, btot;
+  block_scan_excl2(cnt, bcnt, ex, bex, rtot, btot, L.wsum);
+  if (t < (uint32_t)bp.nrc) L.rpref[t] = ex;
   if (t < (uint32_t)bp.nbc) L.bpref[t] = bex;
-  if (t == 0) L.bpref[bp.nbc] = tot;
+  if (t == 0) {
+    L.rpref[bp.nrc] = rtot;
+    L.bpref[bp.nbc] = btot;
+  }
   __syncthreads();
-  const bool fits_r = L.rpref[bp.nrc] <= bp.Rmax, fits_t = L.bpref[bp.nbc] <= bp.T;
+  const bool fits_r = rtot <= bp.Rmax, fits_t = btot <= bp.T;
   if (!fits_r || !fits_t) {
     if (t == 0) atomicOr(bp.status, (fits_r ? 0u : 1u) | (fits_t ? 0u : 2u));
     return false;
@@ -965,7 +974,7 @@ uint32_t brick_layout(BrickParams& bp) {
   bp.off_crow = o;
   o = align8(o + 2u * ((bp.Rmax + kSentinels + 63u) / 64u + 1u));
   bp.off_wsum = o;
-  o += 4u * (16u + 2u);
+  o += 4u * (32u + 2u);
   return align8(o);
 }
 

@@ -31,7 +31,7 @@ struct BrickParams {
   uint32_t* status;  // device word: bit 0 set when a region exceeds Rmax, bit 1 when a brick exceeds T
   uint32_t off_rcnt, off_rpref, off_roff, off_bpref, off_pos, off_type;  // LDS byte offsets
   uint32_t off_rdel, off_crow;  // staging tables: per region row, per 64 region atoms
-  uint32_t off_wsum;            // 16 scan words, then the 2 wave-group counter words
+  uint32_t off_wsum;            // 32 scan words, then the 2 wave-group counter words
 };
 
 // Per type-pair list radii of a multi-material run: a candidate b of an atom a is kept when
