@@ -1158,9 +1158,11 @@ __device__ __forceinline__ void lj_batch_case(const BoxFrame& f, uint32_t b, uin
 
 // Force of every brick of the listed boxes, all of one precision (one kernel per precision,
 // so the mixed form runs with its own, smaller register budget); the epilogue is per box
-// (uniform per workgroup).  EPI_KICK closes a box's run; EPI_NEXT fuses its next step.
+// (uniform per workgroup).  EPI_KICK closes a box's run; EPI_NEXT fuses its next step.  The
+// register cap of 5 waves per SIMD (the FP64 form otherwise takes 110 VGPRs, 4 waves) measured
+// 3% faster on 256 FP64 boxes; the mixed form fits under it anyway.
 template <bool MIXED>
-__global__ void __launch_bounds__(1024) kb_lj_brick(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) kb_lj_brick(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
                                                     const uint32_t* __restrict__ clock) {
   uint32_t box, b;
   if (!batch_brick(work, box, b)) return;  // whole workgroup
