@@ -131,6 +131,8 @@ void frame_from_ctx(xnb_batch* B, int i, const xnb_run_params& p) {
   f.g = c->g;
   f.bp = c->bp;
   f.bp.status = B->dwords.p + 4 * (size_t)i;
+  f.bpn = f.bp;
+  brick_layout(f.bpn, true);
   f.N = c->N;
   f.G = c->G;
   f.cap = (uint32_t)std::min<size_t>(c->cap, 0xffffffffu);
@@ -234,7 +236,7 @@ std::vector<uint32_t> bricks_table(const std::vector<int>& boxes, const std::vec
   return t;
 }
 
-uint32_t brick_lds_bytes(BrickParams bp) { return brick_layout(bp); }
+uint32_t brick_lds_bytes(BrickParams bp, bool f32_build = false) { return brick_layout(bp, f32_build); }
 
 // Workgroups of a box's bricks that fit one CU at once (LDS bytes and the waves the
 // kernel's registers allow), 1..4.
@@ -260,7 +262,7 @@ void brick_launches(const xnb_batch* B, Tables& T, int kind, const std::vector<i
     for (size_t q = 0; q < boxes.size(); q++) {
       const BatchBox& bb = B->box[boxes[q]];
       if (kind != kBatchNbh && (kind == kBatchLjMixed) != (bb.f.mixed != 0)) continue;
-      const uint32_t lds = brick_lds_bytes(bb.f.bp);
+      const uint32_t lds = brick_lds_bytes(bb.f.bp, kind == kBatchNbh);
       if (B->use_classes && brick_class(bb.f.bp.T, lds + ki.static_lds, B->lds_limit, ki.waves_per_cu) != k) continue;
       sub.push_back(boxes[q]);
       nbr.push_back(bb.f.bp.nbricks);

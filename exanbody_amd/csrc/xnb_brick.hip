@@ -56,6 +56,15 @@ __device__ inline void block_scan_excl2(uint32_t a, uint32_t b, uint32_t& ea, ui
 
 __device__ inline uint32_t wave_max_b(uint32_t v) { return wave_max_u32(v); }
 
+// 2 m + (this lane's bit of lanes), one v_addc with the lane mask (a compare's result) as the
+// carry in; the compiler's select-and-shift takes two instructions.
+__device__ inline uint32_t shift_in(uint32_t m, unsigned long long lanes) {
+  uint32_t r;
+  unsigned long long carry_out;
+  asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry_out) : "v"(m), "s"(lanes));
+  return r;
+}
+
 // largest k in [0, n) with pref[k] <= x (pref non-decreasing, pref[0] = 0 <= x)
 __device__ inline uint32_t upper_index(const uint32_t* pref, uint32_t n, uint32_t x) {
   uint32_t lo = 0, hi = n;  // invariant: pref[lo] <= x, answer in [lo, hi)
@@ -81,6 +90,7 @@ struct BrickLds {
   int32_t* rdel;    // per region row: slot - LDS index of its atoms, or kNoDelta
   uint16_t* crow;   // per 64 region atoms: the row holding the first of them
   double *px, *py, *pz;
+  float4* pf;   // kPosF32 staging (same LDS bytes as px): atom i's position relative to the brick centre
   uint8_t* pt;  // particle type (r.w) of each region atom
   uint32_t* wsum;  // scan scratch (32 words)
   uint32_t* wcnt;  // wave_group_count_add's two words
@@ -97,6 +107,7 @@ __device__ inline BrickLds brick_lds(const BrickParams& bp, char* lds) {
   L.px = (double*)lds;  // at off_pos == 0; interleaved x, y, z: atom i at [3 i, 3 i + 2]
   L.py = L.px + 1;
   L.pz = L.px + 2;
+  L.pf = (float4*)lds;
   L.pt = (uint8_t*)(lds + bp.off_type);
   L.wsum = (uint32_t*)(lds + bp.off_wsum);
   L.wcnt = L.wsum + 32;
@@ -151,14 +162,41 @@ __device__ inline BrickPos brick_pos(const Geom& g, const BrickParams& bp, uint3
   return P;
 }
 
-// Region tables (+ positions when load_pos) into LDS.  Needs nrc <= blockDim.x and
+// What brick_setup stages besides the region tables: nothing, the FP64 positions and types
+// (px/py/pz, pt), or FP32 positions relative to the brick centre (pf, 16 bytes an atom, in the
+// bytes of px) for the neighbour build's prefilter, with the largest magnitude of a staged
+// FP32 coordinate (as float bits) in wcnt[0].
+enum { kPosNone = 0, kPosF64 = 1, kPosF32 = 2 };
+
+// FP64 reference point of a brick's FP32 staging: the centre of the brick's cells.
+__device__ inline void brick_centre(const Geom& g, const BrickPos& P, double& cx, double& cy, double& cz) {
+  cx = cell_origin(g.ox, g.gl, P.gi0, g.cs) + 0.5 * (double)P.ni * g.cs;
+  cy = cell_origin(g.oy, g.gl, P.gj0, g.cs) + 0.5 * (double)P.nj * g.cs;
+  cz = cell_origin(g.oz, g.gl, P.gk0, g.cs) + 0.5 * (double)P.nk * g.cs;
+}
+
+// Slot of region atom idx < R, from the staging tables (row table, then cell by cell in a row
+// of several deltas).
+__device__ inline uint32_t region_slot(const BrickParams& bp, const BrickLds& L, uint32_t idx) {
+  const uint32_t RX = (uint32_t)bp.RX;
+  uint32_t q = L.crow[idx >> 6];
+  while (L.rpref[(q + 1u) * RX] <= idx) q++;
+  const int32_t d = L.rdel[q];
+  if (d != kNoDelta) return (uint32_t)((int32_t)idx + d);
+  uint32_t c = q * RX;
+  while (L.rpref[c + 1u] <= idx) c++;
+  return L.roff[c] + (idx - L.rpref[c]);
+}
+
+// Region tables (+ positions per pos_mode) into LDS.  Needs nrc <= blockDim.x and
 // nbc <= blockDim.x.  Ends with a workgroup barrier.  Returns false (the same for every
 // thread, with bp.status flagged) when the region holds more than Rmax atoms or the brick
 // more than T, the sizes its LDS and lists were laid out for.
 __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const BrickPos& P, const BrickLds& L,
                                    const double4* __restrict__ r, const uint32_t* __restrict__ cell_off,
-                                   const uint32_t* __restrict__ cell_n, bool load_pos) {
+                                   const uint32_t* __restrict__ cell_n, int pos_mode) {
   const uint32_t t = threadIdx.x;
+  if (pos_mode == kPosF32 && t == 0) L.wcnt[0] = 0u;  // ordered before the max by the scan's barrier
   uint32_t cnt = 0;
   if (t < (uint32_t)bp.nrc) {
     const int ri = (int)(t % (uint32_t)bp.RX), rj = (int)((t / (uint32_t)bp.RX) % (uint32_t)bp.RY),
@@ -196,7 +234,7 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
     if (t == 0) atomicOr(bp.status, (fits_r ? 0u : 1u) | (fits_t ? 0u : 2u));
     return false;
   }
-  if (load_pos) {
+  if (pos_mode != kPosNone) {
     const uint32_t R = L.rpref[bp.nrc];
     const uint32_t RX = (uint32_t)bp.RX, nrow = (uint32_t)bp.nrc / RX;
     // Row table.  Region rows are RX cells along x; in a row of inner cells the cells' slots
@@ -219,6 +257,9 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
       for (uint32_t k = (L.rpref[c0] + 63u) >> 6; (k << 6) < s1; k++) L.crow[k] = (uint16_t)t;
     }
     __syncthreads();
+    double cx = 0., cy = 0., cz = 0.;
+    if (pos_mode == kPosF32) brick_centre(g, P, cx, cy, cz);
+    uint32_t cmax = 0;  // largest |staged FP32 coordinate| of this thread, as bits
     // Up to four region atoms per thread and round, every load issued before the first LDS
     // store: one memory round trip per round (a loop that stored each load before issuing the
     // next spent ~0.36 ms of a 0.94 ms mixed force launch at 8.4M atoms staging).
@@ -227,38 +268,43 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const uint32_t idx = base + (uint32_t)u * blockDim.x;
-        if (idx < R) {
-          uint32_t q = L.crow[idx >> 6];
-          while (L.rpref[(q + 1u) * RX] <= idx) q++;
-          const int32_t d = L.rdel[q];
-          uint32_t slot;
-          if (d != kNoDelta) {
-            slot = (uint32_t)((int32_t)idx + d);
-          } else {
-            uint32_t c = q * RX;
-            while (L.rpref[c + 1u] <= idx) c++;
-            slot = L.roff[c] + (idx - L.rpref[c]);
-          }
-          p[u] = r[slot];
-        }
+        if (idx < R) p[u] = r[region_slot(bp, L, idx)];
       }
+      if (pos_mode == kPosF32) {
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const uint32_t idx = base + (uint32_t)u * blockDim.x;
-        if (idx < R) {
-          L.px[3 * idx] = p[u].x;
-          L.py[3 * idx] = p[u].y;
-          L.pz[3 * idx] = p[u].z;
-          L.pt[idx] = (uint8_t)p[u].w;
+        for (int u = 0; u < 4; u++) {
+          const uint32_t idx = base + (uint32_t)u * blockDim.x;
+          if (idx < R) {
+            const float4 c = make_float4((float)(p[u].x - cx), (float)(p[u].y - cy), (float)(p[u].z - cz), 0.f);
+            L.pf[idx] = c;
+            // NaN and infinities compare above every finite magnitude as bits too
+            cmax = max(cmax, max(__float_as_uint(fabsf(c.x)), max(__float_as_uint(fabsf(c.y)), __float_as_uint(fabsf(c.z)))));
+          }
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const uint32_t idx = base + (uint32_t)u * blockDim.x;
+          if (idx < R) {
+            L.px[3 * idx] = p[u].x;
+            L.py[3 * idx] = p[u].y;
+            L.pz[3 * idx] = p[u].z;
+            L.pt[idx] = (uint8_t)p[u].w;
+          }
         }
       }
     }
     if (t < kSentinels) {  // sentinel atoms R .. R + kSentinels - 1: every distance to them exceeds any cutoff
-      L.px[3 * (R + t)] = 1e30;
-      L.py[3 * (R + t)] = 1e30;
-      L.pz[3 * (R + t)] = 1e30;
-      L.pt[R + t] = 0;
+      if (pos_mode == kPosF32) {
+        L.pf[R + t] = make_float4(1e30f, 1e30f, 1e30f, 0.f);
+      } else {
+        L.px[3 * (R + t)] = 1e30;
+        L.py[3 * (R + t)] = 1e30;
+        L.pz[3 * (R + t)] = 1e30;
+        L.pt[R + t] = 0;
+      }
     }
+    if (pos_mode == kPosF32 && cmax) __hip_atomic_fetch_max(&L.wcnt[0], cmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __syncthreads();
   }
   return true;
@@ -325,22 +371,32 @@ __global__ void k_brick_stats(Geom g, BrickParams bp, const uint32_t* __restrict
 // bp.T on hold no atom and touch no list.
 // PAIRWISE (multi-material runs): a candidate within nbh_dist is stored only when it is also
 // within its type pair's list radius (PairCut); its d2 is recomputed as in the test.
-template <bool FILTER, bool PAIRWISE = false>
+// PREF (plain build only): the chunk test runs in FP32 on positions relative to the brick
+// centre and decides every candidate whose FP32 d2 lies outside an error band around
+// nbh_dist^2 (or above a bound near 0); band candidates are tested again in FP64 from the
+// global positions, so the lists are those of the FP64 test.
+template <bool FILTER, bool PAIRWISE = false, bool PREF = false>
 __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams& bp, uint32_t b, const double4* __restrict__ r,
                                                const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
                                                double max2, int half_symmetric, int skip_ghosts, uint16_t* __restrict__ blist,
                                                uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt,
                                                const PairCut* pc = nullptr) {
+  static_assert(!PREF || (!FILTER && !PAIRWISE), "the FP32 prefilter serves the plain build");
   extern __shared__ __align__(16) char lds[];
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
-  if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, true)) return;
+  if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, PREF ? kPosF32 : kPosF64)) return;
   const LaneAtom A = lane_atom(bp, L);
   const uint32_t t = threadIdx.x, T = bp.T;
   uint16_t* out = blist + (size_t)b * bp.cap * T;
   // every lane, live or not, tests against its own LDS atom when past the candidate cell's
   // end (A.ia = 0 for lanes without an atom), so d2 == 0 rejects those candidates
-  const double ax = L.px[3 * A.ia], ay = L.py[3 * A.ia], az = L.pz[3 * A.ia];
+  double ax = 0., ay = 0., az = 0.;
+  if (!PREF) {
+    ax = L.px[3 * A.ia];
+    ay = L.py[3 * A.ia];
+    az = L.pz[3 * A.ia];
+  }
   uint32_t ca = 0;
   if (A.live) {
     if (FILTER) {
@@ -365,7 +421,98 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
     }
     cut2 = fmin(cut2, max2);
   }
-  if constexpr (!FILTER) {
+  if constexpr (PREF) {
+    // Error band of the FP32 test.  Staged coordinates are fl32(fl64(x - centre)) with
+    // |.| <= C (the brick's largest, widened by 2^-20 for the rounding of its bits), so each
+    // FP32 difference is off the exact one by at most E = 4.01 u C (u = 2^-24, both staged
+    // coordinates and the subtraction; |dx| <= 2 C), and the FP32 d2 (one product, two fmas)
+    // by at most 2 sqrt(3) s E + 3 E^2 + 3.03 u s^2 at a true distance s.  Both sides grow
+    // with s slower than s^2 - R^2 for s >= R, so d2f < R^2 - H proves s < R and
+    // d2f > R^2 + H proves s > R for H = 2 (2 sqrt(3) R E + 3 E^2 + 4 u R^2); d2f > 16 E^2
+    // proves s > 2.2 E, far from FP64 underflow, so the FP64 d2 is > 0.  The FP64 d2 differs
+    // from s^2 by a relative 2^-51 at most, far inside the factor 2 of H.
+    const float cf = __uint_as_float(L.wcnt[0]);
+    float lo = -1.f, hi = __builtin_inff(), zf = 0.f;  // non-finite positions: every candidate in the band
+    if (cf <= 3.0e38f) {
+      const double u = 0x1p-24;
+      const double C = (double)cf * (1.0 + 0x1p-20);
+      const double E = 4.01 * u * C;
+      const double Rc = sqrt(max2);
+      const double H = 2.0 * (2.0 * 1.7320508075688772 * Rc * E + 3.0 * E * E + 4.0 * u * max2);
+      lo = __double2float_rd(max2 - H);
+      hi = __double2float_ru(max2 + H);
+      zf = __double2float_ru(16.0 * E * E);
+    }
+    const float4 af = L.pf[A.ia];
+    const uint32_t R = L.rpref[bp.nrc];
+    char* const ob = reinterpret_cast<char*>(out);
+    uint32_t wo = 8u * t;
+    const uint32_t wstep = 8u * T - 6u;
+    for (int dk = -1; dk <= 1; dk++)
+      for (int dj = -1; dj <= 1; dj++) {
+        const int rc0 = (int)A.rc + (dk * bp.RY + dj) * bp.RX - 1;
+        const uint32_t rbase = A.live ? L.rpref[rc0] : R;
+        const uint32_t rlen = A.live ? L.rpref[rc0 + 3] - rbase : 0u;
+        const uint32_t lmax = wave_max_b(rlen);
+        for (uint32_t p0 = 0; p0 < lmax; p0 += 32) {
+          const uint32_t np = min((lmax - p0 + 3) & ~3u, 32u);
+          const uint32_t lb = p0 < rlen ? rbase + p0 : R;
+          const float4* pp = L.pf + lb;
+          uint32_t mhi = 0, mlo = 0;  // bit 31 - k: candidate k within R^2 + H; surely kept
+          for (uint32_t q = 0; q < np; q += 4) {
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++) {
+              const float4 c = pp[q + u];
+              const float dx = af.x - c.x, dy = af.y - c.y, dz = af.z - c.z;
+              const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+              mhi = shift_in(mhi, __builtin_amdgcn_ballot_w64(d2 <= hi));
+              mlo = shift_in(mlo, __builtin_amdgcn_ballot_w64(d2 > zf) & __builtin_amdgcn_ballot_w64(d2 < lo));
+            }
+          }
+          if (np < 32) {
+            mhi <<= 32 - np;
+            mlo <<= 32 - np;
+          }
+          const uint32_t own = p0 < rlen ? rlen - p0 : 0u;
+          if (own < 32) {
+            const uint32_t keep = ~(~0u >> own);
+            mhi &= keep;
+            mlo &= keep;
+          }
+          const uint32_t sp = A.ia - lb;  // the lane's own atom (d2 == 0, never kept)
+          if (sp < 32u) mhi &= ~(0x80000000u >> sp);
+          uint32_t m = mlo;
+          uint32_t band = mhi & ~mlo;
+          if (__builtin_expect(__any(band != 0u), 0)) {
+            if (band) {
+              const double4 pa = r[A.slot];
+              while (band) {
+                const uint32_t k = (uint32_t)__builtin_clz(band);
+                band ^= 0x80000000u >> k;
+                const double4 pb = r[region_slot(bp, L, lb + k)];
+                const double dx = pa.x - pb.x;
+                const double dy = pa.y - pb.y;
+                const double dz = pa.z - pb.z;
+                const double d2 = (dx * dx + dy * dy) + dz * dz;
+                if (d2 > 0.0 && d2 <= max2) m |= 0x80000000u >> k;
+              }
+            }
+          }
+          const uint32_t pcnt = (uint32_t)__popc(m);
+          if (cnt + pcnt > bp.cap) {
+            cnt += pcnt;
+            m = 0;
+          }
+          while (m) {
+            const uint32_t k = (uint32_t)__builtin_clz(m);
+            m ^= 0x80000000u >> k;
+            *reinterpret_cast<uint16_t*>(ob + wo) = (uint16_t)(lb + k);
+            ++cnt;
+            wo += (cnt & 3u) ? 2u : wstep;
+          }
+        }
+      }
+  } else if constexpr (!FILTER) {
     // Rows: the cells di = -1, 0, 1 of one (dk, dj) are consecutive in LDS, so a row is one
     // candidate range [rbase, rbase + rlen).  A chunk tests 4-aligned candidates p0 + k
     // (k < np, np uniform) against every lane; bit 31 - k of m is candidate k.  A lane whose
@@ -485,14 +632,14 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
   for (uint32_t q = min(cnt, bp.cap); q < pad_end; q++) out[lpos(q, t, T)] = sentinel;
 }
 
-template <bool FILTER, bool PAIRWISE>
+template <bool FILTER, bool PAIRWISE, bool PREF = false>
 __global__ void __launch_bounds__(1024) k_nbh_brick_mask(Geom g, BrickParams bp, const double4* __restrict__ r,
                                                          const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
                                                          double max2, int half_symmetric, int skip_ghosts, uint16_t* __restrict__ blist,
                                                          uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt, PairCut pc) {
   const uint32_t b = brick_of_block<true>(bp);
   if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
-  nbh_brick_body<FILTER, PAIRWISE>(g, bp, b, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist, bcnt, max_cnt, &pc);
+  nbh_brick_body<FILTER, PAIRWISE, PREF>(g, bp, b, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts, blist, bcnt, max_cnt, &pc);
 }
 
 // ------------------------------------------------------------------ pair traversal
@@ -539,7 +686,7 @@ __device__ __forceinline__ void compute_pair_brick_body(const Geom& g, const Bri
   // the list length is issued before the staging so its latency overlaps it (the build wrote
   // 0 for every lane of the brick without an atom)
   const uint32_t n_list = t < T ? bcnt[(size_t)b * T + t] : 0u;
-  if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, true)) return;
+  if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, kPosF64)) return;
   const LaneAtom At = lane_atom(bp, L);
   const uint32_t n = At.live ? n_list : 0u;
   const uint32_t nmax = wave_max_b(n);
@@ -708,7 +855,7 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
     L.wcnt[0] = 0;
     L.wcnt[1] = 0;
   }
-  if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, true)) return;
+  if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, kPosF64)) return;
   const LaneAtom At = lane_atom(bp, L);
   const uint32_t t = threadIdx.x, T = bp.T;
   const uint32_t n = At.live ? bcnt[(size_t)b * T + t] : 0u;
@@ -917,7 +1064,7 @@ __global__ void __launch_bounds__(1024) k_brick_to_slots(Geom g, BrickParams bp,
   if (b >= bp.nbricks) return;  // whole workgroup, before any barrier
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
-  if (!brick_setup(g, bp, P, L, nullptr, cell_off, cell_n, false)) return;
+  if (!brick_setup(g, bp, P, L, nullptr, cell_off, cell_n, kPosNone)) return;
   const LaneAtom A = lane_atom(bp, L);
   if (!A.live) return;
   const uint32_t t = threadIdx.x, T = bp.T;
@@ -954,13 +1101,15 @@ BrickParams brick_params(const Geom& g, int Bi, int Bj, int Bk) {
 static inline uint32_t align8(uint32_t x) { return (x + 7u) & ~7u; }
 
 // LDS layout of the build, force and conversion kernels; returns its bytes
-uint32_t brick_layout(BrickParams& bp) {
+uint32_t brick_layout(BrickParams& bp, bool f32_build) {
   // positions first, at the start of dynamic LDS, so their addresses need no runtime base
   uint32_t o = 0;
   bp.off_pos = o;
-  o += 24u * (bp.Rmax + kSentinels);  // region atoms + the sentinel block
+  // region atoms + the sentinel block: FP64 x, y, z and a type byte, or the prefiltered
+  // build's FP32 float4 (no types)
+  o += (f32_build ? 16u : 24u) * (bp.Rmax + kSentinels);
   bp.off_type = o;
-  o = align8(o + bp.Rmax + kSentinels);
+  if (!f32_build) o = align8(o + bp.Rmax + kSentinels);
   bp.off_rcnt = o;
   o = align8(o + 4u * bp.nrc);
   bp.off_rpref = o;
@@ -1004,6 +1153,16 @@ static hipError_t set_lds(const void* fn, uint32_t bytes) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+// The plain build runs the FP32 prefilter unless XNB_NBH_PREFILTER=0 (the all-FP64 test; the
+// lists are the same either way).
+static bool nbh_prefilter() {
+  static const bool on = [] {
+    const char* v = getenv("XNB_NBH_PREFILTER");
+    return !(v && v[0] == '0');
+  }();
+  return on;
+}
+
 // pc.nt > 0: per type-pair list radii (not with the half_symmetric / skip_ghosts filters)
 hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n,
                             double max2, const PairCut& pc, int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt,
@@ -1021,6 +1180,13 @@ hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, con
     if (e != hipSuccess) return e;
     k_nbh_brick_mask<false, true><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, 0, 0, blist, bcnt, max_cnt,
                                                                         pc);
+  } else if (nbh_prefilter()) {
+    BrickParams bq = bp;  // the same bricks and lists, the prefilter's smaller LDS layout
+    const uint32_t lds_b = brick_layout(bq, true);
+    e = set_lds((const void*)k_nbh_brick_mask<false, false, true>, lds_b);
+    if (e != hipSuccess) return e;
+    k_nbh_brick_mask<false, false, true><<<kXcds * bq.per_xcd, bq.T, lds_b, st>>>(g, bq, r, cell_off, cell_n, max2, 0, 0, blist, bcnt,
+                                                                                 max_cnt, pc);
   } else {
     e = set_lds((const void*)k_nbh_brick_mask<false, false>, lds);
     if (e != hipSuccess) return e;
@@ -1142,7 +1308,7 @@ __global__ void __launch_bounds__(1024) kb_nbh_brick(const BoxFrame* __restrict_
   // reach past the slot arrays, so the box is left to the one-box recovery
   if (f.words[0] & 4u) return;
   const double4* r = f.rbuf[(f.rcur ^ par) & 1u];
-  nbh_brick_body<false>(f.g, f.bp, b, r, f.cell_off, f.cell_n, f.max2, 0, 0, f.blist, f.bcnt, f.words + 1);
+  nbh_brick_body<false, false, true>(f.g, f.bpn, b, r, f.cell_off, f.cell_n, f.max2, 0, 0, f.blist, f.bcnt, f.words + 1);
 }
 
 template <bool MIXED, int EPI>

@@ -75,6 +75,7 @@ struct PState {
 struct BoxFrame {
   Geom g;
   BrickParams bp;  // bp.status == words + 0
+  BrickParams bpn;  // bp with the prefiltered neighbour build's LDS layout
   uint32_t N, G, cap;
   uint32_t ab;     // 1: the roles of the A and B field arrays are exchanged
   uint32_t rcur;   // current positions are rbuf[rcur ^ parity] (parity: argument of the launch)
@@ -213,7 +214,9 @@ void launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t st)
 void launch_fp64_core(const double* x, size_t n, double* sq, double* rc, hipStream_t st);
 
 BrickParams brick_params(const Geom& g, int Bi, int Bj, int Bk);
-uint32_t brick_layout(BrickParams& bp);
+// LDS layout of the brick kernels (offsets into bp), its bytes; f32_build: the prefiltered
+// neighbour build's (FP32 positions, no types)
+uint32_t brick_layout(BrickParams& bp, bool f32_build = false);
 uint32_t brick_list_granule();
 void launch_brick_stats(const Geom& g, const BrickParams& bp, const uint32_t* cell_n, uint32_t* out, hipStream_t st);
 hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, double max2,
