@@ -445,9 +445,10 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
     }
     const float4 af = L.pf[A.ia];
     const uint32_t R = L.rpref[bp.nrc];
-    char* const ob = reinterpret_cast<char*>(out);
-    uint32_t wo = 8u * t;
-    const uint32_t wstep = 8u * T - 6u;
+    // Entries go through a 64-bit shift register (four u16, the oldest lowest) and are stored a
+    // group of four at a time: group j of the lane is the 8 bytes at grp[j * T].
+    uint2* const grp = reinterpret_cast<uint2*>(out) + t;
+    uint32_t plo = 0, phi = 0;
     for (int dk = -1; dk <= 1; dk++)
       for (int dj = -1; dj <= 1; dj++) {
         const int rc0 = (int)A.rc + (dk * bp.RY + dj) * bp.RX - 1;
@@ -506,12 +507,33 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
           while (m) {
             const uint32_t k = (uint32_t)__builtin_clz(m);
             m ^= 0x80000000u >> k;
-            *reinterpret_cast<uint16_t*>(ob + wo) = (uint16_t)(lb + k);
+            plo = __builtin_amdgcn_alignbit(phi, plo, 16);
+            phi = __builtin_amdgcn_alignbit(lb + k, phi, 16);
             ++cnt;
-            wo += (cnt & 3u) ? 2u : wstep;
+            if ((cnt & 3u) == 0u) grp[(size_t)((cnt >> 2) - 1u) * T] = make_uint2(plo, phi);
           }
         }
       }
+    // close the lane's list: the partial group and whole groups of the sentinel up to the
+    // wave's batch boundary, so force kernels need no per-lane bound inside a batch
+    if (!A.live) cnt = 0;
+    if (t < T) bcnt[(size_t)b * T + t] = cnt;
+    const uint32_t wmax = wave_max_b(cnt);
+    if ((t & 63) == 0 && wmax > 0) atomicMax(max_cnt, wmax);
+    const uint32_t pad_end = min((wmax + kListBatch - 1) / kListBatch * kListBatch, bp.cap);
+    const uint32_t sentinel = L.rpref[bp.nrc];
+    uint32_t q = min(cnt, bp.cap);
+    if (q < pad_end && (q & 3u)) {
+      while (q & 3u) {
+        plo = __builtin_amdgcn_alignbit(phi, plo, 16);
+        phi = __builtin_amdgcn_alignbit(sentinel, phi, 16);
+        ++q;
+      }
+      grp[(size_t)((q >> 2) - 1u) * T] = make_uint2(plo, phi);
+    }
+    const uint32_t s2 = sentinel | (sentinel << 16);
+    for (; q < pad_end; q += 4) grp[(size_t)(q >> 2) * T] = make_uint2(s2, s2);
+    return;
   } else if constexpr (!FILTER) {
     // Rows: the cells di = -1, 0, 1 of one (dk, dj) are consecutive in LDS, so a row is one
     // candidate range [rbase, rbase + rlen).  A chunk tests 4-aligned candidates p0 + k
