@@ -275,7 +275,8 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
         for (int u = 0; u < 4; u++) {
           const uint32_t idx = base + (uint32_t)u * blockDim.x;
           if (idx < R) {
-            const float4 c = make_float4((float)(p[u].x - cx), (float)(p[u].y - cy), (float)(p[u].z - cz), 0.f);
+            const float4 c = make_float4((float)(p[u].x - cx), (float)(p[u].y - cy), (float)(p[u].z - cz),
+                                         __uint_as_float((uint32_t)(uint8_t)p[u].w));  // w: the type's bits
             L.pf[idx] = c;
             // NaN and infinities compare above every finite magnitude as bits too
             cmax = max(cmax, max(__float_as_uint(fabsf(c.x)), max(__float_as_uint(fabsf(c.y)), __float_as_uint(fabsf(c.z)))));
@@ -371,17 +372,19 @@ __global__ void k_brick_stats(Geom g, BrickParams bp, const uint32_t* __restrict
 // bp.T on hold no atom and touch no list.
 // PAIRWISE (multi-material runs): a candidate within nbh_dist is stored only when it is also
 // within its type pair's list radius (PairCut); its d2 is recomputed as in the test.
-// PREF (plain build only): the chunk test runs in FP32 on positions relative to the brick
-// centre and decides every candidate whose FP32 d2 lies outside an error band around
-// nbh_dist^2 (or above a bound near 0); band candidates are tested again in FP64 from the
-// global positions, so the lists are those of the FP64 test.
+// PREF (unfiltered builds): the tests run in FP32 on positions relative to the brick centre
+// and decide every candidate whose FP32 d2 lies outside an error band around its threshold
+// (or above a bound near 0); band candidates are tested again in FP64 from the global
+// positions, so the lists are those of the FP64 test.  With PAIRWISE the chunk test keeps a
+// superset at the atom's largest pair radius and each candidate is decided against its own
+// pair's radius while the mask is expanded.
 template <bool FILTER, bool PAIRWISE = false, bool PREF = false>
 __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams& bp, uint32_t b, const double4* __restrict__ r,
                                                const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
                                                double max2, int half_symmetric, int skip_ghosts, uint16_t* __restrict__ blist,
                                                uint32_t* __restrict__ bcnt, uint32_t* __restrict__ max_cnt,
                                                const PairCut* pc = nullptr) {
-  static_assert(!PREF || (!FILTER && !PAIRWISE), "the FP32 prefilter serves the plain build");
+  static_assert(!PREF || !FILTER, "the FP32 prefilter serves the unfiltered builds");
   extern __shared__ __align__(16) char lds[];
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
@@ -412,7 +415,7 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
   // nbh_dist), so candidates beyond every radius of its type are not expanded and re-tested
   double cut2 = max2;
   if (PAIRWISE) {
-    const unsigned ta = min((unsigned)L.pt[A.ia], 3u);
+    const unsigned ta = min(PREF ? __float_as_uint(L.pf[A.ia].w) : (unsigned)L.pt[A.ia], 3u);
     cut2 = 0.0;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -427,21 +430,33 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
     // FP32 difference is off the exact one by at most E = 4.01 u C (u = 2^-24, both staged
     // coordinates and the subtraction; |dx| <= 2 C), and the FP32 d2 (one product, two fmas)
     // by at most 2 sqrt(3) s E + 3 E^2 + 3.03 u s^2 at a true distance s.  Both sides grow
-    // with s slower than s^2 - R^2 for s >= R, so d2f < R^2 - H proves s < R and
-    // d2f > R^2 + H proves s > R for H = 2 (2 sqrt(3) R E + 3 E^2 + 4 u R^2); d2f > 16 E^2
-    // proves s > 2.2 E, far from FP64 underflow, so the FP64 d2 is > 0.  The FP64 d2 differs
-    // from s^2 by a relative 2^-51 at most, far inside the factor 2 of H.
+    // with s slower than s^2 - T for s^2 >= T, so for a threshold T (= R^2), d2f < T - H
+    // proves s^2 < T and d2f > T + H proves s^2 > T with H = 2 (2 sqrt(3) R E + 3 E^2 +
+    // 4 u R^2); d2f > 16 E^2 proves s > 2.2 E, far from FP64 underflow, so the FP64 d2 is
+    // > 0.  The FP64 d2 differs from s^2 by a relative 2^-51 at most, far inside the factor 2
+    // of H.  Non-finite positions give an unbounded band: every candidate is tested in FP64.
     const float cf = __uint_as_float(L.wcnt[0]);
-    float lo = -1.f, hi = __builtin_inff(), zf = 0.f;  // non-finite positions: every candidate in the band
-    if (cf <= 3.0e38f) {
-      const double u = 0x1p-24;
-      const double C = (double)cf * (1.0 + 0x1p-20);
-      const double E = 4.01 * u * C;
-      const double Rc = sqrt(max2);
-      const double H = 2.0 * (2.0 * 1.7320508075688772 * Rc * E + 3.0 * E * E + 4.0 * u * max2);
-      lo = __double2float_rd(max2 - H);
-      hi = __double2float_ru(max2 + H);
-      zf = __double2float_ru(16.0 * E * E);
+    const bool finite = cf <= 3.0e38f;
+    const double u = 0x1p-24;
+    const double E = 4.01 * u * ((double)cf * (1.0 + 0x1p-20));
+    auto band_lo = [&](double T2) {
+      const double H = 2.0 * (2.0 * 1.7320508075688772 * sqrt(T2) * E + 3.0 * E * E + 4.0 * u * T2);
+      return finite ? __double2float_rd(T2 - H) : -1.f;
+    };
+    auto band_hi = [&](double T2) {
+      const double H = 2.0 * (2.0 * 1.7320508075688772 * sqrt(T2) * E + 3.0 * E * E + 4.0 * u * T2);
+      return finite ? __double2float_ru(T2 + H) : __builtin_inff();
+    };
+    const float zf = finite ? __double2float_ru(16.0 * E * E) : 0.f;
+    // the chunk test: the plain build decides at nbh_dist; a per-pair-radius build keeps a
+    // superset at this atom's largest radius and decides each candidate against its own
+    // pair's radius while expanding
+    const float lo = band_lo(cut2), hi = band_hi(cut2);
+    float tlo[4], thi[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      tlo[q] = PAIRWISE ? band_lo(thr[q]) : 0.f;
+      thi[q] = PAIRWISE ? band_hi(thr[q]) : 0.f;
     }
     const float4 af = L.pf[A.ia];
     const uint32_t R = L.rpref[bp.nrc];
@@ -449,6 +464,8 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
     // group of four at a time: group j of the lane is the 8 bytes at grp[j * T].
     uint2* const grp = reinterpret_cast<uint2*>(out) + t;
     uint32_t plo = 0, phi = 0;
+    double4 pa;  // the lane's FP64 position, read for its first FP64 test
+    bool have_pa = false;
     for (int dk = -1; dk <= 1; dk++)
       for (int dj = -1; dj <= 1; dj++) {
         const int rc0 = (int)A.rc + (dk * bp.RY + dj) * bp.RX - 1;
@@ -459,15 +476,15 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
           const uint32_t np = min((lmax - p0 + 3) & ~3u, 32u);
           const uint32_t lb = p0 < rlen ? rbase + p0 : R;
           const float4* pp = L.pf + lb;
-          uint32_t mhi = 0, mlo = 0;  // bit 31 - k: candidate k within R^2 + H; surely kept
+          uint32_t mhi = 0, mlo = 0;  // bit 31 - k: candidate k within the chunk radius + H; surely kept
           for (uint32_t q = 0; q < np; q += 4) {
 #pragma unroll
-            for (uint32_t u = 0; u < 4; u++) {
-              const float4 c = pp[q + u];
+            for (uint32_t w = 0; w < 4; w++) {
+              const float4 c = pp[q + w];
               const float dx = af.x - c.x, dy = af.y - c.y, dz = af.z - c.z;
               const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
               mhi = shift_in(mhi, __builtin_amdgcn_ballot_w64(d2 <= hi));
-              mlo = shift_in(mlo, __builtin_amdgcn_ballot_w64(d2 > zf) & __builtin_amdgcn_ballot_w64(d2 < lo));
+              if (!PAIRWISE) mlo = shift_in(mlo, __builtin_amdgcn_ballot_w64(d2 > zf) & __builtin_amdgcn_ballot_w64(d2 < lo));
             }
           }
           if (np < 32) {
@@ -482,35 +499,69 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
           }
           const uint32_t sp = A.ia - lb;  // the lane's own atom (d2 == 0, never kept)
           if (sp < 32u) mhi &= ~(0x80000000u >> sp);
-          uint32_t m = mlo;
-          uint32_t band = mhi & ~mlo;
-          if (__builtin_expect(__any(band != 0u), 0)) {
-            if (band) {
-              const double4 pa = r[A.slot];
-              while (band) {
-                const uint32_t k = (uint32_t)__builtin_clz(band);
-                band ^= 0x80000000u >> k;
-                const double4 pb = r[region_slot(bp, L, lb + k)];
-                const double dx = pa.x - pb.x;
-                const double dy = pa.y - pb.y;
-                const double dz = pa.z - pb.z;
-                const double d2 = (dx * dx + dy * dy) + dz * dz;
-                if (d2 > 0.0 && d2 <= max2) m |= 0x80000000u >> k;
+          uint32_t m = PAIRWISE ? mhi : mlo;
+          if (!PAIRWISE) {
+            uint32_t band = mhi & ~mlo;
+            if (__builtin_expect(__any(band != 0u), 0)) {
+              if (band) {
+                if (!have_pa) pa = r[A.slot];
+                have_pa = true;
+                while (band) {
+                  const uint32_t k = (uint32_t)__builtin_clz(band);
+                  band ^= 0x80000000u >> k;
+                  const double4 pb = r[region_slot(bp, L, lb + k)];
+                  const double dx = pa.x - pb.x;
+                  const double dy = pa.y - pb.y;
+                  const double dz = pa.z - pb.z;
+                  const double d2 = (dx * dx + dy * dy) + dz * dz;
+                  if (d2 > 0.0 && d2 <= max2) m |= 0x80000000u >> k;
+                }
               }
             }
-          }
-          const uint32_t pcnt = (uint32_t)__popc(m);
-          if (cnt + pcnt > bp.cap) {
-            cnt += pcnt;
-            m = 0;
+            // a list that would pass cap stores nothing more (its count still grows, so the
+            // caller rebuilds with a larger cap)
+            const uint32_t pcnt = (uint32_t)__popc(m);
+            if (cnt + pcnt > bp.cap) {
+              cnt += pcnt;
+              m = 0;
+            }
           }
           while (m) {
             const uint32_t k = (uint32_t)__builtin_clz(m);
             m ^= 0x80000000u >> k;
+            if (PAIRWISE) {
+              // kept iff 0 < d2 <= the radius^2 of this pair (types above 3 use type 0's)
+              const uint32_t ib = lb + k;
+              const float4 c = L.pf[ib];
+              const float dx = af.x - c.x, dy = af.y - c.y, dz = af.z - c.z;
+              const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+              const uint32_t tb = __float_as_uint(c.w);
+              float tl = tlo[0], th = thi[0];
+#pragma unroll
+              for (int q = 1; q < 4; q++) {
+                tl = tb == (uint32_t)q ? tlo[q] : tl;
+                th = tb == (uint32_t)q ? thi[q] : th;
+              }
+              bool keep = d2 > zf && d2 < tl;
+              if (!keep && d2 <= th) {
+                if (!have_pa) pa = r[A.slot];
+                have_pa = true;
+                const double4 pb = r[region_slot(bp, L, ib)];
+                const double ex = pa.x - pb.x;
+                const double ey = pa.y - pb.y;
+                const double ez = pa.z - pb.z;
+                const double e2 = (ex * ex + ey * ey) + ez * ez;
+                double lim = thr[0];
+#pragma unroll
+                for (int q = 1; q < 4; q++) lim = tb == (uint32_t)q ? thr[q] : lim;
+                keep = e2 > 0.0 && e2 <= lim;
+              }
+              if (!keep) continue;
+            }
             plo = __builtin_amdgcn_alignbit(phi, plo, 16);
             phi = __builtin_amdgcn_alignbit(lb + k, phi, 16);
             ++cnt;
-            if ((cnt & 3u) == 0u) grp[(size_t)((cnt >> 2) - 1u) * T] = make_uint2(plo, phi);
+            if ((cnt & 3u) == 0u && cnt <= bp.cap) grp[(size_t)((cnt >> 2) - 1u) * T] = make_uint2(plo, phi);
           }
         }
       }
@@ -1175,8 +1226,8 @@ static hipError_t set_lds(const void* fn, uint32_t bytes) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-// The plain build runs the FP32 prefilter unless XNB_NBH_PREFILTER=0 (the all-FP64 test; the
-// lists are the same either way).
+// The unfiltered builds run the FP32 prefilter unless XNB_NBH_PREFILTER=0 (the all-FP64
+// test; the lists are the same either way).
 static bool nbh_prefilter() {
   static const bool on = [] {
     const char* v = getenv("XNB_NBH_PREFILTER");
@@ -1197,6 +1248,13 @@ hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, con
     if (e != hipSuccess) return e;
     k_nbh_brick_mask<true, false><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts,
                                                                         blist, bcnt, max_cnt, pc);
+  } else if (pc.nt > 0 && nbh_prefilter()) {
+    BrickParams bq = bp;
+    const uint32_t lds_b = brick_layout(bq, true);
+    e = set_lds((const void*)k_nbh_brick_mask<false, true, true>, lds_b);
+    if (e != hipSuccess) return e;
+    k_nbh_brick_mask<false, true, true><<<kXcds * bq.per_xcd, bq.T, lds_b, st>>>(g, bq, r, cell_off, cell_n, max2, 0, 0, blist, bcnt,
+                                                                                max_cnt, pc);
   } else if (pc.nt > 0) {
     e = set_lds((const void*)k_nbh_brick_mask<false, true>, lds);
     if (e != hipSuccess) return e;
