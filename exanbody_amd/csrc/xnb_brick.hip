@@ -81,6 +81,15 @@ static constexpr int LJB_UNROLL_B = (int)kListBatch;  // list entries per force 
 // the build reads whole candidate chunks of 32 from a row start and needs 32 of them
 static constexpr uint32_t kSentinels = 32;
 static constexpr int32_t kNoDelta = INT32_MIN;  // a region row whose cells' slots do not follow each other
+// region atoms a thread stages per memory round.  A 4x4x2 brick's region holds about 4.2
+// times its own atoms (the workgroup's lanes), so four a round leave a second round trip for
+// the last few; five or six a round measured the same for the mixed force kernel (other
+// workgroups hide the round trip), 2 % faster for the FP64 one (six) and 8 % slower for the
+// prefiltered build (six, more registers).
+#ifndef XNB_STAGE_ROUND
+#define XNB_STAGE_ROUND 4
+#endif
+static constexpr int kStageRound = XNB_STAGE_ROUND;
 
 // position of entry q of lane t in a brick's list slab (T lanes)
 __device__ inline size_t lpos(uint32_t q, uint32_t t, uint32_t T) { return ((size_t)(q >> 2) * T + t) * 4 + (q & 3u); }
@@ -260,19 +269,19 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
     double cx = 0., cy = 0., cz = 0.;
     if (pos_mode == kPosF32) brick_centre(g, P, cx, cy, cz);
     uint32_t cmax = 0;  // largest |staged FP32 coordinate| of this thread, as bits
-    // Up to four region atoms per thread and round, every load issued before the first LDS
-    // store: one memory round trip per round (a loop that stored each load before issuing the
-    // next spent ~0.36 ms of a 0.94 ms mixed force launch at 8.4M atoms staging).
-    for (uint32_t base = t; base < R; base += 4u * blockDim.x) {
-      double4 p[4];
+    // Up to kStageRound region atoms per thread and round, every load issued before the first
+    // LDS store: one memory round trip per round (a loop that stored each load before issuing
+    // the next spent ~0.36 ms of a 0.94 ms mixed force launch at 8.4M atoms staging).
+    for (uint32_t base = t; base < R; base += (uint32_t)kStageRound * blockDim.x) {
+      double4 p[kStageRound];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+      for (int u = 0; u < kStageRound; u++) {
         const uint32_t idx = base + (uint32_t)u * blockDim.x;
         if (idx < R) p[u] = r[region_slot(bp, L, idx)];
       }
       if (pos_mode == kPosF32) {
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < kStageRound; u++) {
           const uint32_t idx = base + (uint32_t)u * blockDim.x;
           if (idx < R) {
             const float4 c = make_float4((float)(p[u].x - cx), (float)(p[u].y - cy), (float)(p[u].z - cz),
@@ -284,7 +293,7 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
         }
       } else {
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < kStageRound; u++) {
           const uint32_t idx = base + (uint32_t)u * blockDim.x;
           if (idx < R) {
             L.px[3 * idx] = p[u].x;
