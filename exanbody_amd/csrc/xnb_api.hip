@@ -376,7 +376,7 @@ int op_nbh_brick(xnb_ctx* c, double nbh_dist, const xnb_chunk_neighbors_config& 
     HIPCHK(c, hipMemsetAsync(c->u32s.p + 6, 0, sizeof(uint32_t), c->st));
     tstart(c, XNB_TIMER_NBH);
     hipError_t he = launch_nbh_brick(c->g, bp, c->A.r, c->cell_off.p, c->cell_n.p, max2, pc, cfg.half_symmetric, cfg.skip_ghosts,
-                                     c->blist.p, c->bcnt.p, c->u32s.p + 2, c->st);
+                                     c->nbh_prefilter, c->blist.p, c->bcnt.p, c->u32s.p + 2, c->st);
     tstop(c);
     if (he != hipSuccess) return fail(c, XNB_E_HIP, std::string("brick neighbour build: ") + hipGetErrorString(he));
     uint32_t w[5];  // u32s[2]: longest list, u32s[6]: brick status
@@ -599,6 +599,7 @@ int xnb_create(xnb_ctx** out, const xnb_domain* d, int device) {
     else std::sscanf(v, "%d,%d,%d", &c->brick_dims[0], &c->brick_dims[1], &c->brick_dims[2]);
   }
   if (const char* v = std::getenv("XNB_FUSE_PUSH")) c->fuse_push = v[0] != '0';
+  if (const char* v = std::getenv("XNB_NBH_PREFILTER")) c->nbh_prefilter = v[0] != '0';
   HIPCHK(c, hipMemsetAsync(c->cell_n.p, 0, c->cell_n.n * sizeof(uint32_t), c->st));
   HIPCHK(c, hipMemsetAsync(c->cell_off.p, 0, c->cell_off.n * sizeof(uint32_t), c->st));
   *out = c;

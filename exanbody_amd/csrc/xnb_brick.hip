@@ -1235,20 +1235,10 @@ static hipError_t set_lds(const void* fn, uint32_t bytes) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-// The unfiltered builds run the FP32 prefilter unless XNB_NBH_PREFILTER=0 (the all-FP64
-// test; the lists are the same either way).
-static bool nbh_prefilter() {
-  static const bool on = [] {
-    const char* v = getenv("XNB_NBH_PREFILTER");
-    return !(v && v[0] == '0');
-  }();
-  return on;
-}
-
 // pc.nt > 0: per type-pair list radii (not with the half_symmetric / skip_ghosts filters)
 hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n,
-                            double max2, const PairCut& pc, int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt,
-                            uint32_t* max_cnt, hipStream_t st) {
+                            double max2, const PairCut& pc, int half_symmetric, int skip_ghosts, bool prefilter, uint16_t* blist,
+                            uint32_t* bcnt, uint32_t* max_cnt, hipStream_t st) {
   const uint32_t lds = brick_layout(bp);
   hipError_t e;
   if (half_symmetric || skip_ghosts) {
@@ -1257,7 +1247,7 @@ hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, con
     if (e != hipSuccess) return e;
     k_nbh_brick_mask<true, false><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, half_symmetric, skip_ghosts,
                                                                         blist, bcnt, max_cnt, pc);
-  } else if (pc.nt > 0 && nbh_prefilter()) {
+  } else if (pc.nt > 0 && prefilter) {
     BrickParams bq = bp;
     const uint32_t lds_b = brick_layout(bq, true);
     e = set_lds((const void*)k_nbh_brick_mask<false, true, true>, lds_b);
@@ -1269,7 +1259,7 @@ hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, con
     if (e != hipSuccess) return e;
     k_nbh_brick_mask<false, true><<<kXcds * bp.per_xcd, bp.T, lds, st>>>(g, bp, r, cell_off, cell_n, max2, 0, 0, blist, bcnt, max_cnt,
                                                                         pc);
-  } else if (nbh_prefilter()) {
+  } else if (prefilter) {
     BrickParams bq = bp;  // the same bricks and lists, the prefilter's smaller LDS layout
     const uint32_t lds_b = brick_layout(bq, true);
     e = set_lds((const void*)k_nbh_brick_mask<false, false, true>, lds_b);

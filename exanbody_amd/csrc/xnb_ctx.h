@@ -93,6 +93,7 @@ struct xnb_ctx {
   DArr<uint32_t> bcnt;
   bool slot_lists_current = false;  // nbr/nbr_cnt mirror the brick lists
   bool fuse_push = true;            // force kernel performs the next step's first half (brick lists)
+  bool nbh_prefilter = true;        // unfiltered brick builds test in FP32 with an FP64 band (same lists)
   uint32_t lds_limit = 65536;
   bool batch_shapes = false;  // brick shapes offered to batched boxes only
   double brick_margin = 1.0;        // brick T / Rmax head room (batched boxes rebuild without re-choosing)

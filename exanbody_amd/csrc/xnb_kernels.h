@@ -219,9 +219,10 @@ BrickParams brick_params(const Geom& g, int Bi, int Bj, int Bk);
 uint32_t brick_layout(BrickParams& bp, bool f32_build = false);
 uint32_t brick_list_granule();
 void launch_brick_stats(const Geom& g, const BrickParams& bp, const uint32_t* cell_n, uint32_t* out, hipStream_t st);
+// prefilter: the unfiltered builds test candidates in FP32 with an FP64 band (the same lists)
 hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, const uint32_t* cell_off, const uint32_t* cell_n, double max2,
-                            const PairCut& pc,
-                            int half_symmetric, int skip_ghosts, uint16_t* blist, uint32_t* bcnt, uint32_t* max_cnt, hipStream_t st);
+                            const PairCut& pc, int half_symmetric, int skip_ghosts, bool prefilter, uint16_t* blist, uint32_t* bcnt,
+                            uint32_t* max_cnt, hipStream_t st);
 hipError_t launch_lj_brick(int precision, bool accum, int epi, const Geom& g, BrickParams bp, const PState& A, const uint32_t* cell_off,
                            const uint32_t* cell_n, const uint16_t* blist, const uint32_t* bcnt, double eps, double sigma, double rcut,
                            double dth, const NextStep* next, hipStream_t st);

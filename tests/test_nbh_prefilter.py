@@ -11,7 +11,7 @@ against the FP32 arithmetic it covers.
 import numpy as np
 import pytest
 
-from exanbody_amd import PRECISION_FP64
+from exanbody_amd import PRECISION_FP64, PRECISION_MIXED, Box, cubic_domain
 from tests.helpers import RCUT, SKIN, fcc_inputs, make_pair
 
 
@@ -64,6 +64,62 @@ def test_coincident_and_near_coincident_atoms():
     b.init(1.0, 1.0, RCUT, SKIN, PRECISION_FP64)
     assert _inner_streams_equal(o, b) > 0
     assert len(r2) == n + 30
+
+
+def _run_fcc(flag, monkeypatch, r, v, ids, L, precision):
+    monkeypatch.setenv("XNB_NBH_PREFILTER", flag)
+    dom = cubic_domain(L, RCUT + SKIN)
+    b = Box(dom["bounds_min"], dom["bounds_max"], dom["grid_dims"], dom["cell_size"])
+    b.nbh_dist(RCUT, SKIN)
+    b.set_particles(r, v, ids)
+    b.init(1.0, 1.0, RCUT, SKIN, precision)
+    res = b.run(40, 0.005, 1.0, 1.0, RCUT, SKIN, precision=precision, rebuild_every=10)
+    e = b.export()
+    gs, gmax = b.streams()
+    b.close()
+    return res, e, gs, gmax
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [PRECISION_FP64, PRECISION_MIXED])
+def test_prefilter_on_off_same_run_at_scale(monkeypatch, precision):
+    """256,000 atoms of a hot, disordered FCC start, 40 steps with a rebuild every 10: the
+    prefiltered and the all-FP64 builds give the same lists, so the runs agree bit for bit."""
+    r, v, ids, L = fcc_inputs(40, temperature=1.5, jitter=0.08, seed=12)
+    ra, ea, gsa, gma = _run_fcc("1", monkeypatch, r, v, ids, L, precision)
+    rb, eb, gsb, gmb = _run_fcc("0", monkeypatch, r, v, ids, L, precision)
+    assert ra == rb and ra["rebuilds"] == 4
+    assert gma == gmb
+    assert len(gsa) == len(gsb)
+    for c in range(len(gsa)):
+        np.testing.assert_array_equal(gsa[c], gsb[c], err_msg=f"cell {c}")
+    for k in ("r", "v", "f", "id"):
+        np.testing.assert_array_equal(ea[k].view(np.uint64) if ea[k].dtype == np.float64 else ea[k],
+                                      eb[k].view(np.uint64) if eb[k].dtype == np.float64 else eb[k], err_msg=k)
+
+
+@pytest.mark.gpu
+def test_prefilter_on_off_same_multimat_slab(monkeypatch):
+    """Per-pair list radii on a two-species slab with vacuum and droplets (150,000 atoms),
+    30 FP64 steps with a rebuild every 10: the same trajectory with and without the prefilter."""
+    from exanbody_amd.workloads import lorentz_berthelot, slab_droplets
+    eps, sig, rc = lorentz_berthelot()
+    r, v, ids, types, dom = slab_droplets(n_atoms=150000, ncell=48, n_droplets=8, seed=4)
+    outs = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("XNB_NBH_PREFILTER", flag)
+        b = Box(dom["bounds_min"], dom["bounds_max"], dom["grid_dims"], dom["cell_size"])
+        b.nbh_dist(float(rc.max()), SKIN)
+        b.set_particles(r, v, ids, types)
+        b.init(1.0, 1.0, float(rc.max()), SKIN, PRECISION_FP64)
+        res = b.run_multimat(30, 0.004, 2, eps, sig, rc, SKIN, precision=PRECISION_FP64, rebuild_every=10)
+        outs.append((res, b.export()))
+        b.close()
+    (ra, ea), (rb, eb) = outs
+    assert ra == rb and ra["rebuilds"] == 3
+    for k in ("r", "v", "f"):
+        np.testing.assert_array_equal(ea[k].view(np.uint64), eb[k].view(np.uint64), err_msg=k)
+    np.testing.assert_array_equal(ea["id"], eb["id"])
 
 
 def _band(max2, cmax):
