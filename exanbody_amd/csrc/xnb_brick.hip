@@ -891,8 +891,9 @@ __device__ __forceinline__ void compute_pair_brick_body(const Geom& g, const Bri
 }
 
 template <bool MIXED, bool ACCUM, int EPI, int U>
-// waves_per_eu(5) caps registers at 96 (a few loop-invariant values spill): measured faster for
-// the FP64 instantiations, neutral for the mixed ones
+// waves_per_eu(5) caps registers at 96: measured faster for the FP64 instantiations, neutral
+// for the mixed ones.  The FP64 form's spills (44-52 bytes) all sit in the pair loop's
+// small-argument branch (d2 < 2^-767, the generic root and division), not in its hot path.
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) k_lj_brick(Geom g, BrickParams bp, const double4* __restrict__ r,
                                                    const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ cell_n,
                                                    const uint16_t* __restrict__ blist, const uint32_t* __restrict__ bcnt,
