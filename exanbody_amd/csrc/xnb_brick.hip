@@ -99,7 +99,8 @@ struct BrickLds {
   int32_t* rdel;    // per region row: slot - LDS index of its atoms, or kNoDelta
   uint16_t* crow;   // per 64 region atoms: the row holding the first of them
   double *px, *py, *pz;
-  float4* pf;   // kPosF32 staging (same LDS bytes as px): atom i's position relative to the brick centre
+  float4* pf;   // kPosF32 staging (same LDS bytes as px): atom i's position relative to the brick
+                // centre, and in w its squared norm (both FP32)
   uint8_t* pt;  // particle type (r.w) of each region atom
   uint32_t* wsum;  // scan scratch (32 words)
   uint32_t* wcnt;  // wave_group_count_add's two words
@@ -172,10 +173,11 @@ __device__ inline BrickPos brick_pos(const Geom& g, const BrickParams& bp, uint3
 }
 
 // What brick_setup stages besides the region tables: nothing, the FP64 positions and types
-// (px/py/pz, pt), or FP32 positions relative to the brick centre (pf, 16 bytes an atom, in the
-// bytes of px) for the neighbour build's prefilter, with the largest magnitude of a staged
-// FP32 coordinate (as float bits) in wcnt[0].
-enum { kPosNone = 0, kPosF64 = 1, kPosF32 = 2 };
+// (px/py/pz, pt), or for the neighbour build's prefilter FP32 positions relative to the brick
+// centre (pf, 16 bytes an atom, in the bytes of px; kPosF32N adds each one's FP32 squared norm
+// in w and the types), with the largest magnitude of a staged FP32 coordinate (as float bits)
+// in wcnt[0].
+enum { kPosNone = 0, kPosF64 = 1, kPosF32 = 2, kPosF32N = 3 };
 
 // FP64 reference point of a brick's FP32 staging: the centre of the brick's cells.
 __device__ inline void brick_centre(const Geom& g, const BrickPos& P, double& cx, double& cy, double& cz) {
@@ -205,7 +207,8 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
                                    const double4* __restrict__ r, const uint32_t* __restrict__ cell_off,
                                    const uint32_t* __restrict__ cell_n, int pos_mode) {
   const uint32_t t = threadIdx.x;
-  if (pos_mode == kPosF32 && t == 0) L.wcnt[0] = 0u;  // ordered before the max by the scan's barrier
+  const bool f32 = pos_mode == kPosF32 || pos_mode == kPosF32N;
+  if (f32 && t == 0) L.wcnt[0] = 0u;  // ordered before the max by the scan's barrier
   uint32_t cnt = 0;
   if (t < (uint32_t)bp.nrc) {
     const int ri = (int)(t % (uint32_t)bp.RX), rj = (int)((t / (uint32_t)bp.RX) % (uint32_t)bp.RY),
@@ -267,7 +270,7 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
     }
     __syncthreads();
     double cx = 0., cy = 0., cz = 0.;
-    if (pos_mode == kPosF32) brick_centre(g, P, cx, cy, cz);
+    if (f32) brick_centre(g, P, cx, cy, cz);
     uint32_t cmax = 0;  // largest |staged FP32 coordinate| of this thread, as bits
     // Up to kStageRound region atoms per thread and round, every load issued before the first
     // LDS store: one memory round trip per round (a loop that stored each load before issuing
@@ -279,14 +282,15 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
         const uint32_t idx = base + (uint32_t)u * blockDim.x;
         if (idx < R) p[u] = r[region_slot(bp, L, idx)];
       }
-      if (pos_mode == kPosF32) {
+      if (f32) {
 #pragma unroll
         for (int u = 0; u < kStageRound; u++) {
           const uint32_t idx = base + (uint32_t)u * blockDim.x;
           if (idx < R) {
-            const float4 c = make_float4((float)(p[u].x - cx), (float)(p[u].y - cy), (float)(p[u].z - cz),
-                                         __uint_as_float((uint32_t)(uint8_t)p[u].w));  // w: the type's bits
+            const float fx = (float)(p[u].x - cx), fy = (float)(p[u].y - cy), fz = (float)(p[u].z - cz);
+            const float4 c = make_float4(fx, fy, fz, pos_mode == kPosF32N ? fmaf(fx, fx, fmaf(fy, fy, fz * fz)) : 0.f);
             L.pf[idx] = c;
+            if (pos_mode == kPosF32N) L.pt[idx] = (uint8_t)p[u].w;
             // NaN and infinities compare above every finite magnitude as bits too
             cmax = max(cmax, max(__float_as_uint(fabsf(c.x)), max(__float_as_uint(fabsf(c.y)), __float_as_uint(fabsf(c.z)))));
           }
@@ -305,8 +309,9 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
       }
     }
     if (t < kSentinels) {  // sentinel atoms R .. R + kSentinels - 1: every distance to them exceeds any cutoff
-      if (pos_mode == kPosF32) {
-        L.pf[R + t] = make_float4(1e30f, 1e30f, 1e30f, 0.f);
+      if (f32) {
+        L.pf[R + t] = make_float4(1e30f, 1e30f, 1e30f, __builtin_inff());
+        if (pos_mode == kPosF32N) L.pt[R + t] = 0;
       } else {
         L.px[3 * (R + t)] = 1e30;
         L.py[3 * (R + t)] = 1e30;
@@ -314,7 +319,7 @@ __device__ inline bool brick_setup(const Geom& g, const BrickParams& bp, const B
         L.pt[R + t] = 0;
       }
     }
-    if (pos_mode == kPosF32 && cmax) __hip_atomic_fetch_max(&L.wcnt[0], cmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (f32 && cmax) __hip_atomic_fetch_max(&L.wcnt[0], cmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __syncthreads();
   }
   return true;
@@ -397,7 +402,7 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
   extern __shared__ __align__(16) char lds[];
   const BrickLds L = brick_lds(bp, lds);
   const BrickPos P = brick_pos(g, bp, b);
-  if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, PREF ? kPosF32 : kPosF64)) return;
+  if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, PREF ? (PAIRWISE ? kPosF32N : kPosF32) : kPosF64)) return;
   const LaneAtom A = lane_atom(bp, L);
   const uint32_t t = threadIdx.x, T = bp.T;
   uint16_t* out = blist + (size_t)b * bp.cap * T;
@@ -424,7 +429,7 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
   // nbh_dist), so candidates beyond every radius of its type are not expanded and re-tested
   double cut2 = max2;
   if (PAIRWISE) {
-    const unsigned ta = min(PREF ? __float_as_uint(L.pf[A.ia].w) : (unsigned)L.pt[A.ia], 3u);
+    const unsigned ta = min((unsigned)L.pt[A.ia], 3u);
     cut2 = 0.0;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -434,20 +439,30 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
     cut2 = fmin(cut2, max2);
   }
   if constexpr (PREF) {
-    // Error band of the FP32 test.  Staged coordinates are fl32(fl64(x - centre)) with
-    // |.| <= C (the brick's largest, widened by 2^-20 for the rounding of its bits), so each
-    // FP32 difference is off the exact one by at most E = 4.01 u C (u = 2^-24, both staged
-    // coordinates and the subtraction; |dx| <= 2 C), and the FP32 d2 (one product, two fmas)
-    // by at most 2 sqrt(3) s E + 3 E^2 + 3.03 u s^2 at a true distance s.  Both sides grow
-    // with s slower than s^2 - T for s^2 >= T, so for a threshold T (= R^2), d2f < T - H
-    // proves s^2 < T and d2f > T + H proves s^2 > T with H = 2 (2 sqrt(3) R E + 3 E^2 +
-    // 4 u R^2); d2f > 16 E^2 proves s > 2.2 E, far from FP64 underflow, so the FP64 d2 is
-    // > 0.  The FP64 d2 differs from s^2 by a relative 2^-51 at most, far inside the factor 2
-    // of H.  Non-finite positions give an unbounded band: every candidate is tested in FP64.
+    // Error bands of the FP32 tests.  Staged coordinates are a_f = fl32(fl64(x - centre)) with
+    // |.| <= C (the brick's largest, widened by 2^-20 for the rounding of its bits), and
+    // w = fl32(|a_f|^2) (a product and two fmas, off by <= 6 u C^2; u = 2^-24).
+    //  * Superset chunk test of the per-pair-radius build, dot form:
+    //    t = fma(-2 a_x, b_x, fma(-2 a_y, b_y, fma(-2 a_z, b_z, w_b)))
+    //    is |b_f|^2 - 2 a_f.b_f + w_b - |b_f|^2 plus three roundings (<= 21 u C^2), so
+    //    D = t + w_a is |a_f - b_f|^2 within 33 u C^2, and |a_f - b_f|^2 is s^2 (s the true
+    //    distance) within 2 sqrt(3) s E2 + 3 E2^2, E2 = 2.01 u C (the two coordinate
+    //    roundings).  The threshold T - w_a is formed in FP64 and rounded outward, so t below
+    //    it proves s^2 < T and t above T + H2 - w_a proves s^2 > T, with
+    //    H2 = 2 (45 u C^2 + 2 sqrt(3) sqrt(T) E2 + 3 E2^2) (both sides of the difference
+    //    grow with s slower than s^2 - T for s^2 >= T).
+    //  * Difference form (the plain chunk test, and the per-pair decision while expanding):
+    //    dx = a_f - b_f in FP32, then d2f = fma(dz, dz, fma(dy, dy, dx dx)), off from s^2 by at
+    //    most 2 sqrt(3) s E + 3 E^2 + 3.03 u s^2 with E = 4.01 u C, so d2f < T - H proves
+    //    s^2 < T and d2f > T + H proves s^2 > T with H = 2 (2 sqrt(3) sqrt(T) E + 3 E^2 + 4 u T);
+    //    d2f > Z = 16 E^2 proves s > 2.2 E, far from FP64 underflow, so the FP64 d2 is > 0.
+    // The FP64 d2 differs from s^2 by a relative 2^-51 at most, far inside the factor 2 of
+    // each band.  Non-finite positions give unbounded bands: every candidate is tested in FP64.
     const float cf = __uint_as_float(L.wcnt[0]);
     const bool finite = cf <= 3.0e38f;
     const double u = 0x1p-24;
-    const double E = 4.01 * u * ((double)cf * (1.0 + 0x1p-20));
+    const double C = (double)cf * (1.0 + 0x1p-20);
+    const double E = 4.01 * u * C, E2 = 2.01 * u * C;
     auto band_lo = [&](double T2) {
       const double H = 2.0 * (2.0 * 1.7320508075688772 * sqrt(T2) * E + 3.0 * E * E + 4.0 * u * T2);
       return finite ? __double2float_rd(T2 - H) : -1.f;
@@ -457,17 +472,23 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
       return finite ? __double2float_ru(T2 + H) : __builtin_inff();
     };
     const float zf = finite ? __double2float_ru(16.0 * E * E) : 0.f;
-    // the chunk test: the plain build decides at nbh_dist; a per-pair-radius build keeps a
-    // superset at this atom's largest radius and decides each candidate against its own
-    // pair's radius while expanding
+    const float4 af = L.pf[A.ia];
+    const double wa = (double)af.w;
+    const double H2 = 2.0 * (45.0 * u * C * C + 2.0 * 1.7320508075688772 * sqrt(cut2) * E2 + 3.0 * E2 * E2);
+    // The plain build decides at nbh_dist in the difference form (12 bytes of LDS and 11 VALU
+    // instructions a candidate; the dot form, 16 bytes and 8, measured the same: the loop
+    // then waits on LDS).  A per-pair-radius build keeps a superset at this atom's largest
+    // radius in the dot form (5 VALU instructions a candidate, 4.5 % faster on the slab) and
+    // decides each candidate against its own pair's radius while expanding.
     const float lo = band_lo(cut2), hi = band_hi(cut2);
+    const float hiA = finite ? __double2float_ru((cut2 + H2) - wa) : __builtin_inff();
+    const float ax2 = -2.f * af.x, ay2 = -2.f * af.y, az2 = -2.f * af.z;
     float tlo[4], thi[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       tlo[q] = PAIRWISE ? band_lo(thr[q]) : 0.f;
       thi[q] = PAIRWISE ? band_hi(thr[q]) : 0.f;
     }
-    const float4 af = L.pf[A.ia];
     const uint32_t R = L.rpref[bp.nrc];
     // Entries go through a 64-bit shift register (four u16, the oldest lowest) and are stored a
     // group of four at a time: group j of the lane is the 8 bytes at grp[j * T].
@@ -485,15 +506,20 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
           const uint32_t np = min((lmax - p0 + 3) & ~3u, 32u);
           const uint32_t lb = p0 < rlen ? rbase + p0 : R;
           const float4* pp = L.pf + lb;
-          uint32_t mhi = 0, mlo = 0;  // bit 31 - k: candidate k within the chunk radius + H; surely kept
+          uint32_t mhi = 0, mlo = 0;  // bit 31 - k: candidate k within the chunk radius + H2; surely kept
           for (uint32_t q = 0; q < np; q += 4) {
 #pragma unroll
             for (uint32_t w = 0; w < 4; w++) {
               const float4 c = pp[q + w];
-              const float dx = af.x - c.x, dy = af.y - c.y, dz = af.z - c.z;
-              const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-              mhi = shift_in(mhi, __builtin_amdgcn_ballot_w64(d2 <= hi));
-              if (!PAIRWISE) mlo = shift_in(mlo, __builtin_amdgcn_ballot_w64(d2 > zf) & __builtin_amdgcn_ballot_w64(d2 < lo));
+              if constexpr (PAIRWISE) {
+                const float tt = fmaf(ax2, c.x, fmaf(ay2, c.y, fmaf(az2, c.z, c.w)));
+                mhi = shift_in(mhi, __builtin_amdgcn_ballot_w64(tt <= hiA));
+              } else {
+                const float dx = af.x - c.x, dy = af.y - c.y, dz = af.z - c.z;
+                const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                mhi = shift_in(mhi, __builtin_amdgcn_ballot_w64(d2 <= hi));
+                mlo = shift_in(mlo, __builtin_amdgcn_ballot_w64(d2 > zf) & __builtin_amdgcn_ballot_w64(d2 < lo));
+              }
             }
           }
           if (np < 32) {
@@ -544,7 +570,7 @@ __device__ __forceinline__ void nbh_brick_body(const Geom& g, const BrickParams&
               const float4 c = L.pf[ib];
               const float dx = af.x - c.x, dy = af.y - c.y, dz = af.z - c.z;
               const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-              const uint32_t tb = __float_as_uint(c.w);
+              const uint32_t tb = L.pt[ib];
               float tl = tlo[0], th = thi[0];
 #pragma unroll
               for (int q = 1; q < 4; q++) {
@@ -1184,15 +1210,15 @@ BrickParams brick_params(const Geom& g, int Bi, int Bj, int Bk) {
 static inline uint32_t align8(uint32_t x) { return (x + 7u) & ~7u; }
 
 // LDS layout of the build, force and conversion kernels; returns its bytes
-uint32_t brick_layout(BrickParams& bp, bool f32_build) {
+uint32_t brick_layout(BrickParams& bp, bool f32_build, bool f32_types) {
   // positions first, at the start of dynamic LDS, so their addresses need no runtime base
   uint32_t o = 0;
   bp.off_pos = o;
   // region atoms + the sentinel block: FP64 x, y, z and a type byte, or the prefiltered
-  // build's FP32 float4 (no types)
+  // build's FP32 float4 (a type byte only for the per-pair-radius build)
   o += (f32_build ? 16u : 24u) * (bp.Rmax + kSentinels);
   bp.off_type = o;
-  if (!f32_build) o = align8(o + bp.Rmax + kSentinels);
+  if (!f32_build || f32_types) o = align8(o + bp.Rmax + kSentinels);
   bp.off_rcnt = o;
   o = align8(o + 4u * bp.nrc);
   bp.off_rpref = o;
@@ -1250,7 +1276,7 @@ hipError_t launch_nbh_brick(const Geom& g, BrickParams bp, const double4* r, con
                                                                         blist, bcnt, max_cnt, pc);
   } else if (pc.nt > 0 && prefilter) {
     BrickParams bq = bp;
-    const uint32_t lds_b = brick_layout(bq, true);
+    const uint32_t lds_b = brick_layout(bq, true, true);
     e = set_lds((const void*)k_nbh_brick_mask<false, true, true>, lds_b);
     if (e != hipSuccess) return e;
     k_nbh_brick_mask<false, true, true><<<kXcds * bq.per_xcd, bq.T, lds_b, st>>>(g, bq, r, cell_off, cell_n, max2, 0, 0, blist, bcnt,

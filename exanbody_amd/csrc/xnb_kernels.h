@@ -215,8 +215,8 @@ void launch_fp64_core(const double* x, size_t n, double* sq, double* rc, hipStre
 
 BrickParams brick_params(const Geom& g, int Bi, int Bj, int Bk);
 // LDS layout of the brick kernels (offsets into bp), its bytes; f32_build: the prefiltered
-// neighbour build's (FP32 positions, no types)
-uint32_t brick_layout(BrickParams& bp, bool f32_build = false);
+// neighbour build's (FP32 positions; types only with f32_types, for per-pair radii)
+uint32_t brick_layout(BrickParams& bp, bool f32_build = false, bool f32_types = false);
 uint32_t brick_list_granule();
 void launch_brick_stats(const Geom& g, const BrickParams& bp, const uint32_t* cell_n, uint32_t* out, hipStream_t st);
 // prefilter: the unfiltered builds test candidates in FP32 with an FP64 band (the same lists)

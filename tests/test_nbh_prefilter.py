@@ -139,6 +139,57 @@ def _fma32(a, b, c):
     return (a.astype(np.float64) * b.astype(np.float64) + c.astype(np.float64)).astype(np.float32)
 
 
+def _dot_band(T2, cmax):
+    """The per-pair-radius build's superset band H2 on t + |a_f|^2 (xnb_brick.hip, PREF), and
+    its distance-independent part (the rounding of the norms and of the three fmas)."""
+    u = 2.0 ** -24
+    C = cmax * (1.0 + 2.0 ** -20)
+    E2 = 2.01 * u * C
+    H2 = 2.0 * (45.0 * u * C * C + 2.0 * np.sqrt(3.0) * np.sqrt(T2) * E2 + 3.0 * E2 * E2)
+    return H2, 45.0 * u * C * C + 3.0 * E2 * E2
+
+
+@pytest.mark.parametrize("cmax", [2.0, 9.8, 40.0])
+def test_dot_form_band_bounds_fp32_error(cmax):
+    """The per-pair-radius build's superset test evaluates t = |b_f|^2 - 2 a_f.b_f with three
+    fmas on the staged FP32 coordinates and their staged FP32 squared norms; t + |a_f|^2 stays
+    within half its band of the FP64 d2 near the cutoff, and within the band's
+    distance-independent part plus its distance term for near-coincident pairs."""
+    rng = np.random.default_rng(5)
+    nbh = RCUT + SKIN
+    max2 = nbh * nbh
+    H2, H0 = _dot_band(max2, cmax)
+    m = 400000
+    centre = rng.uniform(-1e3, 1e3, 3)
+    a = rng.uniform(-cmax, cmax, (m, 3))
+    d = rng.normal(size=(m, 3))
+    scale = np.where(rng.random(m) < 0.1, rng.uniform(0.0, 0.05, m), nbh * rng.uniform(0.98, 1.02, m))
+    d *= (scale / np.linalg.norm(d, axis=1))[:, None]
+    b = np.clip(a + d, -cmax, cmax)
+    pa, pb = a + centre, b + centre
+    ex = pa - pb
+    d2 = (ex[:, 0] * ex[:, 0] + ex[:, 1] * ex[:, 1]) + ex[:, 2] * ex[:, 2]
+    fa = (pa - centre).astype(np.float32)
+    fb = (pb - centre).astype(np.float32)
+
+    def norm2(f):
+        return _fma32(f[:, 0], f[:, 0], _fma32(f[:, 1], f[:, 1], f[:, 2] * f[:, 2]))
+
+    wa, wb = norm2(fa), norm2(fb)
+    m2 = np.float32(-2.0)
+    t = _fma32(m2 * fa[:, 0], fb[:, 0], _fma32(m2 * fa[:, 1], fb[:, 1], _fma32(m2 * fa[:, 2], fb[:, 2], wb)))
+    err = np.abs(t.astype(np.float64) + wa.astype(np.float64) - d2)
+    near = d2 > 0.5 * max2
+    assert err[near].max() <= 0.5 * H2
+    close = d2 < 0.01
+    assert err[close].max() <= H0 + 2.0 * np.sqrt(3.0) * 0.1 * 2.01 * 2.0 ** -24 * cmax * 2.0
+    # the band stays a small fraction of the candidates at the bench's brick size
+    if cmax == 9.8:
+        assert H2 < 2e-3
+        frac = np.mean(np.abs(t[near].astype(np.float64) + wa[near] - max2) <= H2)
+        assert frac < 0.05
+
+
 @pytest.mark.parametrize("cmax", [2.0, 9.8, 40.0])
 def test_band_bounds_fp32_error(cmax):
     """For staged coordinates up to cmax, |d2f - d2| stays below half the band (the factor 2
