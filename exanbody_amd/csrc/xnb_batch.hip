@@ -98,6 +98,12 @@ struct xnb_batch {
   // brick work launched per class of boxes (XNB_BATCH_CLASSES=0: one launch sized for the
   // widest box)
   bool use_classes = true;
+  // a step's force launches (disjoint boxes) alternate between st and st2, forked from and
+  // joined back into st, so one class's tail overlaps another's work (XNB_BATCH_STREAMS=0:
+  // all on st); both streams at the default priority
+  bool split_streams = true;
+  hipStream_t st2 = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<uint32_t> gtab, last_quiet;  // table of the captured graph / of the last quiet step
   size_t g_og = 0, g_ng = 0;
@@ -326,6 +332,35 @@ void destroy_retired(xnb_batch* B) {  // after a stream synchronisation
 // Capture the quiet step of table T2 (force table at o_f, ghost refresh at o_g) into a new
 // graph.  Its table goes to the graph buffer not used by the current graph: graphs launched
 // earlier that read that buffer run before the copy (stream order).
+// n launches over disjoint boxes (one per class), alternately on st and st2 when there are
+// several: st2 waits for st's work so far and st waits for st2's launches before anything
+// after them.  launch(i, stream) issues the i-th.
+template <class Launch>
+hipError_t launch_split(xnb_batch* B, size_t n, Launch&& launch) {
+  const bool split = B->split_streams && n > 1;
+  hipError_t he = hipSuccess;
+  if (split) {
+    if ((he = hipEventRecord(B->fork_ev, B->st)) != hipSuccess) return he;
+    if ((he = hipStreamWaitEvent(B->st2, B->fork_ev, 0)) != hipSuccess) return he;
+  }
+  for (size_t i = 0; i < n && he == hipSuccess; i++) he = launch(i, split && (i & 1) ? B->st2 : B->st);
+  if (split) {
+    const hipError_t a = hipEventRecord(B->join_ev, B->st2);
+    const hipError_t b = a == hipSuccess ? hipStreamWaitEvent(B->st, B->join_ev, 0) : a;
+    if (he == hipSuccess) he = b;
+  }
+  return he;
+}
+
+// the step's force launches; set_lds: set each kernel's LDS attribute first (not under graph
+// capture)
+hipError_t launch_force_classes(xnb_batch* B, const uint32_t* W, const std::vector<BrickLaunch>& lf, bool set_lds) {
+  return launch_split(B, lf.size(), [&](size_t i, hipStream_t s) {
+    const BrickLaunch& L = lf[i];
+    return launch_batch_lj(L.kind == kBatchLjMixed, B->dF.p, W + L.o, (uint32_t)L.n, L.T, L.lds, B->dclock.p, s, set_lds);
+  });
+}
+
 int capture_quiet(xnb_batch* B, const Tables& T2, size_t o_g, size_t n_g, const std::vector<BrickLaunch>& lf) {
   const int j = B->gcur ^ 1;
   const size_t n = T2.v.size();
@@ -350,10 +385,7 @@ int capture_quiet(xnb_batch* B, const Tables& T2, size_t o_g, size_t n_g, const 
   const uint32_t* W = B->dgwork[j].p;
   BHIP(B, hipStreamBeginCapture(B->st, hipStreamCaptureModeRelaxed));
   launch_batch_ghost_update_r(B->dF.p, W + o_g, (uint32_t)n_g, B->dclock.p, B->st);
-  hipError_t le = hipSuccess;
-  for (const BrickLaunch& L : lf)
-    if (le == hipSuccess)
-      le = launch_batch_lj(L.kind == kBatchLjMixed, B->dF.p, W + L.o, (uint32_t)L.n, L.T, L.lds, B->dclock.p, B->st, false);
+  const hipError_t le = launch_force_classes(B, W, lf, false);
   launch_batch_advance(B->dclock.p, B->st);
   hipGraph_t g = nullptr;
   const hipError_t ce = hipStreamEndCapture(B->st, &g);
@@ -433,7 +465,9 @@ int xnb_batch_create(xnb_batch** out, int device) {
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
     B->lds_limit = (uint32_t)lds;
   for (int k = 0; k < 3; k++) B->kinfo[k] = batch_kernel_info(k);
-  bool ok = hipStreamCreate(&B->st) == hipSuccess;
+  bool ok = hipStreamCreate(&B->st) == hipSuccess && hipStreamCreate(&B->st2) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&B->fork_ev, hipEventDisableTiming) == hipSuccess &&
+       hipEventCreateWithFlags(&B->join_ev, hipEventDisableTiming) == hipSuccess;
   ok = ok && hipMalloc((void**)&B->dF.p, kMaxBatchBoxes * sizeof(BoxFrame)) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->hF, kMaxBatchBoxes * sizeof(BoxFrame), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipMalloc((void**)&B->dwords.p, 4 * kMaxBatchBoxes * sizeof(uint32_t)) == hipSuccess;
@@ -450,6 +484,7 @@ int xnb_batch_create(xnb_batch** out, int device) {
   }
   if (const char* s = std::getenv("XNB_BATCH_GRAPH")) B->use_graph = std::atoi(s) != 0;
   if (const char* s = std::getenv("XNB_BATCH_CLASSES")) B->use_classes = std::atoi(s) != 0;
+  if (const char* s = std::getenv("XNB_BATCH_STREAMS")) B->split_streams = std::atoi(s) != 0;
   double lds_frac = 0.5;
   if (const char* s = std::getenv("XNB_BATCH_LDS")) {
     const double f = std::atof(s);
@@ -503,6 +538,9 @@ int xnb_batch_destroy(xnb_batch* B) {
     if (B->hwork[k]) (void)hipHostFree(B->hwork[k]);
     if (B->hev[k]) (void)hipEventDestroy(B->hev[k]);
   }
+  if (B->fork_ev) (void)hipEventDestroy(B->fork_ev);
+  if (B->join_ev) (void)hipEventDestroy(B->join_ev);
+  if (B->st2) (void)hipStreamDestroy(B->st2);
   if (B->st) (void)hipStreamDestroy(B->st);
   delete B;
   return XNB_OK;
@@ -643,10 +681,10 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
       BHIP(B, hipMemsetAsync(B->amr_count.p, 0, sizeof(uint32_t), B->st));
       launch_batch_ghosts(B->dF.p, W + o_rs, (uint32_t)n_rs, W + o_rc, (uint32_t)n_rc, W + o_rg, (uint32_t)n_rg, W + o_list,
                           (uint32_t)reb.size(), B->amr_list.p, B->amr_count.p, par, B->st);
-      for (const BrickLaunch& L : lb) {
-        hipError_t he = launch_batch_nbh(B->dF.p, W + L.o, (uint32_t)L.n, L.T, L.lds, par, B->st);
-        if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched neighbour build: ") + hipGetErrorString(he));
-      }
+      const hipError_t he = launch_split(B, lb.size(), [&](size_t i, hipStream_t s) {
+        return launch_batch_nbh(B->dF.p, W + lb[i].o, (uint32_t)lb[i].n, lb[i].T, lb[i].lds, par, s);
+      });
+      if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched neighbour build: ") + hipGetErrorString(he));
       BHIP(B, hipMemcpyAsync(B->hwords, B->dwords.p, 4 * kMaxBatchBoxes * sizeof(uint32_t), hipMemcpyDeviceToHost, B->st));
       BHIP(B, hipStreamSynchronize(B->st));
       destroy_retired(B);
@@ -733,10 +771,8 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
       if ((e = upload_tables(B, T2))) return e;
       W = B->dwork.p;
       launch_batch_ghost_update_r(B->dF.p, W + o_g, (uint32_t)n_g, B->dclock.p, B->st);
-      for (const BrickLaunch& L : lf) {
-        hipError_t he = launch_batch_lj(L.kind == kBatchLjMixed, B->dF.p, W + L.o, (uint32_t)L.n, L.T, L.lds, B->dclock.p, B->st);
-        if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched force: ") + hipGetErrorString(he));
-      }
+      const hipError_t he = launch_force_classes(B, W, lf, true);
+      if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched force: ") + hipGetErrorString(he));
       launch_batch_advance(B->dclock.p, B->st);
     }
     for (size_t q = 0; q < frc.size(); q++) {
