@@ -519,6 +519,7 @@ int xnb_batch_destroy(xnb_batch* B) {
   // teardown: failures here leave nothing to recover, so results are not checked
   (void)hipSetDevice(B->device);
   if (B->st) (void)hipStreamSynchronize(B->st);
+  if (B->st2) (void)hipStreamSynchronize(B->st2);  // joined into st, unless a launch failed in between
   for (auto& bb : B->box)
     if (bb.counts.p) (void)hipFree(bb.counts.p);
   if (B->dF.p) (void)hipFree(B->dF.p);
