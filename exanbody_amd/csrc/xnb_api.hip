@@ -248,8 +248,8 @@ int op_move(xnb_ctx* c, bool forces = true, bool check = true) {
   launch_move_pack(g, c->cell_off.p, c->cell_n.p, c->cnt_out.p, c->cnt_in.p, c->mv_dst.p, c->pk.p, c->newloc.p, c->new_n.p, c->st);
   int e;
   if ((e = scan(c, c->new_n.p, g.ncells, 0, c->new_off.p))) return e;
-  launch_move_scatter(c->N, c->A, c->B, c->slot_cell_A.p, c->mv_dst.p, c->newloc.p, c->new_off.p, c->cell_n.p, c->cnt_out.p,
-                      c->in_cur.p, c->in_list.p, c->slot_cell_B.p, forces, c->st);
+  launch_move_scatter(c->N, c->A, c->B, c->slot_cell_A.p, c->mv_dst.p, c->newloc.p, c->cell_off.p, c->new_off.p, c->cell_n.p,
+                      c->cnt_out.p, c->in_cur.p, c->in_list.p, c->slot_cell_B.p, forces, c->st);
   launch_move_incoming(g, c->A, c->B, c->new_off.p, c->cell_n.p, c->cnt_out.p, c->cnt_in.p, c->in_list.p, c->slot_cell_B.p, forces,
                        c->st);
   launch_set_inner_cells(g, c->new_n.p, c->new_off.p, c->cell_n.p, c->cell_off.p, c->st);

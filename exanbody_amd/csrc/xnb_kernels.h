@@ -155,8 +155,9 @@ void launch_move_pack(const Geom& g, const uint32_t* cell_off, const uint32_t* c
                       const uint32_t* cnt_in, const uint32_t* mv_dst, uint32_t* pk, uint32_t* newloc, uint32_t* new_n,
                       hipStream_t st);
 void launch_move_scatter(uint32_t N, const PState& A, const PState& B, const uint32_t* slot_cell, const uint32_t* mv_dst,
-                         const uint32_t* newloc, const uint32_t* new_off, const uint32_t* cell_n, const uint32_t* cnt_out,
-                         uint32_t* in_cur, uint32_t* in_list, uint32_t* slot_cell_B, bool forces, hipStream_t st);
+                         const uint32_t* newloc, const uint32_t* cell_off, const uint32_t* new_off, const uint32_t* cell_n,
+                         const uint32_t* cnt_out, uint32_t* in_cur, uint32_t* in_list, uint32_t* slot_cell_B, bool forces,
+                         hipStream_t st);
 void launch_move_incoming(const Geom& g, const PState& A, const PState& B, const uint32_t* new_off, const uint32_t* cell_n,
                           const uint32_t* cnt_out, const uint32_t* cnt_in, uint32_t* in_list, uint32_t* slot_cell_B,
                           bool forces, hipStream_t st);

@@ -172,10 +172,7 @@ __device__ inline void move_pack_item(Geom g, const uint32_t* __restrict__ cell_
   }
   const uint32_t n = cell_n[c], off = cell_off[c], nout = cnt_out[c];
   new_n[c] = n - nout + cnt_in[c];
-  if (nout == 0) {
-    for (uint32_t q = 0; q < n; q++) newloc[off + q] = q;
-    return;
-  }
+  if (nout == 0) return;  // every particle keeps its index (the scatter derives it)
   for (uint32_t q = 0; q < n; q++) pk[off + q] = (mv_dst[off + q] == NONE) ? q : NONE;
   int m = (int)n;
   for (int q = (int)n - 1; q >= 0; q--) {
@@ -213,15 +210,18 @@ __device__ inline void copy_particle(const PState& A, uint32_t s, const PState& 
     if (k < A.nx) B.x[k][t] = A.x[k][s];
 }
 
+// newloc is written only for cells that lose particles; in the others a particle keeps its
+// index, s - cell_off[c].
 __device__ inline void move_scatter_item(uint32_t N, PState A, PState B, const uint32_t* __restrict__ slot_cell,
                                const uint32_t* __restrict__ mv_dst, const uint32_t* __restrict__ newloc,
-                               const uint32_t* __restrict__ new_off, const uint32_t* __restrict__ cell_n,
-                               const uint32_t* __restrict__ cnt_out, uint32_t* __restrict__ in_cur,
-                               uint32_t* __restrict__ in_list, uint32_t* __restrict__ slot_cell_B, uint32_t s, bool forces) {
+                               const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ new_off,
+                               const uint32_t* __restrict__ cell_n, const uint32_t* __restrict__ cnt_out,
+                               uint32_t* __restrict__ in_cur, uint32_t* __restrict__ in_list, uint32_t* __restrict__ slot_cell_B,
+                               uint32_t s, bool forces) {
   const uint32_t c = slot_cell[s];
   const uint32_t dc = mv_dst[s];
   if (dc == NONE) {
-    const uint32_t t = new_off[c] + newloc[s];
+    const uint32_t t = new_off[c] + (cnt_out[c] ? newloc[s] : s - cell_off[c]);
     copy_particle(A, s, B, t, forces);
     slot_cell_B[t] = c;
   } else {
@@ -232,12 +232,14 @@ __device__ inline void move_scatter_item(uint32_t N, PState A, PState B, const u
 
 __global__ void k_move_scatter(uint32_t N, PState A, PState B, const uint32_t* __restrict__ slot_cell,
                                const uint32_t* __restrict__ mv_dst, const uint32_t* __restrict__ newloc,
-                               const uint32_t* __restrict__ new_off, const uint32_t* __restrict__ cell_n,
-                               const uint32_t* __restrict__ cnt_out, uint32_t* __restrict__ in_cur,
-                               uint32_t* __restrict__ in_list, uint32_t* __restrict__ slot_cell_B, int forces) {
+                               const uint32_t* __restrict__ cell_off, const uint32_t* __restrict__ new_off,
+                               const uint32_t* __restrict__ cell_n, const uint32_t* __restrict__ cnt_out,
+                               uint32_t* __restrict__ in_cur, uint32_t* __restrict__ in_list, uint32_t* __restrict__ slot_cell_B,
+                               int forces) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
-  move_scatter_item(N, A, B, slot_cell, mv_dst, newloc, new_off, cell_n, cnt_out, in_cur, in_list, slot_cell_B, s, forces != 0);
+  move_scatter_item(N, A, B, slot_cell, mv_dst, newloc, cell_off, new_off, cell_n, cnt_out, in_cur, in_list, slot_cell_B, s,
+                    forces != 0);
 }
 
 // one thread per cell: order incoming movers by source slot, copy them with wrapped positions
@@ -1077,10 +1079,11 @@ void launch_move_pack(const Geom& g, const uint32_t* cell_off, const uint32_t* c
   k_move_pack<<<nblk(g.ncells, 128), 128, 0, st>>>(g, cell_off, cell_n, cnt_out, cnt_in, mv_dst, pk, newloc, new_n);
 }
 void launch_move_scatter(uint32_t N, const PState& A, const PState& B, const uint32_t* slot_cell, const uint32_t* mv_dst,
-                         const uint32_t* newloc, const uint32_t* new_off, const uint32_t* cell_n, const uint32_t* cnt_out,
-                         uint32_t* in_cur, uint32_t* in_list, uint32_t* slot_cell_B, bool forces, hipStream_t st) {
-  k_move_scatter<<<nblk(N, 256), 256, 0, st>>>(N, A, B, slot_cell, mv_dst, newloc, new_off, cell_n, cnt_out, in_cur, in_list,
-                                               slot_cell_B, forces ? 1 : 0);
+                         const uint32_t* newloc, const uint32_t* cell_off, const uint32_t* new_off, const uint32_t* cell_n,
+                         const uint32_t* cnt_out, uint32_t* in_cur, uint32_t* in_list, uint32_t* slot_cell_B, bool forces,
+                         hipStream_t st) {
+  k_move_scatter<<<nblk(N, 256), 256, 0, st>>>(N, A, B, slot_cell, mv_dst, newloc, cell_off, new_off, cell_n, cnt_out, in_cur,
+                                               in_list, slot_cell_B, forces ? 1 : 0);
 }
 void launch_move_incoming(const Geom& g, const PState& A, const PState& B, const uint32_t* new_off, const uint32_t* cell_n,
                           const uint32_t* cnt_out, const uint32_t* cnt_in, uint32_t* in_list, uint32_t* slot_cell_B,
@@ -1348,8 +1351,8 @@ __global__ void __launch_bounds__(kBatchSlotBlock) kb_move_scatter(const BoxFram
   const BoxFrame& f = F[box];
   if (s >= f.N) return;
   const FrameState X = frame_state(f, par);
-  move_scatter_item(f.N, X.A, X.B, X.scA, f.mv_dst, f.newloc, f.new_off, f.cell_n, f.cnt_out, f.in_cur, f.in_list, X.scB, s,
-                    false);  // inside a run: forces not carried
+  move_scatter_item(f.N, X.A, X.B, X.scA, f.mv_dst, f.newloc, f.cell_off, f.new_off, f.cell_n, f.cnt_out, f.in_cur, f.in_list,
+                    X.scB, s, false);  // inside a run: forces not carried
 }
 
 __global__ void __launch_bounds__(kBatchSlotBlock) kb_move_incoming(const BoxFrame* __restrict__ F, const uint32_t* __restrict__ work,
