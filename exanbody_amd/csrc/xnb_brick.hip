@@ -791,9 +791,19 @@ __device__ __forceinline__ void compute_pair_brick_body(const Geom& g, const Bri
     L.wcnt[1] = 0;
   }
   const uint32_t t = threadIdx.x, T = bp.T;
-  // the list length is issued before the staging so its latency overlaps it (the build wrote
-  // 0 for every lane of the brick without an atom)
+  // U entries per batch = U / 4 eight-byte loads per lane; the next batch's loads are issued
+  // before the current batch is computed.  Batches never pass cap (a multiple of U), and the
+  // build padded every list of the wave with the sentinel up to a multiple of U.
+  static_assert(U % 4 == 0 && U == kListBatch, "batches are whole groups of four entries, as padded");
+  constexpr int W = U / 4;
+  const uint2* lst = reinterpret_cast<const uint2*>(blist + (size_t)b * bp.cap * T) + t;
+  // The list length and the first batch are issued before the staging so their latency
+  // overlaps it (the build wrote 0 for every lane of the brick without an atom).  The first
+  // batch lies inside the lane's slab (cap >= U) and is used only by waves with a list.
   const uint32_t n_list = t < T ? bcnt[(size_t)b * T + t] : 0u;
+  uint2 wn[W];
+#pragma unroll
+  for (int w = 0; w < W; w++) wn[w] = t < T ? lst[(size_t)w * T] : make_uint2(0u, 0u);
   if (!brick_setup(g, bp, P, L, r, cell_off, cell_n, kPosF64)) return;
   const LaneAtom At = lane_atom(bp, L);
   const uint32_t n = At.live ? n_list : 0u;
@@ -803,12 +813,6 @@ __device__ __forceinline__ void compute_pair_brick_body(const Geom& g, const Bri
   ra.y = L.py[3 * At.ia];
   ra.z = L.pz[3 * At.ia];
   ra.w = 0.;
-  // U entries per batch = U / 4 eight-byte loads per lane; the next batch's loads are issued
-  // before the current batch is computed.  Batches never pass cap (a multiple of U), and the
-  // build padded every list of the wave with the sentinel up to a multiple of U.
-  static_assert(U % 4 == 0 && U == kListBatch, "batches are whole groups of four entries, as padded");
-  constexpr int W = U / 4;
-  const uint2* lst = reinterpret_cast<const uint2*>(blist + (size_t)b * bp.cap * T) + t;
   double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
   if (!BUFFERED && ACCUM && At.live) {  // buffer-less form: pairs go straight into the stored force
     tx = A.fx[At.slot];
@@ -817,11 +821,6 @@ __device__ __forceinline__ void compute_pair_brick_body(const Geom& g, const Bri
     if (Tr::CentralEnergy) te = A.ep[At.slot];
   }
   const int type_a = Tr::CentralType ? (int)L.pt[At.ia] : 0;
-  uint2 wn[W];
-  if (nmax > 0) {
-#pragma unroll
-    for (int w = 0; w < W; w++) wn[w] = lst[(size_t)w * T];
-  }
   for (uint32_t q0 = 0; q0 < nmax; q0 += U) {
     uint2 wc[W];
 #pragma unroll
