@@ -10,10 +10,14 @@
 // own.
 //
 // Host synchronisation: none on a step without rebuilds; one read-back of four status words
-// per box on a step where some box rebuilds (ghost counts for the next work tables, overflow
-// flags).  A box whose rebuild overflowed the sizes its frame was laid out for (ghost slots,
-// brick workgroup or region, list capacity) is redone with the one-box operators, which
-// re-size its arrays.
+// per box at the end of a step where some box rebuilds (ghost counts for the next work
+// tables, overflow flags).  A box whose rebuild overflowed the sizes its frame was laid out
+// for (ghost slots, brick workgroup or region, list capacity) is left untouched by that step's
+// force kernel, then redone with the one-box operators, which re-size its arrays, and given
+// its force in a launch of its own.  A step with no box starting or rebuilt on displacement
+// is one launch sequence (rebuilds, ghost refresh, force, clock), replayed from a captured
+// HIP graph once the same sequence has come twice (quiet steps and rebuild steps each have
+// their own graph).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -62,6 +66,32 @@ struct BrickLaunch {
   bool operator==(const BrickLaunch& b) const { return kind == b.kind && o == b.o && n == b.n && T == b.T && lds == b.lds; }
 };
 
+// The launches of one step after its first half, as ranges of the step's work table.
+struct StepLayout {
+  uint32_t par = 0;   // parity of a rebuild step (its rebuild kernels take it as an argument)
+  uint32_t nreb = 0;  // rebuilding boxes, listed at o_list
+  size_t o_rs = 0, n_rs = 0, o_rc = 0, n_rc = 0, o_rg = 0, n_rg = 0, o_list = 0, o_zero = 0;
+  std::vector<BrickLaunch> lb;  // neighbour builds of the rebuilding boxes
+  size_t o_g = 0, n_g = 0;      // ghost refresh of the other boxes
+  std::vector<BrickLaunch> lf;  // force of every box
+  bool operator==(const StepLayout& x) const {
+    return par == x.par && nreb == x.nreb && o_rs == x.o_rs && n_rs == x.n_rs && o_rc == x.o_rc && n_rc == x.n_rc &&
+           o_rg == x.o_rg && n_rg == x.n_rg && o_list == x.o_list && o_zero == x.o_zero && lb == x.lb && o_g == x.o_g &&
+           n_g == x.n_g && lf == x.lf;
+  }
+};
+
+// A captured step: its graph with the table and layout it was captured from (the table in
+// buf[cur]), and the table and layout of the last step of its kind.
+struct StepGraph {
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint32_t> tab, last_tab;
+  StepLayout lay, last_lay;
+  bool have_last = false;
+  DArr<uint32_t> buf[2];
+  int cur = 0;
+};
+
 }  // namespace
 
 struct xnb_batch {
@@ -81,12 +111,11 @@ struct xnb_batch {
   int hcur = 0;
   uint32_t par = 0;
   uint32_t gstep = 1;  // global step number of the next step
-  DArr<uint32_t> dclock;  // {par, gstep} on the device, read by the quiet-step kernels
-  DArr<uint32_t> amr_list, amr_count;  // rebuild_amr: cells of rebuilding boxes to sort, their number
+  DArr<uint32_t> dclock;  // {par, gstep} of the step being run, then of the previous step
+  DArr<uint32_t> amr_list, amr_count;  // rebuild_amr: cells of rebuilding boxes to sort, their number (+3 words)
   uint32_t lds_limit = 65536;
   BatchKernelInfo kinfo[3] = {};  // per batched brick kernel (kBatchLjFp64, kBatchLjMixed, kBatchNbh)
-  // HIP graph of a quiet step (no box fresh, rebuilt on displacement or rebuilding): ghost
-  // refresh, force, clock advance, replayed while the step's work table stays the same
+  // steps replayed from captured HIP graphs (XNB_BATCH_GRAPH=0: every step launched eagerly)
   bool use_graph = true;
   // head room of attached boxes (XNB_BATCH_MARGINS="brick,slot" at create; 1,1 leaves none, so
   // most rebuilds that grow a brick or the ghosts take the one-box recovery)
@@ -103,13 +132,11 @@ struct xnb_batch {
   // all on st); both streams at the default priority
   bool split_streams = true;
   hipStream_t st2 = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  std::vector<uint32_t> gtab, last_quiet;  // table of the captured graph / of the last quiet step
-  size_t g_og = 0, g_ng = 0;
-  std::vector<BrickLaunch> g_lf;
-  DArr<uint32_t> dgwork[2];  // device copies of graph tables (the current graph's and the next)
-  int gcur = 0;
+  hipEvent_t fork_ev[2] = {nullptr, nullptr}, join_ev[2] = {nullptr, nullptr};  // neighbour build, force
+  // graphs of quiet steps (no box starting, rebuilding or rebuilt on displacement) and of
+  // rebuild steps (fixed-interval rebuilds only), one per step parity, which the rebuild
+  // kernels take as an argument (an odd interval alternates it)
+  StepGraph gq, gr[2];
   std::vector<hipGraphExec_t> retired;  // destroyed after the next stream synchronisation
   int64_t n_graph_steps = 0, n_captures = 0;
   std::string err;
@@ -335,75 +362,106 @@ void destroy_retired(xnb_batch* B) {  // after a stream synchronisation
 // n launches over disjoint boxes (one per class), alternately on st and st2 when there are
 // several: st2 waits for st's work so far and st waits for st2's launches before anything
 // after them.  launch(i, stream) issues the i-th.
+// (ev: the event pair of this group of launches; a step has two such groups)
 template <class Launch>
-hipError_t launch_split(xnb_batch* B, size_t n, Launch&& launch) {
+hipError_t launch_split(xnb_batch* B, size_t n, int ev, Launch&& launch) {
   const bool split = B->split_streams && n > 1;
   hipError_t he = hipSuccess;
   if (split) {
-    if ((he = hipEventRecord(B->fork_ev, B->st)) != hipSuccess) return he;
-    if ((he = hipStreamWaitEvent(B->st2, B->fork_ev, 0)) != hipSuccess) return he;
+    if ((he = hipEventRecord(B->fork_ev[ev], B->st)) != hipSuccess) return he;
+    if ((he = hipStreamWaitEvent(B->st2, B->fork_ev[ev], 0)) != hipSuccess) return he;
   }
   for (size_t i = 0; i < n && he == hipSuccess; i++) he = launch(i, split && (i & 1) ? B->st2 : B->st);
   if (split) {
-    const hipError_t a = hipEventRecord(B->join_ev, B->st2);
-    const hipError_t b = a == hipSuccess ? hipStreamWaitEvent(B->st, B->join_ev, 0) : a;
+    const hipError_t a = hipEventRecord(B->join_ev[ev], B->st2);
+    const hipError_t b = a == hipSuccess ? hipStreamWaitEvent(B->st, B->join_ev[ev], 0) : a;
     if (he == hipSuccess) he = b;
   }
   return he;
 }
 
-// the step's force launches; set_lds: set each kernel's LDS attribute first (not under graph
-// capture)
-hipError_t launch_force_classes(xnb_batch* B, const uint32_t* W, const std::vector<BrickLaunch>& lf, bool set_lds) {
-  return launch_split(B, lf.size(), [&](size_t i, hipStream_t s) {
+// the step's force launches at the device clock `clock`; set_lds: set each kernel's LDS
+// attribute first (not under graph capture)
+hipError_t launch_force_classes(xnb_batch* B, const uint32_t* W, const std::vector<BrickLaunch>& lf, bool set_lds,
+                                const uint32_t* clock) {
+  return launch_split(B, lf.size(), 1, [&](size_t i, hipStream_t s) {
     const BrickLaunch& L = lf[i];
-    return launch_batch_lj(L.kind == kBatchLjMixed, B->dF.p, W + L.o, (uint32_t)L.n, L.T, L.lds, B->dclock.p, s, set_lds);
+    return launch_batch_lj(L.kind == kBatchLjMixed, B->dF.p, W + L.o, (uint32_t)L.n, L.T, L.lds, clock, s, set_lds);
   });
 }
 
-int capture_quiet(xnb_batch* B, const Tables& T2, size_t o_g, size_t n_g, const std::vector<BrickLaunch>& lf) {
-  const int j = B->gcur ^ 1;
-  const size_t n = T2.v.size();
+// Everything of a step after its first half, from work table W laid out as L: the rebuilds
+// (move_particles, rebuild_amr, backup_r, ghosts, chunk_neighbors), the ghost refresh of the
+// other boxes, the force of every box, the clock advance.  No host synchronisation and no
+// memset nodes (the rebuild_amr counter is cleared by a kernel), so it can be captured.
+hipError_t launch_step(xnb_batch* B, const uint32_t* W, const StepLayout& L, bool set_lds) {
+  if (L.nreb) {
+    launch_batch_zero_words(B->dwords.p, W + L.o_list, L.nreb, B->st);
+    launch_batch_zero_words(B->amr_count.p, W + L.o_zero, 1, B->st);  // its entry is box 0: words 0..3
+    launch_batch_move(B->dF.p, W + L.o_rs, (uint32_t)L.n_rs, W + L.o_rc, (uint32_t)L.n_rc, W + L.o_list, L.nreb, L.par, B->st);
+    launch_batch_ghosts(B->dF.p, W + L.o_rs, (uint32_t)L.n_rs, W + L.o_rc, (uint32_t)L.n_rc, W + L.o_rg, (uint32_t)L.n_rg,
+                        W + L.o_list, L.nreb, B->amr_list.p, B->amr_count.p, L.par, B->st);
+    const hipError_t he = launch_split(B, L.lb.size(), 0, [&](size_t i, hipStream_t s) {
+      return launch_batch_nbh(B->dF.p, W + L.lb[i].o, (uint32_t)L.lb[i].n, L.lb[i].T, L.lb[i].lds, L.par, s, set_lds);
+    });
+    if (he != hipSuccess) return he;
+  }
+  launch_batch_ghost_update_r(B->dF.p, W + L.o_g, (uint32_t)L.n_g, B->dclock.p, B->st);
+  const hipError_t he = launch_force_classes(B, W, L.lf, set_lds, B->dclock.p);
+  if (he != hipSuccess) return he;
+  launch_batch_advance(B->dclock.p, B->st);
+  return hipGetLastError();
+}
+
+void retire_graph(xnb_batch* B, StepGraph& G) {
+  if (G.exec) B->retired.push_back(G.exec);
+  G.exec = nullptr;
+  G.tab.clear();
+  G.have_last = false;
+}
+
+// Capture the step of table T laid out as L into a new graph of G.  Its table goes to the
+// buffer of G not used by G's current graph: graphs launched earlier that read that buffer run
+// before the copy (stream order).
+int capture_step(xnb_batch* B, StepGraph& G, const Tables& T, const StepLayout& L) {
+  const int j = G.cur ^ 1;
+  const size_t n = T.v.size();
   int e;
   if ((e = ensure_staging(B, n))) return e;
-  if (B->dgwork[j].n < n) {
+  if (G.buf[j].n < n) {
     BHIP(B, hipStreamSynchronize(B->st));
     destroy_retired(B);
-    if (B->dgwork[j].p) (void)hipFree(B->dgwork[j].p);
-    B->dgwork[j].p = nullptr;
-    B->dgwork[j].n = 0;
-    BHIP(B, hipMalloc((void**)&B->dgwork[j].p, B->hwork_cap * sizeof(uint32_t)));
-    B->dgwork[j].n = B->hwork_cap;
+    if (G.buf[j].p) (void)hipFree(G.buf[j].p);
+    G.buf[j].p = nullptr;
+    G.buf[j].n = 0;
+    BHIP(B, hipMalloc((void**)&G.buf[j].p, B->hwork_cap * sizeof(uint32_t)));
+    G.buf[j].n = B->hwork_cap;
   }
-  if ((e = stage_copy(B, T2, B->dgwork[j].p))) return e;
-  for (int kind : {kBatchLjFp64, kBatchLjMixed}) {  // attributes set outside the capture
+  if ((e = stage_copy(B, T, G.buf[j].p))) return e;
+  for (int kind : {kBatchLjFp64, kBatchLjMixed, kBatchNbh}) {  // attributes set outside the capture
     uint32_t lds = 0;
-    for (const BrickLaunch& L : lf)
-      if (L.kind == kind) lds = std::max(lds, L.lds);
+    for (const auto* v : {&L.lf, &L.lb})
+      for (const BrickLaunch& x : *v)
+        if (x.kind == kind) lds = std::max(lds, x.lds);
     if (lds) BHIP(B, batch_set_lds(kind, lds));
   }
-  const uint32_t* W = B->dgwork[j].p;
   BHIP(B, hipStreamBeginCapture(B->st, hipStreamCaptureModeRelaxed));
-  launch_batch_ghost_update_r(B->dF.p, W + o_g, (uint32_t)n_g, B->dclock.p, B->st);
-  const hipError_t le = launch_force_classes(B, W, lf, false);
-  launch_batch_advance(B->dclock.p, B->st);
+  const hipError_t le = launch_step(B, G.buf[j].p, L, false);
   hipGraph_t g = nullptr;
   const hipError_t ce = hipStreamEndCapture(B->st, &g);
   if (le != hipSuccess || ce != hipSuccess || !g) {
     if (g) (void)hipGraphDestroy(g);
-    return bfail(B, XNB_E_HIP, std::string("quiet-step capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+    return bfail(B, XNB_E_HIP, std::string("step capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
   }
   hipGraphExec_t x = nullptr;
   const hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
-  if (ie != hipSuccess) return bfail(B, XNB_E_HIP, std::string("quiet-step graph: ") + hipGetErrorString(ie));
-  if (B->gexec) B->retired.push_back(B->gexec);
-  B->gexec = x;
-  B->gcur = j;
-  B->gtab = T2.v;
-  B->g_og = o_g;
-  B->g_ng = n_g;
-  B->g_lf = lf;
+  if (ie != hipSuccess) return bfail(B, XNB_E_HIP, std::string("step graph: ") + hipGetErrorString(ie));
+  if (G.exec) B->retired.push_back(G.exec);
+  G.exec = x;
+  G.cur = j;
+  G.tab = T.v;
+  G.lay = L;
   B->n_captures++;
   return XNB_OK;
 }
@@ -445,7 +503,11 @@ int recover_box(xnb_batch* B, int i, const xnb_run_params& p) {
   if (c->bp.nbricks > kWorkBrickMask + 1) return fail(c, XNB_E_CAPACITY, "box has too many bricks for a batch");
   BHIP(B, hipStreamSynchronize(c->st));
   frame_from_ctx(B, i, p);
-  return upload_frame(B, i);
+  int e2;
+  if ((e2 = upload_frame(B, i))) return e2;
+  // the overflow the batched rebuild left in the status words no longer holds
+  BHIP(B, hipMemsetAsync(B->dwords.p + 4 * (size_t)i, 0, 4 * sizeof(uint32_t), B->st));
+  return XNB_OK;
 }
 
 }  // namespace
@@ -466,8 +528,9 @@ int xnb_batch_create(xnb_batch** out, int device) {
     B->lds_limit = (uint32_t)lds;
   for (int k = 0; k < 3; k++) B->kinfo[k] = batch_kernel_info(k);
   bool ok = hipStreamCreate(&B->st) == hipSuccess && hipStreamCreate(&B->st2) == hipSuccess;
-  ok = ok && hipEventCreateWithFlags(&B->fork_ev, hipEventDisableTiming) == hipSuccess &&
-       hipEventCreateWithFlags(&B->join_ev, hipEventDisableTiming) == hipSuccess;
+  for (int k = 0; k < 2; k++)
+    ok = ok && hipEventCreateWithFlags(&B->fork_ev[k], hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&B->join_ev[k], hipEventDisableTiming) == hipSuccess;
   ok = ok && hipMalloc((void**)&B->dF.p, kMaxBatchBoxes * sizeof(BoxFrame)) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->hF, kMaxBatchBoxes * sizeof(BoxFrame), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipMalloc((void**)&B->dwords.p, 4 * kMaxBatchBoxes * sizeof(uint32_t)) == hipSuccess;
@@ -475,11 +538,12 @@ int xnb_batch_create(xnb_batch** out, int device) {
   ok = ok && hipMalloc((void**)&B->dtrig.p, kMaxBatchBoxes * sizeof(uint32_t)) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&B->htrig, kMaxBatchBoxes * sizeof(uint32_t), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipEventCreate(&B->hev[0]) == hipSuccess && hipEventCreate(&B->hev[1]) == hipSuccess;
-  ok = ok && hipMalloc((void**)&B->dclock.p, 2 * sizeof(uint32_t)) == hipSuccess;
-  ok = ok && hipMalloc((void**)&B->amr_count.p, sizeof(uint32_t)) == hipSuccess;
+  ok = ok && hipMalloc((void**)&B->dclock.p, 4 * sizeof(uint32_t)) == hipSuccess;
+  ok = ok && hipMalloc((void**)&B->amr_count.p, 4 * sizeof(uint32_t)) == hipSuccess;
   if (ok) {
-    B->dclock.n = 2;
-    const uint32_t clock0[2] = {B->par, B->gstep};
+    B->dclock.n = 4;
+    B->amr_count.n = 4;
+    const uint32_t clock0[4] = {B->par, B->gstep, B->par, B->gstep};
     ok = hipMemcpy(B->dclock.p, clock0, sizeof(clock0), hipMemcpyHostToDevice) == hipSuccess;
   }
   if (const char* s = std::getenv("XNB_BATCH_GRAPH")) B->use_graph = std::atoi(s) != 0;
@@ -530,17 +594,20 @@ int xnb_batch_destroy(xnb_batch* B) {
   if (B->htrig) (void)hipHostFree(B->htrig);
   if (B->dwork.p) (void)hipFree(B->dwork.p);
   destroy_retired(B);
-  if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+  for (StepGraph* G : {&B->gq, &B->gr[0], &B->gr[1]}) {
+    if (G->exec) (void)hipGraphExecDestroy(G->exec);
+    for (int k = 0; k < 2; k++)
+      if (G->buf[k].p) (void)hipFree(G->buf[k].p);
+  }
   if (B->dclock.p) (void)hipFree(B->dclock.p);
   if (B->amr_count.p) (void)hipFree(B->amr_count.p);
   if (B->amr_list.p) (void)hipFree(B->amr_list.p);
   for (int k = 0; k < 2; k++) {
-    if (B->dgwork[k].p) (void)hipFree(B->dgwork[k].p);
     if (B->hwork[k]) (void)hipHostFree(B->hwork[k]);
     if (B->hev[k]) (void)hipEventDestroy(B->hev[k]);
+    if (B->fork_ev[k]) (void)hipEventDestroy(B->fork_ev[k]);
+    if (B->join_ev[k]) (void)hipEventDestroy(B->join_ev[k]);
   }
-  if (B->fork_ev) (void)hipEventDestroy(B->fork_ev);
-  if (B->join_ev) (void)hipEventDestroy(B->join_ev);
   if (B->st2) (void)hipStreamDestroy(B->st2);
   if (B->st) (void)hipStreamDestroy(B->st);
   delete B;
@@ -586,6 +653,8 @@ int xnb_batch_attach(xnb_batch* B, xnb_ctx* c, const xnb_run_params* p, int* slo
   }
   bb.counts.n = (size_t)p->steps;
   BHIP(B, hipMemsetAsync(bb.counts.p, 0, (size_t)p->steps * sizeof(unsigned long long), B->st));
+  // the slot's status words: no overflow (a previous box may have left flags)
+  BHIP(B, hipMemsetAsync(B->dwords.p + 4 * (size_t)i, 0, 4 * sizeof(uint32_t), B->st));
   bb.p = *p;
   c->last_rcut = p->rcut;
   frame_from_ctx(B, i, *p);
@@ -638,33 +707,29 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
       if (trigger) reb.push_back(i);
       else quiet.push_back(i);
     }
-    Tables T1;
-    std::vector<uint32_t> nN, nC, nGh;
-    size_t o_rs = 0, n_rs = 0, o_rc = 0, n_rc = 0, o_rg = 0, n_rg = 0, o_list = 0;
-    std::vector<BrickLaunch> lb;
+    // 2. One work table for the rest of the step: the rebuilds of the boxes that rebuild, the
+    // ghost refresh of the others, the force of every box (launch_step).
+    Tables T;
+    StepLayout L;
     if (!reb.empty()) {
-      nN.clear();
+      std::vector<uint32_t> nN, nC, nGh;
       for (int i : reb) {
         nN.push_back(B->box[i].f.N);
         nC.push_back(B->box[i].f.g.ncells);
         nGh.push_back(ghost_cell_count(B->box[i].f.g) * 64u);  // one wave per ghost cell
       }
-      o_rs = T1.add(blocks_table(reb, nN));
-      n_rs = T1.v.size() - o_rs;
-      o_rc = T1.add(blocks_table(reb, nC));
-      n_rc = T1.v.size() - o_rc;
-      o_rg = T1.add(blocks_table(reb, nGh));
-      n_rg = T1.v.size() - o_rg;
-      brick_launches(B, T1, kBatchNbh, reb, std::vector<uint32_t>(reb.size(), 0u), lb);
-      std::vector<uint32_t> lst(reb.begin(), reb.end());
-      o_list = T1.add(lst);
-    }
-    if ((e = upload_tables(B, T1))) return e;
-    const uint32_t* W = B->dwork.p;
-    // 2. rebuilds (move_particles, rebuild_amr, backup_r, ghosts, chunk_neighbors), one launch each
-    if (!reb.empty()) {
-      launch_batch_zero_words(B->dwords.p, W + o_list, (uint32_t)reb.size(), B->st);
-      launch_batch_move(B->dF.p, W + o_rs, (uint32_t)n_rs, W + o_rc, (uint32_t)n_rc, W + o_list, (uint32_t)reb.size(), par, B->st);
+      L.par = par;
+      L.nreb = (uint32_t)reb.size();
+      L.o_rs = T.add(blocks_table(reb, nN));
+      L.n_rs = T.v.size() - L.o_rs;
+      L.o_rc = T.add(blocks_table(reb, nC));
+      L.n_rc = T.v.size() - L.o_rc;
+      L.o_rg = T.add(blocks_table(reb, nGh));
+      L.n_rg = T.v.size() - L.o_rg;
+      brick_launches(B, T, kBatchNbh, reb, std::vector<uint32_t>(reb.size(), 0u), L.lb);
+      L.o_list = T.add(std::vector<uint32_t>(reb.begin(), reb.end()));
+      L.o_zero = T.add(std::vector<uint32_t>{0u});
+      // the move swaps each rebuilding box's buffer roles on the device; the host mirror follows
       for (int i : reb) {
         B->box[i].f.ab ^= 1u;
         B->box[i].f.rcur ^= 1u;
@@ -673,22 +738,82 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
       size_t cells = 0;
       for (uint32_t x : nC) cells += x;
       if (B->amr_list.n < cells) {
-        if (B->amr_list.p) (void)hipFree(B->amr_list.p);  // waits for work still using it
+        retire_graph(B, B->gr[0]);  // captured with the old list
+        retire_graph(B, B->gr[1]);
+        BHIP(B, hipStreamSynchronize(B->st));
+        destroy_retired(B);
+        if (B->amr_list.p) (void)hipFree(B->amr_list.p);
         B->amr_list.p = nullptr;
         B->amr_list.n = 0;
         BHIP(B, hipMalloc((void**)&B->amr_list.p, cells * sizeof(uint32_t)));
         B->amr_list.n = cells;
       }
-      BHIP(B, hipMemsetAsync(B->amr_count.p, 0, sizeof(uint32_t), B->st));
-      launch_batch_ghosts(B->dF.p, W + o_rs, (uint32_t)n_rs, W + o_rc, (uint32_t)n_rc, W + o_rg, (uint32_t)n_rg, W + o_list,
-                          (uint32_t)reb.size(), B->amr_list.p, B->amr_count.p, par, B->st);
-      const hipError_t he = launch_split(B, lb.size(), [&](size_t i, hipStream_t s) {
-        return launch_batch_nbh(B->dF.p, W + lb[i].o, (uint32_t)lb[i].n, lb[i].T, lb[i].lds, par, s);
-      });
-      if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched neighbour build: ") + hipGetErrorString(he));
+    }
+    // Ghost refresh blocks of a box cover 10 % more than the largest ghost count it has had
+    // (the kernel stops at the current count), so the table stays the same over rebuilds that
+    // move ghost counts a little.
+    std::vector<int> q2;
+    std::vector<uint32_t> nG;
+    for (int i : quiet) {
+      BatchBox& bb = B->box[i];
+      if (bb.f.G > bb.ghost_span) bb.ghost_span = bb.f.G + bb.f.G / 10 + 1;
+      if (bb.ghost_span) {
+        q2.push_back(i);
+        nG.push_back(bb.ghost_span);
+      }
+    }
+    L.o_g = T.add(blocks_table(q2, nG));
+    L.n_g = T.v.size() - L.o_g;
+    std::vector<int> frc;
+    std::vector<uint32_t> tag;
+    for (int i : act) {
+      BatchBox& bb = B->box[i];
+      if (bb.failed) continue;
+      const int64_t k = bb.done + 1;
+      // a fused step stores no forces or energies: only the last step's are read (see run_loop)
+      const bool fuse = k < bb.steps;
+      frc.push_back(i);
+      tag.push_back(fuse ? (uint32_t)EPI_NEXT : (uint32_t)EPI_KICK);
+    }
+    for (int kind : {kBatchLjMixed, kBatchLjFp64}) brick_launches(B, T, kind, frc, tag, L.lf);
+    // 3. Launch it, from a captured graph when a step of the same kind (quiet, or rebuilding
+    // at fixed intervals) had the same table and layout before.
+    const bool plain = fresh.empty() && disp.empty() && !T.v.empty();
+    StepGraph& G = reb.empty() ? B->gq : B->gr[par & 1u];
+    bool graphed = false;
+    if (plain && B->use_graph) {
+      auto same = [&]() { return G.exec && T.v == G.tab && L == G.lay; };
+      if (!same() && G.have_last && T.v == G.last_tab && L == G.last_lay && (e = capture_step(B, G, T, L))) return e;
+      if (same()) {
+        BHIP(B, hipGraphLaunch(G.exec, B->st));
+        graphed = true;
+        B->n_graph_steps++;
+      }
+      G.last_tab = T.v;
+      G.last_lay = L;
+      G.have_last = true;
+      if (B->retired.size() > 4) {
+        BHIP(B, hipStreamSynchronize(B->st));
+        destroy_retired(B);
+      }
+    }
+    // quiet graphs are captured from two quiet steps in a row
+    if (!plain || !reb.empty()) B->gq.have_last = false;
+    if (!graphed) {
+      if ((e = upload_tables(B, T))) return e;
+      const hipError_t he = launch_step(B, B->dwork.p, L, true);
+      if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched step: ") + hipGetErrorString(he));
+    }
+    // 4. After a rebuild step: the status words of the rebuilt boxes (ghost count, longest
+    // list, overflow and failure flags).  A box whose rebuild overflowed was left untouched by
+    // the force kernel; it is redone with the one-box operators and given its force at this
+    // step's clock.
+    if (!reb.empty()) {
       BHIP(B, hipMemcpyAsync(B->hwords, B->dwords.p, 4 * kMaxBatchBoxes * sizeof(uint32_t), hipMemcpyDeviceToHost, B->st));
       BHIP(B, hipStreamSynchronize(B->st));
       destroy_retired(B);
+      std::vector<int> late;
+      std::vector<uint32_t> late_tag;
       for (int i : reb) {
         BatchBox& bb = B->box[i];
         const uint32_t* w = B->hwords + 4 * (size_t)i;
@@ -712,72 +837,24 @@ int xnb_batch_step(xnb_batch* B, int nsteps, int* n_active) {
           if ((e = recover_box(B, i, bb.p))) {
             bb.failed = true;
             bb.err = std::string("rebuild recovery failed: ") + bb.c->err;
+            continue;
           }
+          late.push_back(i);
+          late_tag.push_back(bb.done + 1 < bb.steps ? (uint32_t)EPI_NEXT : (uint32_t)EPI_KICK);
         }
       }
-    }
-    // 3. ghost refresh of boxes that did not rebuild, then the force of every box, then the
-    // device clock; a quiet step replays a captured graph of these three launches.  Ghost
-    // refresh blocks of a box cover 10 % more than the largest ghost count it has had (the
-    // kernel stops at the current count), so the table stays the same over rebuilds that move
-    // ghost counts a little.
-    Tables T2;
-    std::vector<int> q2;
-    std::vector<uint32_t> nG;
-    for (int i : quiet) {
-      BatchBox& bb = B->box[i];
-      if (bb.f.G > bb.ghost_span) bb.ghost_span = bb.f.G + bb.f.G / 10 + 1;
-      if (bb.ghost_span) {
-        q2.push_back(i);
-        nG.push_back(bb.ghost_span);
+      if (!late.empty()) {
+        Tables TL;
+        std::vector<BrickLaunch> ll;
+        for (int kind : {kBatchLjMixed, kBatchLjFp64}) brick_launches(B, TL, kind, late, late_tag, ll);
+        if ((e = upload_tables(B, TL))) return e;
+        const hipError_t he = launch_force_classes(B, B->dwork.p, ll, true, B->dclock.p + 2);
+        if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched force after recovery: ") + hipGetErrorString(he));
       }
-    }
-    const size_t o_g = T2.add(blocks_table(q2, nG));
-    const size_t n_g = T2.v.size() - o_g;
-    std::vector<int> frc;
-    std::vector<uint32_t> tag;
-    for (int i : act) {
-      BatchBox& bb = B->box[i];
-      if (bb.failed) continue;
-      const int64_t k = bb.done + 1;
-      // a fused step stores no forces or energies: only the last step's are read (see run_loop)
-      const bool fuse = k < bb.steps;
-      frc.push_back(i);
-      tag.push_back(fuse ? (uint32_t)EPI_NEXT : (uint32_t)EPI_KICK);
-    }
-    std::vector<BrickLaunch> lf;
-    for (int kind : {kBatchLjMixed, kBatchLjFp64}) brick_launches(B, T2, kind, frc, tag, lf);
-    const bool quiet_step = fresh.empty() && disp.empty() && reb.empty() && !T2.v.empty();
-    bool graphed = false;
-    if (quiet_step && B->use_graph) {
-      auto same = [&]() {
-        return B->gexec && T2.v == B->gtab && o_g == B->g_og && n_g == B->g_ng && lf == B->g_lf;
-      };
-      // capture a table seen on two quiet steps in a row (not one-off tables such as a box's last step)
-      if (!same() && T2.v == B->last_quiet && (e = capture_quiet(B, T2, o_g, n_g, lf))) return e;
-      if (same()) {
-        BHIP(B, hipGraphLaunch(B->gexec, B->st));
-        graphed = true;
-        B->n_graph_steps++;
-      }
-      B->last_quiet = T2.v;
-      if (B->retired.size() > 4) {
-        BHIP(B, hipStreamSynchronize(B->st));
-        destroy_retired(B);
-      }
-    } else {
-      B->last_quiet.clear();
-    }
-    if (!graphed) {
-      if ((e = upload_tables(B, T2))) return e;
-      W = B->dwork.p;
-      launch_batch_ghost_update_r(B->dF.p, W + o_g, (uint32_t)n_g, B->dclock.p, B->st);
-      const hipError_t he = launch_force_classes(B, W, lf, true);
-      if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("batched force: ") + hipGetErrorString(he));
-      launch_batch_advance(B->dclock.p, B->st);
     }
     for (size_t q = 0; q < frc.size(); q++) {
       BatchBox& bb = B->box[frc[q]];
+      if (bb.failed) continue;  // failed in this step's rebuild: its force did not run
       bb.done++;
       if (tag[q] == (uint32_t)EPI_KICK) {
         bb.finished = true;

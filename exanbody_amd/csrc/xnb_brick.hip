@@ -1441,6 +1441,11 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) 
   const uint32_t epi = work_entry_epi(b);
   b = work_entry_brick(b);
   const BoxFrame& f = F[box];
+  // A rebuild in this step that failed or outgrew the box's frame (status bits, a list past
+  // its capacity, an atom outside a non-periodic axis) leaves the box untouched: the host
+  // reads the words after the step and redoes it with the one-box operators (or fails it).
+  // The words are cleared at attach, before each batched rebuild and after a recovery.
+  if (f.words[0] != 0u || f.words[1] > f.bp.cap || f.words[2] != 0u) return;
   if (epi == EPI_NEXT) lj_batch_case<MIXED, EPI_NEXT>(f, b, par, gstep);
   else if (epi == EPI_KICK) lj_batch_case<MIXED, EPI_KICK>(f, b, par, gstep);
 }
@@ -1454,10 +1459,12 @@ static const void* batch_kernel(int kind) {
 }
 
 hipError_t launch_batch_nbh(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds, uint32_t par,
-                            hipStream_t st) {
+                            hipStream_t st, bool set) {
   if (!nwork) return hipSuccess;
-  hipError_t e = set_lds((const void*)kb_nbh_brick, lds);
-  if (e != hipSuccess) return e;
+  if (set) {
+    hipError_t e = set_lds((const void*)kb_nbh_brick, lds);
+    if (e != hipSuccess) return e;
+  }
   kb_nbh_brick<<<nwork, T, lds, st>>>(F, work, par);
   return hipGetLastError();
 }

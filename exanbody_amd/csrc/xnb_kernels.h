@@ -126,13 +126,14 @@ void launch_batch_move(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, con
 void launch_batch_ghosts(BoxFrame* F, const uint32_t* wslots, uint32_t nslots, const uint32_t* wcells, uint32_t ncells,
                          const uint32_t* wghost, uint32_t nghost, const uint32_t* boxes, uint32_t nboxes, uint32_t* amr_list,
                          uint32_t* amr_count, uint32_t par, hipStream_t st);
-// clock: device words {parity, global step} of the step being run (kb_advance moves them on)
+// clock: device words {parity, global step} of the step being run, then those of the step run
+// before it (kb_advance moves them on)
 void launch_batch_ghost_update_r(const BoxFrame* F, const uint32_t* work, uint32_t nwork, const uint32_t* clock, hipStream_t st);
 void launch_batch_advance(uint32_t* clock, hipStream_t st);
 void launch_batch_first_half(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t par, uint32_t gstep, hipStream_t st);
 void launch_batch_trigger(const BoxFrame* F, const uint32_t* boxes, uint32_t n, uint32_t gstep, uint32_t* out, hipStream_t st);
 hipError_t launch_batch_nbh(const BoxFrame* F, const uint32_t* work, uint32_t nwork, uint32_t T, uint32_t lds, uint32_t par,
-                            hipStream_t st);
+                            hipStream_t st, bool set_lds = true);
 // batched brick kernels: the force per precision and the neighbour build
 enum { kBatchLjFp64 = 0, kBatchLjMixed = 1, kBatchNbh = 2 };
 struct BatchKernelInfo {

@@ -1522,6 +1522,8 @@ void launch_batch_ghost_update_r(const BoxFrame* F, const uint32_t* work, uint32
 // end of a batched step: parity flips, global step + 1 (one thread)
 __global__ void kb_advance(uint32_t* __restrict__ clock) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    clock[2] = clock[0];  // the step just run, for force launches the host adds after it
+    clock[3] = clock[1];
     clock[0] ^= 1u;
     clock[1] += 1u;
   }

@@ -276,9 +276,11 @@ int xnb_batch_box_status(xnb_batch* batch, int slot, int64_t* done, int* state);
 /* Rebuilds of the box that outgrew the sizes it was attached with and were redone by the
  * one-box operators (which re-size its arrays). */
 int xnb_batch_box_recoveries(xnb_batch* batch, int slot, int64_t* n);
-/* Steps replayed from a captured HIP graph (quiet steps: no box starting, rebuilding or
- * rebuilt on displacement) and graphs captured so far.  XNB_BATCH_GRAPH=0 in the environment
- * at xnb_batch_create launches every step eagerly. */
+/* Steps replayed from a captured HIP graph and graphs captured so far.  A step with no box
+ * starting or rebuilt on displacement is one launch sequence (rebuilds at fixed intervals,
+ * ghost refresh, force, clock); quiet steps and rebuild steps each replay their own graph once
+ * the same sequence has come twice.  XNB_BATCH_GRAPH=0 in the environment at
+ * xnb_batch_create launches every step eagerly. */
 int xnb_batch_graph_stats(xnb_batch* batch, int64_t* graph_steps, int64_t* captures);
 /* Detach the box of a slot; its xnb_ctx then holds the state xnb_run would have produced
  * (for a finished box) and p->n_rebuilds / n_violations are filled as by xnb_run.  A box

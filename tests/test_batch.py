@@ -142,6 +142,42 @@ def test_batch_quiet_steps_from_graph(monkeypatch, graph):
     batch.close()
 
 
+def test_batch_rebuild_steps_from_graph(monkeypatch):
+    """Boxes that all rebuild every 5 steps: a rebuild step (move, rebuild_amr, backup, ghosts,
+    neighbour build, force, clock) is one launch sequence, captured at its second occurrence
+    with the same step parity and replayed at the later ones, beside the quiet-step graph.  Of
+    the 31 steps, step 1 starts the boxes and step 31 closes them (eager); the quiet steps give
+    at most 22 replays (steps 3-4, then four after each rebuild); the rebuild steps alternate
+    parity, so 15 and 20 are captured and replayed and 25 and 30 replayed: four more.  Every
+    box ends bitwise equal to the same box run alone."""
+    monkeypatch.setenv("XNB_BATCH_GRAPH", "1")
+    jobs = [
+        (6, 0.84, 0.72, 31, 31, PRECISION_MIXED, 5),
+        (5, 0.80, 1.00, 32, 31, PRECISION_FP64, 5),
+        (7, 0.75, 0.80, 33, 31, PRECISION_MIXED, 5),
+        (4, 0.85, 0.60, 34, 31, PRECISION_FP64, 5),
+    ]
+    ref = [_alone(j) for j in jobs]
+    batch = Batch()
+    boxes, slots = [], []
+    for j in jobs:
+        b = _box(j)
+        boxes.append(b)
+        slots.append(batch.attach(b, j[4], DT, 1.0, 1.0, RCUT, SKIN, precision=j[5], rebuild_every=j[6]))
+    while batch.step(4):
+        pass
+    replays, captures = batch.graph_stats()
+    rec = [batch.recoveries(s) for s in slots]
+    if sum(rec) == 0:  # a recovery changes the box's tables, and with them what is replayed
+        assert replays >= 25 and captures >= 2, (replays, captures)
+    for q, (j, b, s) in enumerate(zip(jobs, boxes, slots)):
+        assert batch.status(s) == (j[4], 1)
+        assert batch.detach(s) == ref[q][1], f"job {q}: rebuild / violation counts"
+        _same(b.export(), ref[q][0], f"job {q}")
+        b.close()
+    batch.close()
+
+
 def test_batch_recovery_path(monkeypatch):
     """With no head room (XNB_BATCH_MARGINS=1,1) rebuilds that grow a brick, a region or the
     ghosts overflow the sizes a box was attached with, and the batch redoes them with the
