@@ -266,6 +266,11 @@ def batched_bench(n_boxes, steps, alone_sample, drift_sample, seed=5):
     setup = time.perf_counter() - t
     batch = Batch()
     slots = [batch.attach(b, steps, DT, 1.0, 1.0, RCUT, SKIN, precision=PRECISION_MIXED, rebuild_every=EVERY) for b in boxes]
+    # workgroup shapes: the batch autotuner times the batched build and force under each
+    # candidate brick LDS cap before the first step (not in the timed region)
+    t = time.perf_counter()
+    lds_frac = batch.autotune()
+    tune_s = time.perf_counter() - t
     boxes[0].synchronize()
     t0 = time.perf_counter()
     left = len(slots)
@@ -305,6 +310,7 @@ def batched_bench(n_boxes, steps, alone_sample, drift_sample, seed=5):
                          f"rho* U[0.70, 0.90], T* U[0.5, 1.5], {steps} steps each, mixed, rebuild every {EVERY}",
                 atom_steps_per_s=batched, ms_per_step=1e3 * wall / steps, wall_s=wall, setup_s=setup,
                 recoveries=int(recoveries), graph_steps=int(graph_steps), graph_captures=int(captures),
+                autotuned_brick_lds_fraction=lds_frac, autotune_s=tune_s,
                 one_at_a_time_atom_steps_per_s=alone, one_at_a_time_sample=int(len(pick)),
                 speedup_vs_one_at_a_time=batched / alone,
                 force_error_end=dict(err, note="mixed vs FP64 force on every box's final positions (all boxes pooled)"),

@@ -105,6 +105,7 @@ _SIGS = {
     "xnb_batch_box_status": (C.c_int, [P, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "xnb_batch_box_recoveries": (C.c_int, [P, C.c_int, C.POINTER(C.c_int64)]),
     "xnb_batch_graph_stats": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "xnb_batch_autotune": (C.c_int, [P, C.POINTER(C.c_double)]),
     "xnb_batch_detach": (C.c_int, [P, C.c_int, C.POINTER(RunParams)]),
     "xnb_job_bytes": (C.c_size_t, [C.c_size_t]),
     "xnb_job_bytes_for": (C.c_size_t, [P]),

@@ -52,6 +52,14 @@ class Batch:
         self._check(self.L.xnb_batch_graph_stats(self._h, C.byref(a), C.byref(b)), "graph_stats")
         return a.value, b.value
 
+    def autotune(self):
+        """Choose the batch's brick LDS cap (and with it every box's brick shape) by timing the
+        batched build and force of the attached boxes, before their first step; returns the
+        kept fraction of the CU's LDS."""
+        f = C.c_double()
+        self._check(self.L.xnb_batch_autotune(self._h, C.byref(f)), "autotune")
+        return f.value
+
     def detach(self, slot):
         """Detach a box; returns dict(rebuilds, violations) like Box.run."""
         p = _lib.RunParams()

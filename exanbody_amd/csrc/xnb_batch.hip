@@ -124,6 +124,7 @@ struct xnb_batch {
   // workgroups share a CU (512-box batch, 100 steps: 3.15e9 -> 3.54e9 atom-steps/s against no
   // cap; a third of the CU is slower again); XNB_BATCH_LDS="fraction" at create
   uint32_t brick_lds_cap = 0;
+  double lds_frac = 0.5;  // brick_lds_cap as a fraction of the CU's LDS (xnb_batch_autotune may change it)
   // brick work launched per class of boxes (XNB_BATCH_CLASSES=0: one launch sized for the
   // widest box)
   bool use_classes = true;
@@ -468,6 +469,14 @@ int capture_step(xnb_batch* B, StepGraph& G, const Tables& T, const StepLayout& 
 
 bool active(const BatchBox& bb) { return bb.used && !bb.finished && !bb.failed; }
 
+// LDS bytes a brick workgroup may take for a cap of `frac` of the CU's LDS, less the
+// kernels' static LDS
+uint32_t lds_cap_for(const xnb_batch* B, double frac) {
+  uint32_t st_lds = 0;
+  for (int k = 0; k < 3; k++) st_lds = std::max(st_lds, B->kinfo[k].static_lds);
+  return (uint32_t)(frac * B->lds_limit) - std::min<uint32_t>(st_lds + 64u, B->lds_limit / 8);
+}
+
 // chunk_neighbors of a box for the batch: brick head room and the batch's LDS cap per brick
 // workgroup, or the context's own limit when no brick fits the cap (restored afterwards)
 int batch_box_nbh(xnb_batch* B, xnb_ctx* c) {
@@ -554,9 +563,8 @@ int xnb_batch_create(xnb_batch** out, int device) {
     const double f = std::atof(s);
     if (f > 0.0 && f <= 1.0) lds_frac = f;
   }
-  uint32_t st_lds = 0;
-  for (int k = 0; k < 3; k++) st_lds = std::max(st_lds, B->kinfo[k].static_lds);
-  B->brick_lds_cap = (uint32_t)(lds_frac * B->lds_limit) - std::min<uint32_t>(st_lds + 64u, B->lds_limit / 8);
+  B->lds_frac = lds_frac;
+  B->brick_lds_cap = lds_cap_for(B, lds_frac);
   if (const char* s = std::getenv("XNB_BATCH_MARGINS")) {
     double bm = 0, sm = 0;
     if (std::sscanf(s, "%lf,%lf", &bm, &sm) != 2 || !(bm >= 1.0 && bm <= 4.0) || !(sm >= 1.0 && sm <= 4.0)) {
@@ -897,6 +905,114 @@ int xnb_batch_graph_stats(xnb_batch* B, int64_t* graph_steps, int64_t* captures)
   if (!B) return XNB_E_ARG;
   if (graph_steps) *graph_steps = B->n_graph_steps;
   if (captures) *captures = B->n_captures;
+  return XNB_OK;
+}
+
+// Batch autotuner.  The LDS bytes a brick workgroup may take decide each box's brick shape
+// (cells per workgroup, lanes per workgroup) and how many of its workgroups share a CU.  For
+// each candidate fraction of the CU's LDS, every attached box is laid out again under that
+// cap (its lists rebuilt: lists and forces do not depend on the brick shape) and the batched
+// neighbour build and force (force and energy stored, from the same positions, so the same
+// values) are timed; the fraction with the least force time plus a tenth of the build time
+// (a rebuild every 10 steps) is kept for the boxes and for later attaches and recoveries.
+// Every attached box must be before its first step.
+int xnb_batch_autotune(xnb_batch* B, double* lds_frac_out) {
+  if (!B) return XNB_E_ARG;
+  BHIP(B, hipSetDevice(B->device));
+  std::vector<int> boxes;
+  for (int i = 0; i < (int)kMaxBatchBoxes; i++) {
+    const BatchBox& bb = B->box[i];
+    if (!bb.used) continue;
+    if (!bb.fresh || bb.done != 0 || bb.failed) return bfail(B, XNB_E_STATE, "autotune runs before the first step of every attached box");
+    boxes.push_back(i);
+  }
+  if (boxes.empty()) return bfail(B, XNB_E_STATE, "autotune needs attached boxes");
+  BHIP(B, hipStreamSynchronize(B->st));
+  for (StepGraph* G : {&B->gq, &B->gr[0], &B->gr[1]}) retire_graph(B, *G);
+  destroy_retired(B);
+  int e;
+  // every box laid out again under the cap of `frac`
+  auto layout = [&](double frac) -> int {
+    B->lds_frac = frac;
+    B->brick_lds_cap = lds_cap_for(B, frac);
+    for (int i : boxes) {
+      BatchBox& bb = B->box[i];
+      xnb_ctx* c = bb.c;
+      ctx_from_frame(B, i, B->par);
+      int e2;
+      if ((e2 = batch_box_nbh(B, c))) return bfail(B, e2, std::string("autotune layout: ") + c->err);
+      if (c->list_kind != 1 || brick_lds_bytes(c->bp) > B->lds_limit || c->bp.nbricks > kWorkBrickMask + 1)
+        return bfail(B, XNB_E_CAPACITY, "autotune layout: a box left the batch's brick path");
+      BHIP(B, hipStreamSynchronize(c->st));
+      frame_from_ctx(B, i, bb.p);
+      if ((e2 = upload_frame(B, i))) return e2;
+    }
+    BHIP(B, hipMemsetAsync(B->dwords.p, 0, 4 * kMaxBatchBoxes * sizeof(uint32_t), B->st));
+    return XNB_OK;
+  };
+  struct Events {  // destroyed on every return path
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    ~Events() {
+      for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } E;
+  for (hipEvent_t& x : E.ev) BHIP(B, hipEventCreate(&x));
+  // force time + build time / 10 of the current layout, in ms per step; < 0 when a build overflowed
+  auto score = [&](double& ms) -> int {
+    Tables T;
+    std::vector<BrickLaunch> lb, lf;
+    brick_launches(B, T, kBatchNbh, boxes, std::vector<uint32_t>(boxes.size(), 0u), lb);
+    for (int kind : {kBatchLjMixed, kBatchLjFp64})
+      brick_launches(B, T, kind, boxes, std::vector<uint32_t>(boxes.size(), (uint32_t)EPI_NONE), lf);
+    const size_t o_list = T.add(std::vector<uint32_t>(boxes.begin(), boxes.end()));
+    int e2;
+    if ((e2 = upload_tables(B, T))) return e2;
+    const uint32_t* W = B->dwork.p;
+    const uint32_t par = B->par;
+    auto build = [&]() {
+      launch_batch_zero_words(B->dwords.p, W + o_list, (uint32_t)boxes.size(), B->st);
+      return launch_split(B, lb.size(), 0, [&](size_t i, hipStream_t st) {
+        return launch_batch_nbh(B->dF.p, W + lb[i].o, (uint32_t)lb[i].n, lb[i].T, lb[i].lds, par, st, true);
+      });
+    };
+    constexpr int kReps = 3;
+    hipError_t he = build();  // untimed round
+    if (he == hipSuccess) he = launch_force_classes(B, W, lf, true, B->dclock.p);
+    if (he == hipSuccess) he = hipEventRecord(E.ev[0], B->st);
+    for (int q = 0; q < kReps && he == hipSuccess; q++) he = build();
+    if (he == hipSuccess) he = hipEventRecord(E.ev[1], B->st);
+    for (int q = 0; q < kReps && he == hipSuccess; q++) he = launch_force_classes(B, W, lf, true, B->dclock.p);
+    if (he == hipSuccess) he = hipEventRecord(E.ev[2], B->st);
+    if (he != hipSuccess) return bfail(B, XNB_E_HIP, std::string("autotune launches: ") + hipGetErrorString(he));
+    BHIP(B, hipMemcpyAsync(B->hwords, B->dwords.p, 4 * kMaxBatchBoxes * sizeof(uint32_t), hipMemcpyDeviceToHost, B->st));
+    BHIP(B, hipStreamSynchronize(B->st));
+    float tb = 0.f, tf = 0.f;
+    BHIP(B, hipEventElapsedTime(&tb, E.ev[0], E.ev[1]));
+    BHIP(B, hipEventElapsedTime(&tf, E.ev[1], E.ev[2]));
+    ms = (tf + 0.1 * tb) / kReps;
+    for (int i : boxes) {
+      const uint32_t* w = B->hwords + 4 * (size_t)i;
+      if (w[0] || w[1] > B->box[i].f.bp.cap || w[2]) ms = -1.0;
+    }
+    return XNB_OK;
+  };
+  static const double cand[] = {1.0 / 3.0, 0.5, 0.75, 1.0};
+  double best = -1.0, best_ms = 0.0;
+  for (double frac : cand) {
+    double ms = 0.0;
+    if ((e = layout(frac)) || (e = score(ms))) return e;
+    if (ms >= 0.0 && (best < 0.0 || ms < best_ms)) {
+      best = frac;
+      best_ms = ms;
+    }
+  }
+  if (best < 0.0) best = 0.5;
+  // the boxes as they will run: laid out under the kept cap, lists, forces and status words
+  // those of that layout
+  if ((e = layout(best))) return e;
+  BHIP(B, hipStreamSynchronize(B->st));
+  if (lds_frac_out) *lds_frac_out = best;
   return XNB_OK;
 }
 

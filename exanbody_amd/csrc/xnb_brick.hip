@@ -1429,7 +1429,8 @@ __device__ __forceinline__ void lj_batch_case(const BoxFrame& f, uint32_t b, uin
 
 // Force of every brick of the listed boxes, all of one precision (one kernel per precision,
 // so the mixed form runs with its own, smaller register budget); the epilogue is per box
-// (uniform per workgroup).  EPI_KICK closes a box's run; EPI_NEXT fuses its next step.  The
+// (uniform per workgroup).  EPI_KICK closes a box's run; EPI_NEXT fuses its next step;
+// EPI_NONE stores the force and energy only.  The
 // register cap of 5 waves per SIMD (the FP64 form otherwise takes 110 VGPRs, 4 waves) measured
 // 3% faster on 256 FP64 boxes; the mixed form fits under it anyway.
 template <bool MIXED>
@@ -1448,6 +1449,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) 
   if (f.words[0] != 0u || f.words[1] > f.bp.cap || f.words[2] != 0u) return;
   if (epi == EPI_NEXT) lj_batch_case<MIXED, EPI_NEXT>(f, b, par, gstep);
   else if (epi == EPI_KICK) lj_batch_case<MIXED, EPI_KICK>(f, b, par, gstep);
+  else lj_batch_case<MIXED, EPI_NONE>(f, b, par, gstep);  // force and energy only (the batch autotuner's timings)
 }
 
 static const void* batch_kernel(int kind) {

@@ -282,6 +282,14 @@ int xnb_batch_box_recoveries(xnb_batch* batch, int slot, int64_t* n);
  * the same sequence has come twice.  XNB_BATCH_GRAPH=0 in the environment at
  * xnb_batch_create launches every step eagerly. */
 int xnb_batch_graph_stats(xnb_batch* batch, int64_t* graph_steps, int64_t* captures);
+/* Autotuner, before the first step of every attached box: the LDS a brick workgroup may take
+ * (as a fraction of the CU's: 1/3, 1/2, 3/4 or all of it) decides each box's brick shape
+ * (cells and lanes per workgroup) and how many of its workgroups share a CU.  Every box is
+ * laid out under each candidate and the batched neighbour build and force are timed; the
+ * fraction with the least force time plus a tenth of the build time is kept (*lds_frac) for
+ * the boxes and for later attaches and recoveries.  Results do not depend on it.  Default
+ * without a tuning: 1/2 (XNB_BATCH_LDS). */
+int xnb_batch_autotune(xnb_batch* batch, double* lds_frac);
 /* Detach the box of a slot; its xnb_ctx then holds the state xnb_run would have produced
  * (for a finished box) and p->n_rebuilds / n_violations are filled as by xnb_run.  A box
  * detached before its last step holds part of its next step: its lists and backup are marked

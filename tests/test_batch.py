@@ -178,6 +178,33 @@ def test_batch_rebuild_steps_from_graph(monkeypatch):
     batch.close()
 
 
+def test_batch_autotune_keeps_results():
+    """The batch autotuner lays every box out under each candidate LDS cap (rebuilding its lists
+    and recomputing its forces in place) and keeps the fastest; the boxes then run to the same
+    bitwise end states as alone, and tuning after a step is refused."""
+    jobs = [JOBS[0], JOBS[1], JOBS[3], JOBS[5]]
+    ref = [_alone(j) for j in jobs]
+    batch = Batch()
+    boxes, slots = [], []
+    for j in jobs:
+        b = _box(j)
+        boxes.append(b)
+        slots.append(batch.attach(b, j[4], DT, 1.0, 1.0, RCUT, SKIN, precision=j[5], rebuild_every=j[6]))
+    frac = batch.autotune()
+    assert any(abs(frac - c) < 1e-12 for c in (1.0 / 3.0, 0.5, 0.75, 1.0)), frac
+    batch.step(1)
+    with pytest.raises(Exception):
+        batch.autotune()
+    while batch.step(5):
+        pass
+    for q, (j, b, s) in enumerate(zip(jobs, boxes, slots)):
+        assert batch.status(s) == (j[4], 1)
+        assert batch.detach(s) == ref[q][1], f"job {q}: rebuild / violation counts"
+        _same(b.export(), ref[q][0], f"job {q} (autotuned LDS cap {frac})")
+        b.close()
+    batch.close()
+
+
 def test_batch_recovery_path(monkeypatch):
     """With no head room (XNB_BATCH_MARGINS=1,1) rebuilds that grow a brick, a region or the
     ghosts overflow the sizes a box was attached with, and the batch redoes them with the
