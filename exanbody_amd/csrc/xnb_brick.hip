@@ -94,7 +94,9 @@ static constexpr int kStageRound = XNB_STAGE_ROUND;
 // One group of four list entries (8 bytes), loaded past the caches: lists are read once per
 // force launch, so streaming them keeps L2 and the MALL for the staged halo positions that
 // neighbouring bricks read again (with the fused epilogue's streaming stores: C2 mixed force
-// 0.937 -> 0.921 ms, two runs each).
+// 0.937 -> 0.921 ms, slab multimat force 1.270 -> 1.256 ms mixed, 2.542 -> 2.516 ms FP64;
+// two runs each).  The neighbour build's list stores stay cached: streamed, its scattered
+// 8-byte stores took the C2 build from 2.5 to 6.9 ms.
 __device__ inline uint2 load_list_group(const uint2* p) {
   const unsigned long long q = __builtin_nontemporal_load(reinterpret_cast<const unsigned long long*>(p));
   return make_uint2((uint32_t)q, (uint32_t)(q >> 32));
@@ -1012,7 +1014,7 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
   uint2 wn[W];
   if (nmax > 0) {
 #pragma unroll
-    for (int w = 0; w < W; w++) wn[w] = lst[(size_t)w * T];
+    for (int w = 0; w < W; w++) wn[w] = load_list_group(lst + (size_t)w * T);
   }
   for (uint32_t q0 = 0; q0 < nmax; q0 += U) {
     uint2 wc[W];
@@ -1021,7 +1023,7 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
     if (q0 + U < nmax) {
       const uint2* nxt = lst + (size_t)((q0 + U) >> 2) * T;
 #pragma unroll
-      for (int w = 0; w < W; w++) wn[w] = nxt[(size_t)w * T];
+      for (int w = 0; w < W; w++) wn[w] = load_list_group(nxt + (size_t)w * T);
     }
     uint32_t j[U];
 #pragma unroll
@@ -1148,7 +1150,10 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
         p.x += ux * nx.dt + fx * nx.dt2;
         p.y += uy * nx.dt + fy * nx.dt2;
         p.z += uz * nx.dt + fz * nx.dt2;
-        nx.r_next[s] = p;
+        __builtin_nontemporal_store(p.x, &nx.r_next[s].x);
+        __builtin_nontemporal_store(p.y, &nx.r_next[s].y);
+        __builtin_nontemporal_store(p.z, &nx.r_next[s].z);
+        __builtin_nontemporal_store(p.w, &nx.r_next[s].w);
         ux += fx * dth;
         uy += fy * dth;
         uz += fz * dth;
@@ -1156,9 +1161,9 @@ __device__ __forceinline__ void ljmm_brick_body(const Geom& g, const BrickParams
                   rk = (int)(At.rc / ((uint32_t)bp.RX * (uint32_t)bp.RY));
         over = displaced_ijk(g, p, P.gi0 - 1 + ri, P.gj0 - 1 + rj, P.gk0 - 1 + rk, nx.backup, s, nx.thr2);
       }
-      A.vx[s] = ux;
-      A.vy[s] = uy;
-      A.vz[s] = uz;
+      __builtin_nontemporal_store(ux, &A.vx[s]);
+      __builtin_nontemporal_store(uy, &A.vy[s]);
+      __builtin_nontemporal_store(uz, &A.vz[s]);
     }
   }
   if (NEXT) wave_group_count_add(over, nx.count, L.wcnt);
